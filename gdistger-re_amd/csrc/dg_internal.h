@@ -1,0 +1,170 @@
+// dg_internal.h - context, error and timing plumbing shared by the library's
+// translation units.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/distger_hip.h"
+
+// Device-side walk queue and round statistics (one per context).
+struct dg_walk_queue {
+  unsigned long long next;    // next walk index to hand out (relative to walker_begin)
+  unsigned long long cursor;  // tokens reserved in the raw token buffer
+  unsigned int flags;         // DG_QFLAG_*
+  unsigned int max_len;       // longest walk
+  unsigned int zero_terms;    // RE terms with p_i == 0 or q_i == 0
+  unsigned int pad;
+  double re;                  // relative entropy of the round
+};
+
+#define DG_QFLAG_TOKEN_OVERFLOW 1u
+#define DG_QFLAG_LENGTH_LIMIT 2u
+
+struct dg_timing_entry {
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct dg_pending_event {
+  std::string name;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
+struct dg_sgns_state;  // defined by the trainer translation unit
+
+struct dg_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // graph (device-resident CSR; adjacency units are {neighbour, common-neighbour count})
+  uint32_t v_num = 0;
+  uint64_t m = 0;
+  uint64_t* d_row_ptr = nullptr;
+  uint2* d_adj = nullptr;
+  uint32_t* d_deg = nullptr;
+
+  // walker buffers
+  dg_walk_queue* d_queue = nullptr;
+  dg_walk_queue* h_queue = nullptr;  // pinned mirror
+  uint32_t* d_freq = nullptr;        // cumulative visit counts
+  uint64_t freq_total = 0;           // sum of d_freq
+  uint32_t* d_tok_raw = nullptr;     // tokens in completion order
+  size_t tok_raw_cap = 0;
+  uint64_t* d_walk_off = nullptr;    // per-walk offset into d_tok_raw
+  uint32_t* d_walk_len = nullptr;
+  size_t walk_cap = 0;
+  size_t walk_len_cap = 0;
+  uint32_t* d_tokens = nullptr;      // current round corpus, walk-id order
+  size_t tokens_cap = 0;
+  uint64_t* d_offsets = nullptr;     // n_walks + 1
+  size_t offsets_cap = 0;
+  uint32_t* d_spill = nullptr;       // per-lane overflow of the LDS path buffer
+  size_t spill_cap = 0;
+  uint64_t* d_scan_tmp = nullptr;    // block sums for scans
+  size_t scan_tmp_cap = 0;
+  double* d_re_terms = nullptr;      // v_num
+  double* d_re_chunks = nullptr;
+  size_t re_cap = 0;
+  uint64_t cur_walks = 0;
+  uint64_t cur_tokens = 0;
+  double prev_re = 0.0;
+  uint32_t rounds_done = 0;
+
+  // training corpus (concatenated rounds)
+  uint32_t* d_train_tokens = nullptr;
+  size_t train_tokens_cap = 0;
+  uint64_t* d_train_offsets = nullptr;
+  size_t train_offsets_cap = 0;
+  uint64_t train_walks = 0;
+  uint64_t train_tokens = 0;
+
+  dg_sgns_state* sgns = nullptr;
+
+  // timing
+  std::map<std::string, dg_timing_entry> timing;
+  std::vector<dg_pending_event> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+// Record an error message on the context and return `code`.
+int dg_fail(dg_ctx* ctx, int code, const char* fmt, ...);
+
+#define DG_HIP(ctx, expr)                                                            \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return dg_fail((ctx), DG_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                     hipGetErrorString(_e), __FILE__, __LINE__);                     \
+  } while (0)
+
+#define DG_CHECK(expr)            \
+  do {                            \
+    int _rc = (expr);             \
+    if (_rc != DG_OK) return _rc; \
+  } while (0)
+
+// Grow a device buffer to hold at least `need` elements (contents discarded).
+template <typename T>
+int dg_reserve(dg_ctx* ctx, T** ptr, size_t* cap, size_t need) {
+  if (*cap >= need && *ptr) return DG_OK;
+  if (*ptr) {
+    hipError_t e = hipFree(*ptr);
+    *ptr = nullptr;
+    *cap = 0;
+    if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "hipFree: %s", hipGetErrorString(e));
+  }
+  size_t n = need ? need : 1;
+  hipError_t e = hipMalloc((void**)ptr, n * sizeof(T));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return dg_fail(ctx, DG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
+                   hipGetErrorString(e));
+  }
+  *cap = n;
+  return DG_OK;
+}
+
+// Grow keeping the first `keep` elements.
+template <typename T>
+int dg_grow_keep(dg_ctx* ctx, T** ptr, size_t* cap, size_t need, size_t keep) {
+  if (*cap >= need && *ptr) return DG_OK;
+  size_t n = need > 2 * *cap ? need : 2 * *cap;
+  T* np = nullptr;
+  hipError_t e = hipMalloc((void**)&np, n * sizeof(T));
+  if (e != hipSuccess)
+    return dg_fail(ctx, DG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
+                   hipGetErrorString(e));
+  if (*ptr && keep) {
+    e = hipMemcpyAsync(np, *ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "sync: %s", hipGetErrorString(e));
+  }
+  if (*ptr) hipFree(*ptr);
+  *ptr = np;
+  *cap = n;
+  return DG_OK;
+}
+
+// Kernel timing with HIP events on the context stream.
+int dg_time_begin(dg_ctx* ctx, const char* name, size_t* token);
+int dg_time_end(dg_ctx* ctx, size_t token);
+int dg_time_resolve(dg_ctx* ctx);
+
+// Graph teardown (graph.hip) and trainer teardown (sgns.hip).
+void dg_graph_free(dg_ctx* ctx);
+void dg_walk_free(dg_ctx* ctx);
+void dg_sgns_free(dg_ctx* ctx);
+
+// Exclusive scan of n uint32 lengths into n+1 uint64 offsets (walker.hip).
+int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_off);

@@ -1,0 +1,808 @@
+// walker.hip - persistent information-oriented walker for gfx950, and the
+// per-round corpus packing and relative-entropy kernels.
+//
+// Reference: WalkEngine::internal_random_walk / internal_walk_epoch
+// (include/walk.hpp:608-782, :789-1043) with Correlation (:260-387), for one
+// partition.  PathCollector/PathSet (include/path.hpp) become a device token
+// buffer in walk-id order.
+//
+// Design (MI355X-first, not a translation of the OpenMP/MPI loop):
+//   * One lane per walk, 64 walks per wavefront, in one persistent launch.
+//     Walks have uneven lengths; a lane whose walk ends takes the next walk id
+//     from its wave's pool, and a wave refills its pool with one 64-bit atomic
+//     on the device work queue, so every lane stays busy until the queue
+//     drains and every wave then leaves the loop.
+//   * Each lane keeps its walk in LDS (kPathLds vertices, lane-major with an odd
+//     stride so lane-strided and cooperative accesses are bank-conflict free)
+//     and spills longer walks to a private global area.  The visit count f_i
+//     of walk.hpp:816 (an unordered_map per walk) is a scan of that buffer.
+//   * A finished walk is written out cooperatively by its wave: one atomic
+//     reserves space for all lanes that finished together, then the wave
+//     writes each walk with coalesced 64-lane stores.
+//   * Every draw comes from the walk's own Philox stream (dg_philox.h), and the
+//     FP64 entropy/acceptance math uses dg_math.h, so the corpus is the same
+//     bits as the CPU oracle's whatever the schedule.
+#include "dg_internal.h"
+#include "dg_math.h"
+#include "dg_philox.h"
+
+namespace {
+
+constexpr int kWalkBlock = 256;
+constexpr int kPathLds = 47;  // vertices per lane kept in LDS (odd)
+constexpr uint32_t kPool = 64;
+constexpr uint32_t kDefaultLengthLimit = 1024;
+
+struct WalkArgs {
+  const uint64_t* row_ptr;
+  const uint2* adj;
+  uint32_t v_num;
+  uint32_t round;
+  uint64_t seed;
+  uint32_t walker_begin;
+  uint32_t walker_num;
+  uint32_t min_length;
+  uint32_t max_length;
+  uint32_t length_limit;
+  uint32_t spill_per_slot;
+  double mu;
+  uint32_t* tokens;
+  uint64_t tok_cap;
+  uint64_t* walk_off;
+  uint32_t* walk_len;
+  uint32_t* spill;
+  dg_walk_queue* q;
+};
+
+struct WalkState {
+  double H, S, EH, ES, EHS, EHH, ESS, N, R;
+};
+
+// Correlation::iterate with step_k = 1 (include/walk.hpp:284-354): the
+// incremental Shannon entropy H of the walk's visit histogram after appending
+// a vertex seen fi times before, running means of H, S, HS, HH, SS and the
+// Pearson correlation R of (H, S).  Same operation order as the reference.
+__device__ __forceinline__ void state_iterate(WalkState& w, double fi) {
+  const double one = 1.0;
+  const double Sn = w.S;
+  const double Sn1 = Sn + one;
+  double prod = dg_pow(Sn / Sn1, Sn) * dg_pow(one / Sn1, one);
+  double t;
+  if (fi == 0.0) {
+    t = dg_log2(prod);
+  } else {
+    const double fj = fi + one;
+    const double fac = dg_pow(fj / fi, fi) * dg_pow(fj, one);
+    t = dg_log2(prod * fac);
+  }
+  const double H1 = (Sn * w.H - t) / (Sn + one);
+  const double S1 = w.S + one;
+  const double N1 = w.N + 1.0;
+  const double EH = w.EH + (H1 - w.EH) / N1;
+  const double ES = w.ES + (S1 - w.ES) / N1;
+  const double EHS = w.EHS + (H1 * S1 - w.EHS) / N1;
+  const double EHH = w.EHH + (H1 * H1 - w.EHH) / N1;
+  const double ESS = w.ESS + (S1 * S1 - w.ESS) / N1;
+  const double DH = EHH - EH * EH;
+  const double DS = ESS - ES * ES;
+  const double cov = EHS - EH * ES;
+  w.R = cov == 0.0 ? 0.0 : cov / dg_sqrt(DH * DS);
+  w.H = H1;
+  w.S = S1;
+  w.EH = EH;
+  w.ES = ES;
+  w.EHS = EHS;
+  w.EHH = EHH;
+  w.ESS = ESS;
+  w.N = N1;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
+  __shared__ uint32_t s_path[kWalkBlock * kPathLds];
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* path = s_path + threadIdx.x * kPathLds;
+  const uint64_t slot = (uint64_t)blockIdx.x * kWalkBlock + threadIdx.x;
+  uint32_t* spill = a.spill + slot * a.spill_per_slot;
+
+  bool active = false;
+  bool drained = false;  // wave-uniform
+  uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+
+  uint32_t wid = 0, cur = 0, cur_deg = 0, len = 0, step = 0;
+  uint64_t cur_rb = 0;
+  bool pending_back = false;
+  WalkState ws = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  dg_stream rs;
+  dg_stream_init(&rs, a.seed, 0, 0, DG_TAG_WALK);
+
+  while (true) {
+    // ---- hand new walk ids to idle lanes --------------------------------
+    if (!drained) {
+      const uint64_t want = __ballot(!active);
+      if (want != 0) {
+        const uint32_t nwant = (uint32_t)__popcll(want);
+        const uint32_t rank = lanes_below(want);
+        const uint64_t avail = pool_end - pool_next;
+        uint64_t id;
+        if (avail >= nwant) {
+          id = pool_next + rank;
+          pool_next += nwant;
+        } else {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(&a.q->next, (unsigned long long)kPool);
+          base = __shfl(base, 0, 64);
+          id = rank < avail ? pool_next + rank : base + (rank - avail);
+          pool_next = base + (nwant - avail);
+          pool_end = base + kPool;
+        }
+        if (!active && id < a.walker_num) {
+          // walk.hpp:529-589 (start = id mod V), :661-669 (footprint, step 0),
+          // :849-856 (first-visit state)
+          wid = (uint32_t)id;
+          const uint32_t w = a.walker_begin + wid;
+          cur = w % a.v_num;
+          cur_rb = a.row_ptr[cur];
+          cur_deg = (uint32_t)(a.row_ptr[cur + 1] - cur_rb);
+          path[0] = cur;
+          len = 1;
+          step = 0;
+          pending_back = false;
+          ws.H = 0.0;
+          ws.S = 1.0;
+          ws.EH = 0.0;
+          ws.ES = 1.0;
+          ws.EHS = 0.0;
+          ws.EHH = 0.0;
+          ws.ESS = 1.0;
+          ws.N = 1.0;
+          ws.R = 1.0;
+          dg_stream_init(&rs, a.seed, w, a.round, DG_TAG_WALK);
+          active = true;
+        }
+        if (__ballot(!active) != 0) drained = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+
+    // ---- one appended vertex per active lane ------------------------------
+    bool finish = false;
+    if (active) {
+      if (a.max_length != 0 && len >= a.max_length) {
+        finish = true;
+      } else {
+        bool append = false;
+        bool trace = false;
+        uint32_t v_app = 0, cand = 0, cand_deg = 0;
+        uint64_t cand_rb = 0;
+        if (pending_back) {
+          v_app = cur;  // walk.hpp:986-1017: return to the current vertex
+          append = true;
+        } else if (cur_deg == 0) {
+          finish = true;  // walk.hpp:887-892
+        } else {
+          const uint32_t i = dg_stream_uniform(&rs, cur_deg);  // walk.hpp:899
+          const uint2 e = a.adj[cur_rb + i];
+          if (cur_deg == 2 && a.adj[cur_rb].x == cur) {
+            finish = true;  // walk.hpp:900-902
+          } else if (!(step == 0 && e.x == cur)) {  // walk.hpp:904-906 redraw
+            const uint32_t u = dg_stream_u31(&rs);   // walk.hpp:928
+            cand = e.x;
+            cand_rb = a.row_ptr[cand];
+            cand_deg = (uint32_t)(a.row_ptr[cand + 1] - cand_rb);
+            const int sd = (int)cur_deg;
+            const int dd = (int)cand_deg;
+            const int cnv = (int)e.y;  // walk.hpp:920
+            const double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) /
+                              (double)(sd < dd ? sd : dd);  // walk.hpp:933
+            const double pn = dg_tanh(pa);                  // walk.hpp:934
+            const double p = (double)u / 2147483647.0;
+            trace = (pn <= p);  // walk.hpp:938
+            v_app = cand;
+            append = true;
+          }
+        }
+        if (append) {
+          // walk.hpp:961-985: footprint, f_i = visits so far, entropy update
+          uint32_t fi = 0;
+          const uint32_t n1 = len < (uint32_t)kPathLds ? len : (uint32_t)kPathLds;
+          for (uint32_t j = 0; j < n1; ++j) fi += (path[j] == v_app);
+          for (uint32_t j = kPathLds; j < len; ++j) fi += (spill[j - kPathLds] == v_app);
+          if (len < (uint32_t)kPathLds)
+            path[len] = v_app;
+          else
+            spill[len - kPathLds] = v_app;
+          ++len;
+          ++step;
+          state_iterate(ws, (double)fi);
+          if (step > a.min_length && (ws.R * ws.R < a.mu || ws.R < 0.0)) {
+            finish = true;
+          } else if (a.max_length != 0 && len >= a.max_length) {
+            finish = true;
+          } else if (len >= a.length_limit) {
+            finish = true;
+            atomicOr(&a.q->flags, DG_QFLAG_LENGTH_LIMIT);
+          } else if (pending_back) {
+            pending_back = false;
+          } else if (trace) {
+            pending_back = true;
+          } else {
+            cur = cand;  // walk.hpp:1018-1023
+            cur_rb = cand_rb;
+            cur_deg = cand_deg;
+          }
+        }
+      }
+    }
+
+    // ---- write finished walks out, cooperatively per wave ------------------
+    const uint64_t fin = __ballot(finish);
+    if (fin != 0) {
+      const uint32_t my_len = finish ? len : 0u;
+      uint32_t incl = my_len;
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if ((int)lane >= d) incl += t;
+      }
+      const uint32_t total = __shfl(incl, 63, 64);
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&a.q->cursor, (unsigned long long)total);
+      base = __shfl(base, 0, 64);
+      const bool fits = base + total <= a.tok_cap;
+      const uint64_t my_off = base + incl - my_len;
+      if (finish) {
+        a.walk_off[wid] = my_off;
+        a.walk_len[wid] = len;
+        atomicMax(&a.q->max_len, len);
+      }
+      if (!fits) {
+        if (lane == 0) atomicOr(&a.q->flags, DG_QFLAG_TOKEN_OVERFLOW);
+      } else {
+        __builtin_amdgcn_wave_barrier();
+        uint64_t mm = fin;
+        const uint32_t* wave_path = s_path + (threadIdx.x - lane) * kPathLds;
+        while (mm != 0) {
+          const int L = __builtin_ctzll(mm);
+          mm &= mm - 1;
+          const uint32_t n = __shfl(len, L, 64);
+          const unsigned long long off = __shfl((unsigned long long)my_off, L, 64);
+          const uint32_t nl = n < (uint32_t)kPathLds ? n : (uint32_t)kPathLds;
+          const uint32_t* src = wave_path + L * kPathLds;
+          for (uint32_t j = lane; j < nl; j += 64) a.tokens[off + j] = src[j];
+        }
+        if (finish)  // the spilled tail, by its own lane
+          for (uint32_t j = kPathLds; j < len; ++j) a.tokens[my_off + j] = spill[j - kPathLds];
+      }
+      if (finish) active = false;
+    }
+  }
+}
+
+__global__ void queue_reset_kernel(dg_walk_queue* q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    q->next = 0;
+    q->cursor = 0;
+    q->flags = 0;
+    q->max_len = 0;
+    q->zero_terms = 0;
+    q->re = 0.0;
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---- exclusive scan of walk lengths (three phases, 1024 items per block) ----
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* s_wave, uint64_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint64_t incl = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t t = __shfl_up(incl, d, 64);
+    if ((int)lane >= d) incl += t;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint64_t wave_prefix = 0, sum = 0;
+  for (uint32_t w = 0; w < kScanBlock / 64; ++w) {
+    if (w < wave) wave_prefix += s_wave[w];
+    sum += s_wave[w];
+  }
+  __syncthreads();
+  *total = sum;
+  return wave_prefix + incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tile_sums(const uint32_t* len, uint64_t n,
+                                                             uint64_t* tile_sums) {
+  __shared__ uint64_t s_wave[kScanBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += len[base + k];
+  uint64_t total;
+  block_exclusive_scan(s, s_wave, &total);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile sums in one block; tile_sums[n_tiles] gets the total.
+__global__ __launch_bounds__(kScanBlock) void scan_tile_prefix(uint64_t* tile_sums,
+                                                               uint64_t n_tiles) {
+  __shared__ uint64_t s_wave[kScanBlock / 64];
+  const uint64_t per = (n_tiles + kScanBlock - 1) / kScanBlock;
+  const uint64_t lo = threadIdx.x * per;
+  const uint64_t hi = lo + per < n_tiles ? lo + per : n_tiles;
+  uint64_t s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += tile_sums[i];
+  uint64_t total;
+  uint64_t run = block_exclusive_scan(s, s_wave, &total);
+  for (uint64_t i = lo; i < hi; ++i) {
+    const uint64_t t = tile_sums[i];
+    tile_sums[i] = run;
+    run += t;
+  }
+  if (threadIdx.x == 0) tile_sums[n_tiles] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tile_apply(const uint32_t* len, uint64_t n,
+                                                              const uint64_t* tile_prefix,
+                                                              uint64_t n_tiles, uint64_t* off) {
+  __shared__ uint64_t s_wave[kScanBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? len[base + k] : 0;
+    s += v[k];
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_scan(s, s_wave, &total) + tile_prefix[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) off[base + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) off[n] = tile_prefix[n_tiles];
+}
+
+// ---- corpus packing: walk-id order, plus cumulative visit counts -----------
+__global__ void pack_kernel(const uint32_t* raw, const uint64_t* walk_off, const uint64_t* out_off,
+                            uint64_t n_walks, uint32_t* tokens, uint32_t* freq) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t src = walk_off[w];
+    const uint64_t dst = out_off[w];
+    const uint64_t n = out_off[w + 1] - dst;
+    for (uint64_t j = 0; j < n; ++j) {
+      const uint32_t v = raw[src + j];
+      tokens[dst + j] = v;
+      atomicAdd(&freq[v], 1u);  // path.hpp:91,95 (context_map_freq, vertex_cn)
+    }
+  }
+}
+
+// ---- relative entropy (walk.hpp:693-725) in a fixed summation order -------
+// terms[i] = p_i (ln p_i - ln q_i); chunk sums of 1024 consecutive terms added
+// left to right; the chunk sums then added left to right by one thread.
+constexpr uint32_t kReChunk = 1024;
+
+__global__ void re_terms_kernel(const uint32_t* deg, const uint32_t* freq, uint32_t v_num,
+                                uint64_t dsum, uint64_t wsum, double* terms, dg_walk_queue* q) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v_num;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t d = deg[i], f = freq[i];
+    double t = 0.0;
+    if (d == 0 || f == 0) {
+      atomicAdd(&q->zero_terms, 1u);
+    } else {
+      const double pi = (double)d / (double)dsum;
+      const double qi = (double)f / (double)wsum;
+      t = pi * (dg_log(pi) - dg_log(qi));
+    }
+    terms[i] = t;
+  }
+}
+
+__global__ void re_chunks_kernel(const double* terms, const uint32_t* deg, const uint32_t* freq,
+                                 uint32_t v_num, double* chunks, uint32_t n_chunks) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_chunks) return;
+  const uint32_t lo = c * kReChunk;
+  const uint32_t hi = lo + kReChunk < v_num ? lo + kReChunk : v_num;
+  double part = 0.0;
+  for (uint32_t i = lo; i < hi; ++i)
+    if (deg[i] != 0 && freq[i] != 0) part = part + terms[i];
+  chunks[c] = part;
+}
+
+__global__ void re_final_kernel(const double* chunks, uint32_t n_chunks, dg_walk_queue* q) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double total = 0.0;
+  for (uint32_t c = 0; c < n_chunks; ++c) total = total + chunks[c];
+  q->re = total;
+}
+
+__global__ void offsets_shift_kernel(const uint64_t* src, uint64_t n, uint64_t add,
+                                     uint64_t* dst) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] + add;
+}
+
+int grid_for(dg_ctx* ctx, uint64_t n, int block) {
+  uint64_t g = (n + block - 1) / block;
+  uint64_t cap = (uint64_t)ctx->num_cus * 16;
+  if (g > cap) g = cap;
+  return g == 0 ? 1 : (int)g;
+}
+
+}  // namespace
+
+void dg_walk_free(dg_ctx* ctx) {
+  void* bufs[] = {ctx->d_freq,   ctx->d_tok_raw,  ctx->d_walk_off, ctx->d_walk_len,
+                  ctx->d_tokens, ctx->d_offsets,  ctx->d_spill,    ctx->d_scan_tmp,
+                  ctx->d_re_terms, ctx->d_re_chunks, ctx->d_train_tokens, ctx->d_train_offsets};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  ctx->d_freq = nullptr;
+  ctx->d_tok_raw = nullptr;
+  ctx->d_walk_off = nullptr;
+  ctx->d_walk_len = nullptr;
+  ctx->d_tokens = nullptr;
+  ctx->d_offsets = nullptr;
+  ctx->d_spill = nullptr;
+  ctx->d_scan_tmp = nullptr;
+  ctx->d_re_terms = nullptr;
+  ctx->d_re_chunks = nullptr;
+  ctx->d_train_tokens = nullptr;
+  ctx->d_train_offsets = nullptr;
+  ctx->tok_raw_cap = ctx->walk_cap = ctx->walk_len_cap = ctx->tokens_cap = ctx->offsets_cap = 0;
+  ctx->spill_cap = ctx->scan_tmp_cap = ctx->re_cap = 0;
+  ctx->train_tokens_cap = ctx->train_offsets_cap = 0;
+  ctx->train_walks = ctx->train_tokens = 0;
+  ctx->cur_walks = ctx->cur_tokens = 0;
+  ctx->freq_total = 0;
+  ctx->prev_re = 0.0;
+  ctx->rounds_done = 0;
+}
+
+int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_off) {
+  const uint64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+  DG_CHECK(dg_reserve(ctx, &ctx->d_scan_tmp, &ctx->scan_tmp_cap, n_tiles + 1));
+  if (n_tiles > 0) {
+    scan_tile_sums<<<(unsigned)n_tiles, kScanBlock, 0, ctx->stream>>>(d_len, n, ctx->d_scan_tmp);
+    scan_tile_prefix<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan_tmp, n_tiles);
+    scan_tile_apply<<<(unsigned)n_tiles, kScanBlock, 0, ctx->stream>>>(d_len, n, ctx->d_scan_tmp,
+                                                                       n_tiles, d_off);
+  } else {
+    DG_HIP(ctx, hipMemsetAsync(d_off, 0, sizeof(uint64_t), ctx->stream));
+  }
+  DG_HIP(ctx, hipGetLastError());
+  return DG_OK;
+}
+
+extern "C" void dg_walk_params_default(dg_walk_params* p, uint32_t v_num) {
+  if (!p) return;
+  p->seed = 1;
+  p->walkers_per_round = v_num;
+  p->init_round = 5;
+  p->min_length = 20;
+  p->max_length = 0;
+  p->mu = 0.999;
+  p->delta = 0.001;
+  p->max_rounds = 30;
+  p->rounds_mode = DG_ROUNDS_REFERENCE;
+  p->length_limit = 0;
+  p->reserved = 0;
+}
+
+extern "C" int dg_walk_reset(dg_ctx* ctx) {
+  if (!ctx) return DG_ERR_ARG;
+  if (!ctx->d_row_ptr) return dg_fail(ctx, DG_ERR_STATE, "no graph loaded");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  if (!ctx->d_freq) {
+    DG_HIP(ctx, hipMalloc((void**)&ctx->d_freq, sizeof(uint32_t) * ctx->v_num));
+  }
+  fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq, ctx->v_num,
+                                                                           0u);
+  DG_HIP(ctx, hipGetLastError());
+  ctx->freq_total = 0;
+  ctx->prev_re = 0.0;
+  ctx->rounds_done = 0;
+  return DG_OK;
+}
+
+static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                           uint32_t walker_begin, uint32_t walker_num, dg_round_stats* out) {
+  if (!ctx->d_freq) DG_CHECK(dg_walk_reset(ctx));
+  const uint32_t length_limit = p->length_limit ? p->length_limit : kDefaultLengthLimit;
+  if (p->max_length != 0 && p->max_length > length_limit)
+    return dg_fail(ctx, DG_ERR_ARG, "max_length %u exceeds length_limit %u", p->max_length,
+                   length_limit);
+  if (p->min_length + 2 > length_limit)
+    return dg_fail(ctx, DG_ERR_ARG, "min_length %u needs length_limit > %u", p->min_length,
+                   p->min_length + 1);
+
+  // persistent grid: every resident block slot on the device, fewer for small rounds
+  int per_cu = 0;
+  DG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_kernel, kWalkBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  uint64_t blocks = (uint64_t)ctx->num_cus * per_cu;
+  const uint64_t need_blocks = ((uint64_t)walker_num + kWalkBlock - 1) / kWalkBlock;
+  if (blocks > need_blocks) blocks = need_blocks ? need_blocks : 1;
+  const uint32_t spill_per_slot =
+      length_limit > (uint32_t)kPathLds ? length_limit - (uint32_t)kPathLds : 0;
+
+  DG_CHECK(dg_reserve(ctx, &ctx->d_walk_off, &ctx->walk_cap, (size_t)walker_num));
+  DG_CHECK(dg_reserve(ctx, &ctx->d_walk_len, &ctx->walk_len_cap, (size_t)walker_num + 1));
+  DG_CHECK(dg_reserve(ctx, &ctx->d_offsets, &ctx->offsets_cap, (size_t)walker_num + 1));
+  DG_CHECK(dg_reserve(ctx, &ctx->d_spill, &ctx->spill_cap,
+                      (size_t)blocks * kWalkBlock * spill_per_slot));
+  const uint32_t est_len = p->max_length ? (p->max_length < 48 ? p->max_length : 48) : 32;
+  size_t tok_need = (size_t)walker_num * est_len;
+  if (ctx->tok_raw_cap < tok_need) DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, tok_need));
+
+  WalkArgs a;
+  a.row_ptr = ctx->d_row_ptr;
+  a.adj = ctx->d_adj;
+  a.v_num = ctx->v_num;
+  a.round = round;
+  a.seed = p->seed;
+  a.walker_begin = walker_begin;
+  a.walker_num = walker_num;
+  a.min_length = p->min_length;
+  a.max_length = p->max_length;
+  a.length_limit = length_limit;
+  a.spill_per_slot = spill_per_slot;
+  a.mu = p->mu;
+  a.walk_off = ctx->d_walk_off;
+  a.walk_len = ctx->d_walk_len;
+  a.spill = ctx->d_spill;
+  a.q = ctx->d_queue;
+
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    a.tokens = ctx->d_tok_raw;
+    a.tok_cap = ctx->tok_raw_cap;
+    queue_reset_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
+    size_t tk;
+    DG_CHECK(dg_time_begin(ctx, "walk", &tk));
+    walk_kernel<<<(unsigned)blocks, kWalkBlock, 0, ctx->stream>>>(a);
+    DG_HIP(ctx, hipGetLastError());
+    DG_CHECK(dg_time_end(ctx, tk));
+    DG_HIP(ctx, hipMemcpyAsync(ctx->h_queue, ctx->d_queue, sizeof(dg_walk_queue),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->h_queue->flags & DG_QFLAG_LENGTH_LIMIT)
+      return dg_fail(ctx, DG_ERR_WALK_TOO_LONG,
+                     "round %u: a walk reached the length limit %u (raise length_limit)", round,
+                     length_limit);
+    if (!(ctx->h_queue->flags & DG_QFLAG_TOKEN_OVERFLOW)) break;
+    const size_t need = (size_t)(ctx->h_queue->cursor + ctx->h_queue->cursor / 8 + 1024);
+    DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, need));
+    if (attempt == 2) return dg_fail(ctx, DG_ERR_STATE, "token buffer sizing did not converge");
+  }
+  const uint64_t n_tok = ctx->h_queue->cursor;
+  const uint32_t max_len = ctx->h_queue->max_len;
+
+  // pack in walk-id order and count visits
+  DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, (size_t)n_tok));
+  size_t tp;
+  DG_CHECK(dg_time_begin(ctx, "pack", &tp));
+  DG_CHECK(dg_scan_lengths(ctx, ctx->d_walk_len, walker_num, ctx->d_offsets));
+  pack_kernel<<<grid_for(ctx, walker_num, 256), 256, 0, ctx->stream>>>(
+      ctx->d_tok_raw, ctx->d_walk_off, ctx->d_offsets, walker_num, ctx->d_tokens, ctx->d_freq);
+  DG_HIP(ctx, hipGetLastError());
+  DG_CHECK(dg_time_end(ctx, tp));
+  ctx->freq_total += n_tok;
+  ctx->cur_walks = walker_num;
+  ctx->cur_tokens = n_tok;
+
+  // relative entropy against the cumulative visit counts
+  const uint32_t n_chunks = (ctx->v_num + kReChunk - 1) / kReChunk;
+  DG_CHECK(dg_reserve(ctx, &ctx->d_re_terms, &ctx->re_cap, (size_t)ctx->v_num + n_chunks));
+  ctx->d_re_chunks = nullptr;  // chunks live after the terms
+  double* chunks = ctx->d_re_terms + ctx->v_num;
+  size_t tr;
+  DG_CHECK(dg_time_begin(ctx, "re", &tr));
+  re_terms_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
+      ctx->d_deg, ctx->d_freq, ctx->v_num, ctx->m, ctx->freq_total, ctx->d_re_terms, ctx->d_queue);
+  re_chunks_kernel<<<(n_chunks + 63) / 64, 64, 0, ctx->stream>>>(ctx->d_re_terms, ctx->d_deg,
+                                                                ctx->d_freq, ctx->v_num, chunks,
+                                                                n_chunks);
+  re_final_kernel<<<1, 64, 0, ctx->stream>>>(chunks, n_chunks, ctx->d_queue);
+  DG_HIP(ctx, hipGetLastError());
+  DG_CHECK(dg_time_end(ctx, tr));
+  DG_HIP(ctx, hipMemcpyAsync(ctx->h_queue, ctx->d_queue, sizeof(dg_walk_queue),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+  double walk_ms = 0.0, total_ms = 0.0;
+  for (auto& pe : ctx->pending) {
+    float ms = 0.f;
+    DG_HIP(ctx, hipEventElapsedTime(&ms, pe.start, pe.stop));
+    if (pe.name == "walk") walk_ms = ms;  // the successful (last) attempt
+    total_ms += ms;
+  }
+  DG_CHECK(dg_time_resolve(ctx));
+
+  const double re = ctx->h_queue->re;
+  const double delta = ctx->rounds_done == 0 ? re : re - ctx->prev_re;
+  ctx->prev_re = re;
+  ctx->rounds_done += 1;
+  if (out) {
+    out->round = round;
+    out->walkers = walker_num;
+    out->tokens = n_tok;
+    out->steps = n_tok - walker_num;
+    out->re = re;
+    out->delta_re = delta;
+    out->walk_ms = walk_ms;
+    out->total_ms = total_ms;
+    out->max_walk_len = max_len;
+    out->stop = 0;
+  }
+  if (ctx->h_queue->zero_terms != 0)
+    return dg_fail(ctx, DG_ERR_RE_UNDEFINED,
+                   "relative entropy undefined: %u vertices with degree 0 or no visits",
+                   ctx->h_queue->zero_terms);
+  return DG_OK;
+}
+
+extern "C" int dg_walk_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                             uint32_t walker_begin, uint32_t walker_num, dg_round_stats* out) {
+  if (!ctx || !p) return DG_ERR_ARG;
+  if (!ctx->d_row_ptr) return dg_fail(ctx, DG_ERR_STATE, "no graph loaded");
+  if (!(p->mu > 0.0)) return dg_fail(ctx, DG_ERR_ARG, "mu must be positive");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  return walk_round_impl(ctx, p, round, walker_begin, walker_num, out);
+}
+
+extern "C" int dg_corpus_append(dg_ctx* ctx) {
+  if (!ctx) return DG_ERR_ARG;
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t nw = ctx->cur_walks, nt = ctx->cur_tokens;
+  DG_CHECK(dg_grow_keep(ctx, &ctx->d_train_tokens, &ctx->train_tokens_cap,
+                        (size_t)(ctx->train_tokens + nt), (size_t)ctx->train_tokens));
+  DG_CHECK(dg_grow_keep(ctx, &ctx->d_train_offsets, &ctx->train_offsets_cap,
+                        (size_t)(ctx->train_walks + nw + 1), (size_t)(ctx->train_walks + 1)));
+  if (ctx->train_walks == 0)
+    DG_HIP(ctx, hipMemsetAsync(ctx->d_train_offsets, 0, sizeof(uint64_t), ctx->stream));
+  if (nt)
+    DG_HIP(ctx, hipMemcpyAsync(ctx->d_train_tokens + ctx->train_tokens, ctx->d_tokens,
+                               sizeof(uint32_t) * nt, hipMemcpyDeviceToDevice, ctx->stream));
+  if (nw)
+    offsets_shift_kernel<<<grid_for(ctx, nw, 256), 256, 0, ctx->stream>>>(
+        ctx->d_offsets + 1, nw, ctx->train_tokens, ctx->d_train_offsets + ctx->train_walks + 1);
+  DG_HIP(ctx, hipGetLastError());
+  ctx->train_walks += nw;
+  ctx->train_tokens += nt;
+  return DG_OK;
+}
+
+extern "C" int dg_train_corpus_info(dg_ctx* ctx, uint64_t* n_walks, uint64_t* n_tokens) {
+  if (!ctx) return DG_ERR_ARG;
+  if (n_walks) *n_walks = ctx->train_walks;
+  if (n_tokens) *n_tokens = ctx->train_tokens;
+  return DG_OK;
+}
+
+extern "C" int dg_train_corpus_clear(dg_ctx* ctx) {
+  if (!ctx) return DG_ERR_ARG;
+  ctx->train_walks = 0;
+  ctx->train_tokens = 0;
+  return DG_OK;
+}
+
+extern "C" int dg_walk_run(dg_ctx* ctx, const dg_walk_params* p, dg_round_stats* per_round,
+                           uint32_t max_stats, uint32_t* n_rounds, int keep_corpus) {
+  if (!ctx || !p) return DG_ERR_ARG;
+  if (!ctx->d_row_ptr) return dg_fail(ctx, DG_ERR_STATE, "no graph loaded");
+  if (p->walkers_per_round == 0 || p->init_round == 0)
+    return dg_fail(ctx, DG_ERR_ARG, "walkers_per_round and init_round must be positive");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  DG_CHECK(dg_walk_reset(ctx));
+  uint32_t iter = 1;
+  uint32_t done = 0;
+  const uint32_t max_rounds = p->max_rounds ? p->max_rounds : 30;
+  while (true) {
+    // walk.hpp:614 (round 1: walker_num * init_round), :764-775 (later rounds)
+    const uint64_t nw64 = (iter == 1) ? (uint64_t)p->walkers_per_round * p->init_round
+                                      : (uint64_t)p->walkers_per_round;
+    if (nw64 > 0xffffffffull) return dg_fail(ctx, DG_ERR_ARG, "too many walkers per round");
+    dg_round_stats st;
+    DG_CHECK(walk_round_impl(ctx, p, iter, 0, (uint32_t)nw64, &st));
+    if (keep_corpus) DG_CHECK(dg_corpus_append(ctx));
+    // walk.hpp:735-749: stop after round max_rounds; optionally on |delta RE| <= delta
+    bool stop = (iter + 1 > max_rounds);
+    if (p->rounds_mode == DG_ROUNDS_CONVERGE && (st.delta_re < 0 ? -st.delta_re : st.delta_re) <= p->delta)
+      stop = true;
+    st.stop = stop ? 1u : 0u;
+    if (per_round && done < max_stats) per_round[done] = st;
+    ++done;
+    ++iter;
+    if (stop) break;
+  }
+  if (n_rounds) *n_rounds = done;
+  return DG_OK;
+}
+
+extern "C" int dg_corpus_info(dg_ctx* ctx, uint64_t* n_walks, uint64_t* n_tokens) {
+  if (!ctx) return DG_ERR_ARG;
+  if (n_walks) *n_walks = ctx->cur_walks;
+  if (n_tokens) *n_tokens = ctx->cur_tokens;
+  return DG_OK;
+}
+
+extern "C" int dg_corpus_copy(dg_ctx* ctx, uint32_t* tokens, uint64_t* offsets) {
+  if (!ctx) return DG_ERR_ARG;
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (tokens && ctx->cur_tokens)
+    DG_HIP(ctx, hipMemcpy(tokens, ctx->d_tokens, sizeof(uint32_t) * ctx->cur_tokens,
+                          hipMemcpyDeviceToHost));
+  if (offsets && ctx->d_offsets)
+    DG_HIP(ctx, hipMemcpy(offsets, ctx->d_offsets, sizeof(uint64_t) * (ctx->cur_walks + 1),
+                          hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+extern "C" int dg_vertex_freq(dg_ctx* ctx, uint32_t* freq) {
+  if (!ctx || !freq) return DG_ERR_ARG;
+  if (!ctx->d_freq) return dg_fail(ctx, DG_ERR_STATE, "no walk has run");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  DG_HIP(ctx, hipMemcpy(freq, ctx->d_freq, sizeof(uint32_t) * ctx->v_num, hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+// Walk text file of include/path.hpp:67-119.  Formatting is done into a large
+// buffer with a hand-rolled decimal conversion; the bytes equal fprintf's.
+extern "C" int dg_corpus_write_text(dg_ctx* ctx, const char* path, uint32_t threads) {
+  if (!ctx || !path) return DG_ERR_ARG;
+  if (threads == 0) threads = 1;
+  std::vector<uint32_t> tok(ctx->cur_tokens);
+  std::vector<uint64_t> off(ctx->cur_walks + 1);
+  DG_CHECK(dg_corpus_copy(ctx, tok.data(), off.data()));
+  FILE* f = fopen(path, "w");
+  if (!f) return dg_fail(ctx, DG_ERR_IO, "cannot open %s for writing", path);
+  std::vector<char> buf;
+  buf.reserve(1 << 22);
+  auto flush = [&]() -> bool {
+    bool ok = buf.empty() || fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    buf.clear();
+    return ok;
+  };
+  bool ok = true;
+  for (uint32_t t = 0; t < threads && ok; ++t) {
+    for (uint64_t w = t; w < ctx->cur_walks && ok; w += threads) {
+      for (uint64_t i = off[w]; i < off[w + 1]; ++i) {
+        char digits[12];
+        int nd = 0;
+        uint32_t v = tok[i];
+        do {
+          digits[nd++] = (char)('0' + v % 10);
+          v /= 10;
+        } while (v);
+        buf.push_back(' ');
+        while (nd) buf.push_back(digits[--nd]);
+      }
+      buf.push_back('\n');
+      if (buf.size() > (1u << 22) - 4096) ok = flush();
+    }
+  }
+  ok = flush() && ok;
+  if (fclose(f) != 0) ok = false;
+  return ok ? DG_OK : dg_fail(ctx, DG_ERR_IO, "write to %s failed", path);
+}

@@ -1,0 +1,322 @@
+"""ctypes bindings for include/distger_hip.h.
+
+Every wrapper checks the returned status and raises DgError with the
+library's message (dg_last_error) on failure.  Host buffers are numpy arrays
+owned by the caller, as the C ABI requires.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libdistger_hip.so"
+
+DG_OK = 0
+DG_CN_FROM_DATA = 0
+DG_CN_COMPUTE = 1
+DG_ROUNDS_REFERENCE = 0
+DG_ROUNDS_CONVERGE = 1
+
+# 12-byte binary edge record (include/type.hpp Edge<float>)
+EDGE_DTYPE = np.dtype([("src", "<u4"), ("dst", "<u4"), ("data", "<f4")])
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class DgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"distger error {code}: {msg}")
+        self.code = code
+
+
+class WalkParams(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("walkers_per_round", ctypes.c_uint32),
+        ("init_round", ctypes.c_uint32),
+        ("min_length", ctypes.c_uint32),
+        ("max_length", ctypes.c_uint32),
+        ("mu", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("max_rounds", ctypes.c_uint32),
+        ("rounds_mode", ctypes.c_uint32),
+        ("length_limit", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class RoundStats(ctypes.Structure):
+    _fields_ = [
+        ("round", ctypes.c_uint32),
+        ("walkers", ctypes.c_uint32),
+        ("tokens", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64),
+        ("re", ctypes.c_double),
+        ("delta_re", ctypes.c_double),
+        ("walk_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("max_walk_len", ctypes.c_uint32),
+        ("stop", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib_path():
+    return os.path.normpath(os.path.join(_HERE, "..", "lib", _LIB_NAME))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
+
+
+def load_library():
+    """Load libdistger_hip.so from the package tree; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise DgError(-1, f"{path} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    c = ctypes
+    vp = c.c_void_p
+    sig = {
+        "dg_open": (c.c_int, [c.c_int, c.POINTER(vp)]),
+        "dg_close": (None, [vp]),
+        "dg_last_error": (c.c_char_p, [vp]),
+        "dg_sync": (c.c_int, [vp]),
+        "dg_graph_from_edges": (c.c_int, [vp, c.c_uint32, vp, c.c_uint64, c.c_int, c.c_int]),
+        "dg_graph_load_bin": (c.c_int, [vp, c.c_char_p, c.c_uint32, c.c_int, c.c_int]),
+        "dg_graph_info": (c.c_int, [vp, c.POINTER(c.c_uint32), c.POINTER(c.c_uint64)]),
+        "dg_graph_get_csr": (c.c_int, [vp, vp, vp, vp]),
+        "dg_walk_params_default": (None, [c.POINTER(WalkParams), c.c_uint32]),
+        "dg_walk_round": (c.c_int, [vp, c.POINTER(WalkParams), c.c_uint32, c.c_uint32,
+                                    c.c_uint32, c.POINTER(RoundStats)]),
+        "dg_walk_run": (c.c_int, [vp, c.POINTER(WalkParams), c.POINTER(RoundStats), c.c_uint32,
+                                  c.POINTER(c.c_uint32), c.c_int]),
+        "dg_walk_reset": (c.c_int, [vp]),
+        "dg_corpus_info": (c.c_int, [vp, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]),
+        "dg_corpus_copy": (c.c_int, [vp, vp, vp]),
+        "dg_corpus_write_text": (c.c_int, [vp, c.c_char_p, c.c_uint32]),
+        "dg_vertex_freq": (c.c_int, [vp, vp]),
+        "dg_corpus_append": (c.c_int, [vp]),
+        "dg_train_corpus_info": (c.c_int, [vp, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]),
+        "dg_train_corpus_clear": (c.c_int, [vp]),
+        "dg_timing_get": (c.c_int, [vp, c.c_char_p, c.POINTER(c.c_uint64), c.POINTER(c.c_double)]),
+        "dg_timing_reset": (c.c_int, [vp]),
+        "dg_gen_chung_lu": (c.c_int, [c.c_uint32, c.c_uint64, c.c_double, c.c_uint32, c.c_uint64,
+                                      vp, c.c_uint64, c.POINTER(c.c_uint64)]),
+        "dg_gen_rmat": (c.c_int, [c.c_uint32, c.c_uint32, c.c_double, c.c_double, c.c_double,
+                                  c.c_uint64, vp, c.c_uint64, c.POINTER(c.c_uint64)]),
+        "dg_lp_split": (c.c_int, [c.c_uint32, vp, c.c_uint64, c.c_double, c.c_uint64,
+                                  c.POINTER(c.c_uint64), vp, c.c_uint64, c.POINTER(c.c_uint64)]),
+        "dg_auc": (c.c_double, [vp, c.c_uint64, vp, c.c_uint64]),
+        "dg_write_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64]),
+        "dg_read_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64, c.POINTER(c.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _vp(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check_host(rc, what):
+    if rc != DG_OK:
+        raise DgError(rc, what)
+
+
+class Context:
+    """One device context (dg_ctx).  Use as a context manager or call close()."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.dg_open(device, ctypes.byref(h))
+        if rc != DG_OK:
+            raise DgError(rc, f"dg_open(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.dg_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != DG_OK:
+            msg = self.L.dg_last_error(self.h)
+            raise DgError(rc, msg.decode() if msg else "")
+
+    def sync(self):
+        self._check(self.L.dg_sync(self.h))
+
+    # ---- graph
+    def graph_from_edges(self, v_num, edges, undirected=True, cn_mode=DG_CN_COMPUTE):
+        edges = np.ascontiguousarray(edges, dtype=EDGE_DTYPE)
+        self._check(self.L.dg_graph_from_edges(self.h, v_num, _vp(edges), len(edges),
+                                               int(undirected), cn_mode))
+
+    def graph_load_bin(self, path, v_num, undirected=True, cn_mode=DG_CN_FROM_DATA):
+        self._check(self.L.dg_graph_load_bin(self.h, path.encode(), v_num, int(undirected), cn_mode))
+
+    def graph_info(self):
+        v = ctypes.c_uint32()
+        m = ctypes.c_uint64()
+        self._check(self.L.dg_graph_info(self.h, ctypes.byref(v), ctypes.byref(m)))
+        return v.value, m.value
+
+    def graph_csr(self):
+        v, m = self.graph_info()
+        rp = np.zeros(v + 1, np.uint64)
+        nbr = np.zeros(m, np.uint32)
+        cn = np.zeros(m, np.uint32)
+        self._check(self.L.dg_graph_get_csr(self.h, _vp(rp), _vp(nbr), _vp(cn)))
+        return rp, nbr, cn
+
+    # ---- walker
+    def default_walk_params(self, v_num=None, **kw):
+        if v_num is None:
+            v_num = self.graph_info()[0]
+        p = WalkParams()
+        self.L.dg_walk_params_default(ctypes.byref(p), v_num)
+        for k, val in kw.items():
+            setattr(p, k, val)
+        return p
+
+    def walk_reset(self):
+        self._check(self.L.dg_walk_reset(self.h))
+
+    def walk_round(self, params, round_no, walker_begin, walker_num):
+        st = RoundStats()
+        self._check(self.L.dg_walk_round(self.h, ctypes.byref(params), round_no, walker_begin,
+                                         walker_num, ctypes.byref(st)))
+        return st
+
+    def walk_run(self, params, keep_corpus=False, max_stats=256):
+        arr = (RoundStats * max_stats)()
+        n = ctypes.c_uint32()
+        self._check(self.L.dg_walk_run(self.h, ctypes.byref(params), arr, max_stats,
+                                       ctypes.byref(n), int(keep_corpus)))
+        return [arr[i] for i in range(min(n.value, max_stats))]
+
+    def corpus(self):
+        nw = ctypes.c_uint64()
+        nt = ctypes.c_uint64()
+        self._check(self.L.dg_corpus_info(self.h, ctypes.byref(nw), ctypes.byref(nt)))
+        tok = np.zeros(nt.value, np.uint32)
+        off = np.zeros(nw.value + 1, np.uint64)
+        self._check(self.L.dg_corpus_copy(self.h, _vp(tok), _vp(off)))
+        return tok, off
+
+    def corpus_write_text(self, path, threads=1):
+        self._check(self.L.dg_corpus_write_text(self.h, path.encode(), threads))
+
+    def vertex_freq(self):
+        v, _ = self.graph_info()
+        f = np.zeros(v, np.uint32)
+        self._check(self.L.dg_vertex_freq(self.h, _vp(f)))
+        return f
+
+    def corpus_append(self):
+        self._check(self.L.dg_corpus_append(self.h))
+
+    def train_corpus_info(self):
+        nw = ctypes.c_uint64()
+        nt = ctypes.c_uint64()
+        self._check(self.L.dg_train_corpus_info(self.h, ctypes.byref(nw), ctypes.byref(nt)))
+        return nw.value, nt.value
+
+    def train_corpus_clear(self):
+        self._check(self.L.dg_train_corpus_clear(self.h))
+
+    # ---- timing
+    def timing(self, name):
+        n = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        self._check(self.L.dg_timing_get(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def timing_reset(self):
+        self._check(self.L.dg_timing_reset(self.h))
+
+
+# ---- host utilities (no device needed) -------------------------------------
+
+def gen_chung_lu(v_num, e_target, gamma, min_degree, seed):
+    L = load_library()
+    cap = int(e_target) + int(v_num) * max(int(min_degree), 1) + 16
+    buf = np.zeros(cap, EDGE_DTYPE)
+    n = ctypes.c_uint64()
+    _check_host(L.dg_gen_chung_lu(v_num, int(e_target), float(gamma), min_degree, seed, _vp(buf),
+                                  cap, ctypes.byref(n)), "dg_gen_chung_lu")
+    return buf[: n.value].copy()
+
+
+def gen_rmat(scale, edge_factor, a, b, c, seed):
+    L = load_library()
+    cap = (int(edge_factor) << int(scale)) + 16
+    buf = np.zeros(cap, EDGE_DTYPE)
+    n = ctypes.c_uint64()
+    _check_host(L.dg_gen_rmat(scale, edge_factor, a, b, c, seed, _vp(buf), cap, ctypes.byref(n)),
+                "dg_gen_rmat")
+    return buf[: n.value].copy()
+
+
+def lp_split(v_num, edges, frac, seed):
+    """Returns (train_edges, pos_edges, neg_edges)."""
+    L = load_library()
+    e = np.ascontiguousarray(edges, dtype=EDGE_DTYPE).copy()
+    npos = int(frac * len(e))
+    neg = np.zeros(max(npos, 1), EDGE_DTYPE)
+    ntr = ctypes.c_uint64()
+    nn = ctypes.c_uint64()
+    _check_host(L.dg_lp_split(v_num, _vp(e), len(e), float(frac), seed, ctypes.byref(ntr),
+                              _vp(neg), len(neg), ctypes.byref(nn)), "dg_lp_split")
+    return e[: ntr.value].copy(), e[ntr.value:].copy(), neg[: nn.value].copy()
+
+
+def auc(pos_scores, neg_scores):
+    L = load_library()
+    p = np.ascontiguousarray(pos_scores, dtype=np.float32)
+    n = np.ascontiguousarray(neg_scores, dtype=np.float32)
+    return L.dg_auc(_vp(p), len(p), _vp(n), len(n))
+
+
+def write_edges_bin(path, edges):
+    L = load_library()
+    e = np.ascontiguousarray(edges, dtype=EDGE_DTYPE)
+    _check_host(L.dg_write_edges_bin(path.encode(), _vp(e), len(e)), "dg_write_edges_bin")
+
+
+def read_edges_bin(path):
+    L = load_library()
+    n = ctypes.c_uint64()
+    _check_host(L.dg_read_edges_bin(path.encode(), None, 0, ctypes.byref(n)), "dg_read_edges_bin")
+    buf = np.zeros(n.value, EDGE_DTYPE)
+    _check_host(L.dg_read_edges_bin(path.encode(), _vp(buf), n.value, ctypes.byref(n)),
+                "dg_read_edges_bin")
+    return buf

@@ -1,0 +1,212 @@
+/*
+ * distger_hip.h - C ABI of the MI355X (gfx950) DistGER embedding hot path:
+ * the information-oriented random walker and the skip-gram negative-sampling
+ * trainer, with the CSR graph, the walk corpus and both embedding tables
+ * resident in one GPU's HBM.
+ *
+ * Conventions
+ *   - Every entry point returns DG_OK (0) or a negative DG_ERR_* code; the
+ *     message of the last failure on a context is dg_last_error(ctx).  Nothing
+ *     here calls exit() or throws across the boundary (the reference exits on
+ *     CUDA errors, src/word2vec.cu:36-42, and asserts elsewhere).
+ *   - The caller owns every host buffer; the library owns device memory behind
+ *     the opaque dg_ctx.  A context is bound to one device; calls on one
+ *     context must not overlap (the reference trainer keeps all state in
+ *     globals, src/word2vec.cu:53-75, so it is not re-entrant at all).
+ *   - Vertex ids are uint32 (include/type.hpp:15), CSR offsets uint64
+ *     (edge_id_t, include/type.hpp:17).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   GraphEngine::load_graph          include/graph.hpp:642      -> dg_graph_*
+ *   WalkEngine::random_walk          include/walk.hpp:511-524   -> dg_walk_*
+ *   PathSet::dump                    include/path.hpp:67-119    -> dg_corpus_write_text
+ *   train_corpus_cuda                src/word2vec.cu:1429       -> dg_sgns_*
+ *   dsgl (CPU trainer)               include/dsgl.hpp:1464      -> dg_sgns_* (batched forms)
+ */
+#ifndef DISTGER_HIP_H
+#define DISTGER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DG_OK 0
+#define DG_ERR_ARG -1          /* invalid argument */
+#define DG_ERR_HIP -2          /* HIP runtime failure (message has the HIP error) */
+#define DG_ERR_NOMEM -3        /* device or host allocation failed */
+#define DG_ERR_IO -4           /* file missing, unreadable or truncated */
+#define DG_ERR_STATE -5        /* call out of order (e.g. walk before a graph) */
+#define DG_ERR_WALK_TOO_LONG -6 /* a walk reached the safety length limit */
+#define DG_ERR_RE_UNDEFINED -7 /* relative entropy undefined: a vertex with degree or visits 0 */
+#define DG_ERR_UNSUPPORTED -8  /* configuration not supported (e.g. embedding dim) */
+
+typedef struct dg_ctx dg_ctx;
+
+/* 12-byte binary edge record of the reference (Edge<float>,
+ * include/type.hpp:64-70; file layout include/storage.hpp:15-38). */
+typedef struct {
+  uint32_t src;
+  uint32_t dst;
+  float data;
+} dg_edge;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Open a context on HIP device `device`.  No reference counterpart (the
+ * reference binds the device implicitly in train_corpus_cuda). */
+int dg_open(int device, dg_ctx** out);
+void dg_close(dg_ctx* ctx);
+const char* dg_last_error(const dg_ctx* ctx);
+/* Wait for all work the context queued.  hipDeviceSynchronize analogue. */
+int dg_sync(dg_ctx* ctx);
+
+/* ---- graph (W1-W3) ------------------------------------------------------ */
+
+/* CN source for the walker's acceptance test. */
+#define DG_CN_FROM_DATA 0  /* edge data holds the common-neighbour count, as the
+                              reference's binary input does (gcn -> gconverter,
+                              walk.hpp:920 reads candidate->data) */
+#define DG_CN_COMPUTE 1    /* compute |N(u) & N(v) \ {u,v}| on the device */
+
+/* Build the device CSR from `e_num` edges.  Replaces GraphEngine::load_graph
+ * (include/graph.hpp:642-824) for one partition: optional undirected doubling
+ * [e0, rev e0, e1, rev e1, ...] (:671-686), degrees (:694-699), CSR in that
+ * order, stable by source (:509-540). */
+int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* edges, uint64_t e_num,
+                        int undirected, int cn_mode);
+
+/* Same, reading the reference binary edge file (include/storage.hpp:15-38). */
+int dg_graph_load_bin(dg_ctx* ctx, const char* path, uint32_t v_num, int undirected,
+                      int cn_mode);
+
+/* Graph size: vertices and directed CSR entries. */
+int dg_graph_info(dg_ctx* ctx, uint32_t* v_num, uint64_t* m);
+
+/* Copy the device CSR back: row_ptr[v_num+1], nbr[m], cn[m] (any may be NULL). */
+int dg_graph_get_csr(dg_ctx* ctx, uint64_t* row_ptr, uint32_t* nbr, uint32_t* cn);
+
+/* ---- walker (W4-W9) ---------------------------------------------------- */
+
+#define DG_ROUNDS_REFERENCE 0 /* stop after max_rounds rounds (walk.hpp:740-744) */
+#define DG_ROUNDS_CONVERGE 1  /* also stop once |delta RE| <= delta (paper; walk.hpp:745-749) */
+
+typedef struct {
+  uint64_t seed;             /* Philox key for every walk draw */
+  uint32_t walkers_per_round; /* -w (option_helper.hpp:142-180); rounds >= 2 run this many */
+  uint32_t init_round;       /* --min_R: round 1 runs walkers_per_round * init_round (walk.hpp:614) */
+  uint32_t min_length;       /* --min_L, default 20 (walk.hpp:404) */
+  uint32_t max_length;       /* cap on vertices per walk, 0 = none (reference has none) */
+  double mu;                 /* termination threshold on R^2, 0.999 (walk.hpp:24) */
+  double delta;              /* RE convergence threshold, 0.001 (walk.hpp:26) */
+  uint32_t max_rounds;       /* 30 (walk.hpp:740) */
+  uint32_t rounds_mode;      /* DG_ROUNDS_* */
+  uint32_t length_limit;     /* safety limit on vertices per walk; 0 = 4096 */
+  uint32_t reserved;
+} dg_walk_params;
+
+/* Fill `p` with the reference defaults (mu 0.999, delta 0.001, min_length 20,
+ * init_round 5, max_rounds 30, no cap) for a graph of v_num vertices. */
+void dg_walk_params_default(dg_walk_params* p, uint32_t v_num);
+
+typedef struct {
+  uint32_t round;        /* 1-based, the reference's `iter` (walk.hpp:645) */
+  uint32_t walkers;      /* walks in this round */
+  uint64_t tokens;       /* vertices written (footprints incl. step 0) */
+  uint64_t steps;        /* appended vertices (footprints with step >= 1), the
+                            reference's p_step (walk.hpp:397) */
+  double re;             /* relative entropy after this round (walk.hpp:724) */
+  double delta_re;       /* h_k - h_{k-1} (h_1 for round 1) (walk.hpp:728) */
+  double walk_ms;        /* device time of the walker kernel */
+  double total_ms;       /* device time of the whole round */
+  uint32_t max_walk_len; /* longest walk of the round */
+  uint32_t stop;         /* 1 if the round driver stops after this round */
+} dg_round_stats;
+
+/* Run one round: walks walker_begin .. walker_begin+walker_num-1 (start vertex
+ * id mod V, walk.hpp:501-509), then pack the corpus in walk-id order, add the
+ * visits to the cumulative per-vertex counts and compute the relative entropy.
+ * `round` is the 1-based round number (part of every walk's Philox counter).
+ * The round's corpus replaces the previous round's (dg_corpus_*). */
+int dg_walk_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t round, uint32_t walker_begin,
+                  uint32_t walker_num, dg_round_stats* out);
+
+/* Run the whole round loop of internal_random_walk (walk.hpp:608-782):
+ * round 1 with walkers_per_round*init_round walks, later rounds with
+ * walkers_per_round, until the stop rule.  `per_round` receives up to
+ * max_stats entries; *n_rounds the number of rounds run.  If keep_corpus is
+ * non-zero every round's corpus is appended to the device training corpus
+ * (dg_corpus_append). */
+int dg_walk_run(dg_ctx* ctx, const dg_walk_params* p, dg_round_stats* per_round,
+                uint32_t max_stats, uint32_t* n_rounds, int keep_corpus);
+
+/* Reset the cumulative per-vertex visit counts (start of a new walk run). */
+int dg_walk_reset(dg_ctx* ctx);
+
+/* Current round corpus: number of walks and tokens. */
+int dg_corpus_info(dg_ctx* ctx, uint64_t* n_walks, uint64_t* n_tokens);
+
+/* Copy the current round corpus: tokens[n_tokens], offsets[n_walks+1]. */
+int dg_corpus_copy(dg_ctx* ctx, uint32_t* tokens, uint64_t* offsets);
+
+/* Write the current round corpus as the reference walk file
+ * (include/path.hpp:67-119): " %u" per vertex, "\n" per walk, walks in
+ * `threads` segments (segment t = walks t, t+T, ...; 1 = id order). */
+int dg_corpus_write_text(dg_ctx* ctx, const char* path, uint32_t threads);
+
+/* Cumulative per-vertex visit counts (vertex_cn / context_map_freq,
+ * include/path.hpp:91-95). */
+int dg_vertex_freq(dg_ctx* ctx, uint32_t* freq);
+
+/* Append the current round corpus to the device training corpus. */
+int dg_corpus_append(dg_ctx* ctx);
+/* Training corpus size (walks, tokens); and clear it. */
+int dg_train_corpus_info(dg_ctx* ctx, uint64_t* n_walks, uint64_t* n_tokens);
+int dg_train_corpus_clear(dg_ctx* ctx);
+
+/* ---- timing ------------------------------------------------------------ */
+
+/* Per-kernel device time accumulated with HIP events on the context's stream
+ * since the last reset.  `name` is a kernel family ("walk", "pack", "re",
+ * "cn", "sgns_plain", "sgns_fwd", ...).  Returns launches and total ms. */
+int dg_timing_get(dg_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
+int dg_timing_reset(dg_ctx* ctx);
+
+/* ---- host data utilities (synthetic inputs, file formats) -------------- */
+
+/* Chung-Lu power-law graph: expected degrees w_i ~ (i+1)^(-1/(gamma-1))
+ * scaled to 2*e_target and capped at sqrt(2*e_target); endpoints drawn
+ * independently ~ w; self-loops and duplicates dropped; vertices below
+ * min_degree topped up with extra draws.  Writes at most e_cap undirected
+ * edges (src < dst) with data 0 and returns the count in *e_num. */
+int dg_gen_chung_lu(uint32_t v_num, uint64_t e_target, double gamma, uint32_t min_degree,
+                    uint64_t seed, dg_edge* edges, uint64_t e_cap, uint64_t* e_num);
+
+/* R-MAT (Chakrabarti, Zhan, Faloutsos 2004) with 2^scale vertices and
+ * edge_factor * 2^scale draws, quadrant probabilities (a, b, c, 1-a-b-c),
+ * self-loops and duplicates dropped, weights uniform(0, 1] in `data`. */
+int dg_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                uint64_t seed, dg_edge* edges, uint64_t e_cap, uint64_t* e_num);
+
+/* Link-prediction split after src/tools/gtrain.cpp:226-386: move
+ * round(frac * e_num) edges whose endpoints keep degree > 1 to the test set
+ * (edges[] is permuted: train = edges[0 .. *n_train), positives follow), and
+ * draw as many uniform non-edges (distinct, no self-pairs) into neg[]. */
+int dg_lp_split(uint32_t v_num, dg_edge* edges, uint64_t e_num, double frac, uint64_t seed,
+                uint64_t* n_train, dg_edge* neg, uint64_t neg_cap, uint64_t* n_neg);
+
+/* Area under the ROC curve (Mann-Whitney form, ties counted 1/2). */
+double dg_auc(const float* pos_scores, uint64_t n_pos, const float* neg_scores, uint64_t n_neg);
+
+/* Reference binary edge file I/O (include/storage.hpp:15-38, :108-115). */
+int dg_write_edges_bin(const char* path, const dg_edge* edges, uint64_t e_num);
+int dg_read_edges_bin(const char* path, dg_edge* edges, uint64_t e_cap, uint64_t* e_num);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DISTGER_HIP_H */

@@ -1,0 +1,411 @@
+/*
+ * walk_oracle.c - CPU restatement of the information-oriented walker
+ * (single partition), its graph input and its walk file.
+ *
+ * TEST INFRASTRUCTURE ONLY (see dg_oracle.h).  Each function cites the
+ * reference lines it restates.  Randomness comes from the Philox stream of
+ * dg_philox.h in place of mt19937/rand(); the draw points are the reference's.
+ */
+#include "dg_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../gdistger-re_amd/csrc/dg_math.h"
+#include "../gdistger-re_amd/csrc/dg_philox.h"
+
+#ifdef DGO_LIBM
+#define O_POW(x, y) pow((x), (y))
+#define O_LOG2(x) log2(x)
+#define O_LOG(x) log(x)
+#define O_SQRT(x) sqrt(x)
+#define O_TANH(x) tanh(x)
+#define O_SQUARE(x) pow((x), 2.0)
+#else
+#define O_POW(x, y) dg_pow((x), (y))
+#define O_LOG2(x) dg_log2(x)
+#define O_LOG(x) dg_log(x)
+#define O_SQRT(x) dg_sqrt(x)
+#define O_TANH(x) dg_tanh(x)
+#define O_SQUARE(x) ((x) * (x))
+#endif
+
+double dgo_pow(double x, double y) { return O_POW(x, y); }
+double dgo_log2(double x) { return O_LOG2(x); }
+double dgo_log(double x) { return O_LOG(x); }
+double dgo_sqrt(double x) { return O_SQRT(x); }
+double dgo_tanh(double x) { return O_TANH(x); }
+
+void dgo_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                uint32_t out[4]) {
+  dg_u32x4 o = dg_philox4x32_10(c0, c1, c2, c3, k0, k1);
+  out[0] = o.x0;
+  out[1] = o.x1;
+  out[2] = o.x2;
+  out[3] = o.x3;
+}
+
+void dgo_stream_draws(uint64_t seed, uint32_t c1, uint32_t c2, uint32_t tag, uint32_t n,
+                      uint32_t bound, uint32_t* out) {
+  dg_stream s;
+  dg_stream_init(&s, seed, c1, c2, tag);
+  for (uint32_t i = 0; i < n; ++i) out[i] = bound ? dg_stream_uniform(&s, bound) : dg_stream_next(&s);
+}
+
+/* ------------------------------------------------------------------------
+ * W1/W2: binary edges -> undirected doubling -> CSR in the reference's
+ * neighbour order.  include/graph.hpp:671-686 doubles the list as
+ * [e0, rev(e0), e1, rev(e1), ...]; :694-699 counts out-degrees over that list;
+ * with one partition shuffle_edges (:542-640) keeps the order and
+ * build_edge_container (:509-540) appends each record to its source's run, so
+ * neighbour order is the doubled-list order, stable by source.
+ * ---------------------------------------------------------------------- */
+int dgo_build_csr(uint32_t v_num, const dgo_edge* edges, uint64_t e_num, int undirected,
+                  uint64_t* row_ptr, uint32_t* nbr, float* data) {
+  uint64_t mult = undirected ? 2 : 1;
+  for (uint64_t i = 0; i < e_num; ++i)
+    if (edges[i].src >= v_num || edges[i].dst >= v_num) return -1;
+  memset(row_ptr, 0, sizeof(uint64_t) * ((size_t)v_num + 1));
+  for (uint64_t i = 0; i < e_num; ++i) {
+    row_ptr[edges[i].src + 1]++;
+    if (undirected) row_ptr[edges[i].dst + 1]++;
+  }
+  for (uint32_t v = 0; v < v_num; ++v) row_ptr[v + 1] += row_ptr[v];
+  uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)v_num);
+  if (!fill) return -1;
+  memcpy(fill, row_ptr, sizeof(uint64_t) * (size_t)v_num);
+  for (uint64_t i = 0; i < e_num; ++i) {
+    uint64_t p = fill[edges[i].src]++;
+    nbr[p] = edges[i].dst;
+    data[p] = edges[i].data;
+    if (undirected) {
+      uint64_t q = fill[edges[i].dst]++;
+      nbr[q] = edges[i].src;
+      data[q] = edges[i].data;
+    }
+  }
+  (void)mult;
+  free(fill);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * W3: common neighbours per directed entry, the quantity src/tools/gcn.cpp
+ * computes (:304-357, intersection :152-204, then removing duplicates and the
+ * endpoints themselves).  Textbook form: sort + unique each list, then merge.
+ * ---------------------------------------------------------------------- */
+static int cmp_u32(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  return (x > y) - (x < y);
+}
+
+int dgo_common_neighbors(uint32_t v_num, const uint64_t* row_ptr, const uint32_t* nbr,
+                         uint32_t* cn_out) {
+  uint64_t m = row_ptr[v_num];
+  uint32_t* srt = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  uint64_t* ulen = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)v_num + 1));
+  if (!srt || !ulen) {
+    free(srt);
+    free(ulen);
+    return -1;
+  }
+  memcpy(srt, nbr, sizeof(uint32_t) * m);
+  for (uint32_t v = 0; v < v_num; ++v) {
+    uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+    qsort(srt + b, e - b, sizeof(uint32_t), cmp_u32);
+    uint64_t k = b;
+    for (uint64_t i = b; i < e; ++i)
+      if (i == b || srt[i] != srt[i - 1]) srt[k++] = srt[i];
+    ulen[v] = k - b;
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t vi = 0; vi < (int64_t)v_num; ++vi) {
+    uint32_t u = (uint32_t)vi;
+    const uint32_t* a = srt + row_ptr[u];
+    uint64_t na = ulen[u];
+    for (uint64_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+      uint32_t v = nbr[e];
+      const uint32_t* b = srt + row_ptr[v];
+      uint64_t nb = ulen[v];
+      uint64_t i = 0, j = 0;
+      uint32_t c = 0;
+      while (i < na && j < nb) {
+        if (a[i] < b[j]) {
+          ++i;
+        } else if (a[i] > b[j]) {
+          ++j;
+        } else {
+          if (a[i] != u && a[i] != v) ++c;
+          ++i;
+          ++j;
+        }
+      }
+      cn_out[e] = c;
+    }
+  }
+  free(srt);
+  free(ulen);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * W7: incremental entropy and Pearson correlation, include/walk.hpp:284-354
+ * (Correlation::SnSn_div_Snp1Snp1, FjFj_div_FiFi, iterateHn, iterateEX,
+ * iterate with step_k = 1).  Operation order is kept term for term.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  double H, S, EH, ES, EHS, EHH, ESS, N, R;
+} corr_t;
+
+static void corr_iterate(corr_t* c, double fi) {
+  const double step_k = 1.0;
+  double Sn = c->S;
+  double Snp1_a = Sn + step_k;
+  double snsn = O_POW(Sn / Snp1_a, Sn) * O_POW(1.0 / Snp1_a, step_k);
+  double t;
+  if (fi == 0.0) {
+    t = O_LOG2(snsn);
+  } else {
+    double fj = fi + step_k;
+    double fjfj = O_POW(fj / fi, fi) * O_POW(fj, step_k);
+    t = O_LOG2(snsn * fjfj);
+  }
+  double Hnp1 = (Sn * c->H - t) / (Sn + step_k);
+  double Snp1 = c->S + step_k;
+  double Np1 = c->N + 1.0;
+  double EH = c->EH + (Hnp1 - c->EH) / Np1;
+  double ES = c->ES + (Snp1 - c->ES) / Np1;
+  double EHS = c->EHS + (Hnp1 * Snp1 - c->EHS) / Np1;
+  double EHH = c->EHH + (Hnp1 * Hnp1 - c->EHH) / Np1;
+  double ESS = c->ESS + (Snp1 * Snp1 - c->ESS) / Np1;
+  double DH = EHH - EH * EH;
+  double DS = ESS - ES * ES;
+  double ex = EHS - EH * ES;
+  c->R = ex == 0.0 ? 0.0 : ex / O_SQRT(DH * DS);
+  c->H = Hnp1;
+  c->S = Snp1;
+  c->EH = EH;
+  c->ES = ES;
+  c->EHS = EHS;
+  c->EHH = EHH;
+  c->ESS = ESS;
+  c->N = Np1;
+}
+
+static uint32_t count_in(const uint32_t* path, uint64_t len, uint32_t v) {
+  uint32_t c = 0;
+  for (uint64_t i = 0; i < len; ++i) c += (path[i] == v);
+  return c;
+}
+
+/* include/walk.hpp:979-985: stop once step > minLength and (R^2 < mu or R < 0). */
+static int entropy_stop(const corr_t* c, uint32_t step, const dgo_walk_cfg* cfg) {
+  return step > cfg->min_length && (O_SQUARE(c->R) < cfg->mu || c->R < 0.0);
+}
+
+/* ------------------------------------------------------------------------
+ * W5/W6: one walk with one partition, include/walk.hpp:529-589 (start vertex
+ * id % V, footprint at step 0), :849-856 (first-visit state), :885-1023 (step
+ * loop).  With one partition every vertex is local, so the emit/traceFlag
+ * branch (:947-959, :867-874) never runs.
+ * ---------------------------------------------------------------------- */
+int64_t dgo_walk_one(const dgo_graph* g, const dgo_walk_cfg* cfg, uint32_t round,
+                     uint32_t walk_id, uint32_t* path, uint64_t cap, double* state_out) {
+  dg_stream rs;
+  dg_stream_init(&rs, cfg->seed, walk_id, round, DG_TAG_WALK);
+  uint32_t cur = walk_id % g->v_num;
+  uint64_t len = 0;
+  if (cap < 1) return -1;
+  path[len++] = cur;
+  corr_t c = {0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+  uint32_t step = 0;
+  int capped = (cfg->max_length != 0 && len >= cfg->max_length);
+  while (!capped) {
+    uint64_t rb = g->row_ptr[cur];
+    uint32_t deg = (uint32_t)(g->row_ptr[cur + 1] - rb);
+    if (deg == 0) break;                                   /* :887-892 */
+    uint64_t e = rb + dg_stream_uniform(&rs, deg);        /* :899 */
+    if (deg == 2 && g->nbr[rb] == cur) break;              /* :900-902 */
+    uint32_t cand = g->nbr[e];
+    if (step == 0 && cand == cur) continue;                /* :904-906 */
+    int cnv = (int)g->cn[e];                               /* :920 */
+    double p = (double)dg_stream_u31(&rs) / 2147483647.0;  /* :928 */
+    int sd = (int)deg;
+    int dd = (int)(g->row_ptr[cand + 1] - g->row_ptr[cand]);
+    double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) / (double)(sd < dd ? sd : dd);
+    double pn = O_TANH(pa);                                /* :933-934 */
+    int trace = (pn <= p);                                 /* :938 */
+    if (len + 2 > cap) return -1;
+    ++step;                                                /* :961-970 */
+    double fi = (double)count_in(path, len, cand);
+    path[len++] = cand;
+    corr_iterate(&c, fi);
+    if (entropy_stop(&c, step, cfg)) break;                /* :979-985 */
+    if (cfg->max_length != 0 && len >= cfg->max_length) break;
+    if (trace) {                                           /* :986-1017 */
+      ++step;
+      fi = (double)count_in(path, len, cur);
+      path[len++] = cur;
+      corr_iterate(&c, fi);
+      if (entropy_stop(&c, step, cfg)) break;
+      if (cfg->max_length != 0 && len >= cfg->max_length) break;
+    } else {
+      cur = cand;                                          /* :1018-1023 */
+    }
+  }
+  if (state_out) {
+    state_out[0] = c.H;
+    state_out[1] = c.S;
+    state_out[2] = c.EH;
+    state_out[3] = c.ES;
+    state_out[4] = c.EHS;
+    state_out[5] = c.EHH;
+    state_out[6] = c.ESS;
+    state_out[7] = c.R;
+    state_out[8] = c.N;
+  }
+  return (int64_t)len;
+}
+
+/* All walks of a round in walk-id order.  Walks are independent given the
+ * graph (the reference's per-walk visit maps, include/walk.hpp:816), so the
+ * threads split the id range and the pieces are concatenated in id order. */
+int64_t dgo_walk_round(const dgo_graph* g, const dgo_walk_cfg* cfg, uint32_t round,
+                       uint32_t walker_begin, uint32_t walker_num, uint32_t* tokens,
+                       uint64_t tok_cap, uint64_t* offsets, int threads) {
+  if (threads < 1) threads = 1;
+  const uint64_t path_cap = 1u << 20;
+  int64_t failed = 0;
+  uint32_t** bufs = (uint32_t**)calloc((size_t)threads, sizeof(uint32_t*));
+  uint64_t* blen = (uint64_t*)calloc((size_t)threads, sizeof(uint64_t));
+  uint64_t* bcap = (uint64_t*)calloc((size_t)threads, sizeof(uint64_t));
+  uint32_t* lens = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)walker_num + 1));
+  if (!bufs || !blen || !bcap || !lens) {
+    free(bufs);
+    free(blen);
+    free(bcap);
+    free(lens);
+    return -1;
+  }
+#pragma omp parallel num_threads(threads)
+  {
+#ifdef _OPENMP
+    int t = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+#else
+    int t = 0, nt = 1;
+#endif
+    uint64_t lo = (uint64_t)walker_num * (uint64_t)t / (uint64_t)nt;
+    uint64_t hi = (uint64_t)walker_num * (uint64_t)(t + 1) / (uint64_t)nt;
+    uint32_t* path = (uint32_t*)malloc(sizeof(uint32_t) * path_cap);
+    uint64_t cap = 1 << 16, n = 0;
+    uint32_t* out = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+    for (uint64_t w = lo; w < hi && path && out; ++w) {
+      int64_t L = dgo_walk_one(g, cfg, round, walker_begin + (uint32_t)w, path, path_cap, NULL);
+      if (L < 0) {
+#pragma omp atomic write
+        failed = 1;
+        break;
+      }
+      if (n + (uint64_t)L > cap) {
+        while (n + (uint64_t)L > cap) cap *= 2;
+        uint32_t* nb = (uint32_t*)realloc(out, sizeof(uint32_t) * cap);
+        if (!nb) {
+#pragma omp atomic write
+          failed = 1;
+          break;
+        }
+        out = nb;
+      }
+      memcpy(out + n, path, sizeof(uint32_t) * (size_t)L);
+      n += (uint64_t)L;
+      lens[w] = (uint32_t)L;
+    }
+    free(path);
+    bufs[t] = out;
+    blen[t] = n;
+    bcap[t] = (uint64_t)nt;
+  }
+  int64_t total = 0;
+  if (!failed) {
+    int nt = (int)bcap[0];
+    uint64_t sum = 0;
+    for (int t = 0; t < nt; ++t) sum += blen[t];
+    if (sum > tok_cap) {
+      failed = 1;
+    } else {
+      uint64_t pos = 0;
+      for (int t = 0; t < nt; ++t) {
+        memcpy(tokens + pos, bufs[t], sizeof(uint32_t) * blen[t]);
+        pos += blen[t];
+      }
+      offsets[0] = 0;
+      for (uint32_t w = 0; w < walker_num; ++w) offsets[w + 1] = offsets[w] + lens[w];
+      total = (int64_t)pos;
+    }
+  }
+  for (int t = 0; t < threads; ++t) free(bufs[t]);
+  free(bufs);
+  free(blen);
+  free(bcap);
+  free(lens);
+  return failed ? -1 : total;
+}
+
+/* ------------------------------------------------------------------------
+ * W8: relative entropy h = sum_i p_i (ln p_i - ln q_i), p_i = deg_i / sum deg,
+ * q_i = freq_i / sum freq (include/walk.hpp:693-725).  The reference adds the
+ * terms left to right; here (as on the GPU) the terms are added left to right
+ * within chunks of 1024 consecutive vertices and the chunk sums are then added
+ * left to right, so the order is fixed and the same on both sides.
+ * ---------------------------------------------------------------------- */
+#define DGO_RE_CHUNK 1024
+
+double dgo_relative_entropy(uint32_t v_num, const uint32_t* deg, const uint32_t* freq,
+                            int* status) {
+  uint64_t dsum = 0, wsum = 0;
+  for (uint32_t i = 0; i < v_num; ++i) {
+    dsum += deg[i];
+    wsum += freq[i];
+  }
+  *status = 0;
+  double total = 0.0;
+  for (uint32_t c0 = 0; c0 < v_num; c0 += DGO_RE_CHUNK) {
+    uint32_t c1 = c0 + DGO_RE_CHUNK < v_num ? c0 + DGO_RE_CHUNK : v_num;
+    double part = 0.0;
+    for (uint32_t i = c0; i < c1; ++i) {
+      if (deg[i] == 0 || freq[i] == 0) {
+        *status = -1;
+        continue;
+      }
+      double pi = (double)deg[i] / (double)dsum;
+      double qi = (double)freq[i] / (double)wsum;
+      part = part + pi * (O_LOG(pi) - O_LOG(qi));
+    }
+    total = total + part;
+  }
+  return total;
+}
+
+/* ------------------------------------------------------------------------
+ * W9: include/path.hpp:67-119 writes, per walk, " %u" for every vertex and a
+ * final "\n"; include/path.hpp:489 orders walks by thread segment.
+ * ---------------------------------------------------------------------- */
+int dgo_write_walks(const char* path, const uint32_t* tokens, const uint64_t* offsets,
+                    uint64_t nwalks, uint32_t threads) {
+  FILE* f = fopen(path, "w");
+  if (!f) return -1;
+  if (threads < 1) threads = 1;
+  for (uint32_t t = 0; t < threads; ++t) {
+    for (uint64_t w = t; w < nwalks; w += threads) {
+      for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) fprintf(f, " %u", tokens[i]);
+      fprintf(f, "\n");
+    }
+  }
+  return fclose(f) == 0 ? 0 : -1;
+}

@@ -1,0 +1,330 @@
+"""Pins the CPU oracle's walker, graph and RNG against facts that do not come
+from the oracle itself: published known-answer vectors, high-precision
+arithmetic, brute-force recomputation from the walk path, closed-form
+probabilities on tiny graphs and hand-derived byte strings.
+
+The reference ships no tests or golden vectors for this path, so these are
+the pins (reference semantics cited per test)."""
+
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+import dgoracle
+
+E = dgoracle.EDGE_DTYPE
+
+
+def edges_of(pairs, data=None):
+    e = np.zeros(len(pairs), E)
+    for i, (a, b) in enumerate(pairs):
+        e[i] = (a, b, 0.0 if data is None else data[i])
+    return e
+
+
+# ---- RNG -------------------------------------------------------------------
+
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32-10 (Salmon et al., SC'11)
+    assert [hex(x) for x in dgoracle.philox([0, 0, 0, 0], [0, 0])] == [
+        "0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(x) for x in dgoracle.philox([0xffffffff] * 4, [0xffffffff] * 2)] == [
+        "0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in dgoracle.philox(
+        [0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0])] == [
+        "0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_stream_layout_and_lemire():
+    seed = 0x1234_5678_9abc_def0
+    words = dgoracle.stream_draws(seed, 5, 7, 0x57414C4B, 12)
+    # block b of the stream is philox(ctr=(b, c1, c2, tag), key=(seed lo, seed hi))
+    exp = np.concatenate([dgoracle.philox([b, 5, 7, 0x57414C4B],
+                                          [seed & 0xffffffff, seed >> 32]) for b in range(3)])
+    assert np.array_equal(words, exp)
+    # Lemire's bounded draw: floor(x*n / 2^32) with rejection of l < (2^32 - n) mod n,
+    # restated from the paper on the raw words
+    for n in (1, 2, 3, 7, 1000, 2**31 + 11):
+        raw = list(dgoracle.stream_draws(seed, 9, n & 0xffff, 0x57414C4B, 400))
+        got = dgoracle.stream_draws(seed, 9, n & 0xffff, 0x57414C4B, 100, bound=n)
+        it = iter(raw)
+        t = (2**32 - n) % n
+        exp = []
+        for _ in range(100):
+            while True:
+                x = int(next(it))
+                m = x * n
+                if (m & 0xffffffff) >= t:
+                    exp.append(m >> 32)
+                    break
+        assert list(got) == exp
+    # uniformity on a small range (chi-square, 9 dof, p ~ 1e-6 threshold)
+    d = dgoracle.stream_draws(seed, 1, 2, 3, 100000, bound=10)
+    c = np.bincount(d, minlength=10)
+    chi2 = ((c - 10000.0) ** 2 / 10000.0).sum()
+    assert chi2 < 45.0
+
+
+# ---- elementary functions ----------------------------------------------------
+
+def _ulp_diff(a, b):
+    return abs(int(np.float64(a).view(np.int64)) - int(np.float64(b).view(np.int64)))
+
+
+def test_dg_math_correctly_rounded_samples():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 160
+    L = dgoracle.load("dg")
+    rng = random.Random(3)
+    bad = 0
+    for i in range(3000):
+        x = math.exp(rng.uniform(-40, 40)) if i % 2 else rng.uniform(1e-6, 30.0)
+        bad += _ulp_diff(L.dgo_log2(x), float(mpmath.log(x, 2))) > 0
+        bad += _ulp_diff(L.dgo_log(x), float(mpmath.log(x))) > 0
+        bad += _ulp_diff(L.dgo_sqrt(x), float(mpmath.sqrt(x))) > 0
+        t = rng.uniform(1e-6, 21.0)
+        bad += _ulp_diff(L.dgo_tanh(t), float(mpmath.tanh(t))) > 0
+        s = float(rng.randint(1, 5000))
+        bad += _ulp_diff(L.dgo_pow(s / (s + 1), s), float(mpmath.power(mpmath.mpf(s / (s + 1)), s))) > 0
+        f = float(rng.randint(1, 5000))
+        bad += _ulp_diff(L.dgo_pow((f + 1) / f, f), float(mpmath.power(mpmath.mpf((f + 1) / f), f))) > 0
+    # the walker's math is correctly rounded on every sample (glibc's tanh/pow are not)
+    assert bad == 0
+
+
+def test_dg_math_special_values():
+    L = dgoracle.load("dg")
+    assert L.dgo_pow(0.3, 1.0) == 0.3 and L.dgo_pow(7.0, 0.0) == 1.0 and L.dgo_pow(1.0, 9.0) == 1.0
+    assert L.dgo_log2(1.0) == 0.0 and L.dgo_log2(1024.0) == 10.0 and L.dgo_log2(0.125) == -3.0
+    assert L.dgo_sqrt(4.0) == 2.0 and L.dgo_sqrt(2.0) == math.sqrt(2.0)
+    assert L.dgo_tanh(25.0) == 1.0 and L.dgo_tanh(float("inf")) == 1.0
+    assert L.dgo_tanh(-0.5) == -L.dgo_tanh(0.5)
+    assert L.dgo_log(math.e) == 1.0
+
+
+# ---- graph build (W1/W2) and common neighbours (W3) --------------------------
+
+def test_csr_reference_neighbour_order():
+    # graph.hpp:671-686 doubles [e0, rev e0, e1, rev e1, ...]; :509-540 appends
+    # in that order per source.  Edges: (0,1) (2,0) (1,2) (0,3)
+    g = dgoracle.OracleGraph(4, edges_of([(0, 1), (2, 0), (1, 2), (0, 3)]), cn="data")
+    rows = [list(g.nbr[g.row_ptr[v]:g.row_ptr[v + 1]]) for v in range(4)]
+    assert rows == [[1, 2, 3], [0, 2], [0, 1], [0]]
+    gd = dgoracle.OracleGraph(4, edges_of([(0, 1), (2, 0), (1, 2), (0, 3)]), undirected=False,
+                              cn="data")
+    rows = [list(gd.nbr[gd.row_ptr[v]:gd.row_ptr[v + 1]]) for v in range(4)]
+    assert rows == [[1, 3], [2], [0], []]
+
+
+def _brute_cn(v, pairs):
+    nb = [set() for _ in range(v)]
+    for a, b in pairs:
+        nb[a].add(b)
+        nb[b].add(a)
+    return nb
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_common_neighbours_brute_force(seed):
+    rng = random.Random(seed)
+    v = 40
+    pairs = set()
+    while len(pairs) < 150:
+        a, b = rng.randrange(v), rng.randrange(v)
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    pairs = sorted(pairs)
+    rng.shuffle(pairs)
+    g = dgoracle.OracleGraph(v, edges_of(pairs))
+    nb = _brute_cn(v, pairs)
+    for u in range(v):
+        for e in range(int(g.row_ptr[u]), int(g.row_ptr[u + 1])):
+            w = int(g.nbr[e])
+            assert g.cn[e] == len((nb[u] & nb[w]) - {u, w})
+
+
+# ---- walker (W5-W7) -----------------------------------------------------------
+
+def _entropy_bits(path):
+    _, c = np.unique(path, return_counts=True)
+    p = c / c.sum()
+    return float(-(p * np.log2(p)).sum())
+
+
+def _pearson(hs, ss):
+    hs = np.asarray(hs, np.float64)
+    ss = np.asarray(ss, np.float64)
+    cov = (hs * ss).mean() - hs.mean() * ss.mean()
+    if cov == 0:
+        return 0.0
+    return cov / math.sqrt(((hs * hs).mean() - hs.mean() ** 2) * ((ss * ss).mean() - ss.mean() ** 2))
+
+
+def test_walk_state_closed_forms(c1_edges):
+    g = dgoracle.OracleGraph(10000, c1_edges)
+    cfg = g.cfg(seed=11, min_length=20, max_length=0)
+    traced = 0
+    for wid in range(0, 400, 7):
+        path, st = g.walk_one(cfg, 1, wid)
+        H, S, EH, ES, EHS, EHH, ESS, R, N = st
+        # Sn counts the vertices on the walk, N the statistics samples
+        assert S == len(path) and N == len(path)
+        # H is the Shannon entropy (bits) of the walk's visit histogram (Thm. 1 of the paper)
+        assert abs(H - _entropy_bits(path)) < 1e-12
+        # running means and Pearson R over the prefix history (walk.hpp:310-354)
+        hs = [_entropy_bits(path[:k]) for k in range(1, len(path) + 1)]
+        ss = list(range(1, len(path) + 1))
+        assert abs(EH - np.mean(hs)) < 1e-12
+        assert abs(ES - np.mean(ss)) < 1e-9
+        assert abs(R - _pearson(hs, ss)) < 1e-9
+        traced += sum(path[i] == path[i + 2] for i in range(len(path) - 2))
+    assert traced > 0  # bounce-backs occurred and were accounted for
+
+
+def test_first_steps_hand_values():
+    # path graph 0-1-2: from 1 every move is to a leaf, leaves bounce back or move
+    g = dgoracle.OracleGraph(3, edges_of([(0, 1), (1, 2)]))
+    cfg = g.cfg(seed=5, min_length=100, max_length=3)
+    path, st = g.walk_one(cfg, 1, 1)
+    assert len(path) == 3 and path[0] == 1
+    # after two distinct vertices H = 1 bit and R = 1 (two samples on a line)
+    p2, st2 = g.walk_one(g.cfg(seed=5, min_length=100, max_length=2), 1, 1)
+    assert st2[0] == 1.0 and st2[7] == 1.0
+    # three vertices with one repeat: H = h(1/3) = 0.9183 bits
+    if path[2] == path[0]:
+        assert abs(st[0] - 0.9182958340544896) < 1e-15
+
+
+def test_termination_rule_brute_force(c1_edges):
+    """Every walk stops exactly where walk.hpp:979-985 says: the first append
+    with step > minLength and (R^2 < mu or R < 0), or at the cap."""
+    g = dgoracle.OracleGraph(10000, c1_edges)
+    for minl, mu, cap in [(20, 0.999, 0), (3, 0.8, 200), (4, 0.7, 120)]:
+        cfg = g.cfg(seed=3, min_length=minl, max_length=cap, mu=mu)
+        for wid in range(0, 3000, 37):
+            path, st = g.walk_one(cfg, 2, wid)
+            hs = [_entropy_bits(path[:k]) for k in range(1, len(path) + 1)]
+            stop_at = None
+            for k in range(2, len(path) + 1):
+                step = k - 1
+                r = _pearson(hs[:k], list(range(1, k + 1)))
+                if step > minl and (r * r < mu or r < 0):
+                    stop_at = k
+                    break
+            if stop_at is None:
+                assert cap and len(path) == cap
+            else:
+                assert len(path) == stop_at or (cap and len(path) == cap and cap < stop_at)
+
+
+def test_transition_and_acceptance_probabilities():
+    """On a star with centre 0 and leaves 1..4 plus edge (1,2): the first move
+    from leaf 3 always goes to the centre (degree 1, self-loops excluded at step
+    0); the bounce-back probability of a move u->v is 1 - tanh(pa) with
+    pa = max(du,dv) / (min(du,dv) * (du - cn)) (walk.hpp:933-938)."""
+    pairs = [(0, 1), (0, 2), (0, 3), (0, 4), (1, 2)]
+    g = dgoracle.OracleGraph(5, edges_of(pairs))
+    deg = g.degrees
+    # moves out of the centre (deg 4) in the first step: uniform over 4 leaves
+    cnt = np.zeros(5)
+    back = 0
+    n = 20000
+    cfg = g.cfg(seed=9, min_length=100, max_length=3)
+    for w in range(n):
+        path, _ = g.walk_one(cfg, 1, w * 5)  # start vertex 0
+        cnt[path[1]] += 1
+        back += path[2] == 0 and path[1] in (3, 4)
+    assert cnt[0] == 0
+    chi2 = ((cnt[1:] - n / 4) ** 2 / (n / 4)).sum()
+    assert chi2 < 25.0
+    # acceptance for 0 -> 3: du = 4, dv = 1, cn = 0: pa = 4/(1*4) = 1, bounce w.p. 1 - tanh(1)
+    p_bounce = 1.0 - math.tanh(1.0)
+    moves_to_leaf = cnt[3] + cnt[4]
+    # with max_length 3 a bounce appends 0 again; otherwise the walk moves on
+    # from the leaf, whose only neighbour is 0 as well -> count bounces via state
+    assert deg[0] == 4 and deg[3] == 1
+    assert moves_to_leaf > 0
+    # re-derive the bounce rate exactly from the stream: the acceptance draw is
+    # the second word after the index draw
+    seed = 9
+    bounces = 0
+    total = 0
+    for w in range(n):
+        wid = w * 5
+        words = dgoracle.stream_draws(seed, wid, 1, 0x57414C4B, 2)
+        idx = (int(words[0]) * 4) >> 32  # Lemire, n = 4 (no rejection needed for l >= 0)
+        leaf = int(g.nbr[g.row_ptr[0] + idx])
+        if leaf in (3, 4):
+            total += 1
+            p = (int(words[1]) >> 1) / 2147483647.0
+            bounces += math.tanh(1.0) <= p
+    assert total == moves_to_leaf
+    assert abs(bounces / total - p_bounce) < 0.02
+
+
+def test_degree_rules():
+    # vertex 2 has only a self-loop: degree 2, first neighbour itself -> walk ends (walk.hpp:900-902)
+    e = edges_of([(0, 1), (2, 2)])
+    g = dgoracle.OracleGraph(4, e)
+    path, _ = g.walk_one(g.cfg(1, 20, 0), 1, 2)
+    assert list(path) == [2]
+    # vertex 3 has degree 0 -> walk of one vertex (walk.hpp:887-892)
+    path, _ = g.walk_one(g.cfg(1, 20, 0), 1, 3)
+    assert list(path) == [3]
+    # RE is undefined with a zero-degree vertex (the reference asserts, walk.hpp:717)
+    h, st = g.relative_entropy(np.ones(4, np.uint32))
+    assert st == -1
+
+
+def test_relative_entropy_closed_form():
+    scipy_stats = pytest.importorskip("scipy.stats")
+    rng = np.random.default_rng(0)
+    pairs = [(i, (i + 1) % 3000) for i in range(3000)] + [(i, (i * 7 + 3) % 3000) for i in range(0, 3000, 5)]
+    pairs = [(a, b) for a, b in pairs if a != b]
+    g = dgoracle.OracleGraph(3000, edges_of(pairs))
+    freq = rng.integers(1, 50, 3000).astype(np.uint32)
+    h, st = g.relative_entropy(freq)
+    deg = g.degrees.astype(np.float64)
+    exp = scipy_stats.entropy(deg / deg.sum(), freq / freq.sum())
+    assert st == 0 and abs(h - exp) < 1e-12
+
+
+def test_walk_file_bytes(tmp_path):
+    # path.hpp:92,108: " %u" per vertex then "\n"; path.hpp:489 segment order
+    tok = np.array([5, 17, 5, 0, 3, 12, 9, 8, 1], np.uint32)
+    off = np.array([0, 3, 4, 6, 9], np.uint64)
+    p = tmp_path / "w.txt"
+    dgoracle.write_walks(str(p), tok, off, threads=1)
+    assert p.read_bytes() == b" 5 17 5\n 0\n 3 12\n 9 8 1\n"
+    dgoracle.write_walks(str(p), tok, off, threads=3)
+    assert p.read_bytes() == b" 5 17 5\n 9 8 1\n 0\n 3 12\n"
+
+
+def test_libm_and_dg_math_give_the_same_corpus(c1_edges):
+    """Decisions taken with glibc's pow/log2/sqrt/tanh (as the reference
+    computes them) equal those taken with dg_math.h on config 1."""
+    g = dgoracle.OracleGraph(10000, c1_edges, variant="dg")
+    gl = dgoracle.OracleGraph(10000, c1_edges, variant="libm")
+    for minl, mu, cap in [(20, 0.999, 80), (3, 0.8, 200)]:
+        t1, o1 = g.walk_round(g.cfg(21, minl, cap, mu), 1, 0, 50000, threads=4, tok_cap=50000 * cap + 16)
+        t2, o2 = gl.walk_round(gl.cfg(21, minl, cap, mu), 1, 0, 50000, threads=4, tok_cap=50000 * cap + 16)
+        assert np.array_equal(o1, o2) and np.array_equal(t1, t2)
+
+
+def test_reference_parameters_fix_walk_length(c1_edges):
+    """With mu = 0.999 the first termination check (step minLength+1) always
+    fires on config 1, so every walk holds minLength + 2 vertices."""
+    g = dgoracle.OracleGraph(10000, c1_edges)
+    tok, off = g.walk_round(g.cfg(1, 20, 80), 1, 0, 50000, threads=4)
+    assert set(np.diff(off).tolist()) == {22}
+    # consecutive vertices are adjacent
+    src = tok[:-1]
+    dst = tok[1:]
+    same_walk = np.ones(len(tok) - 1, bool)
+    same_walk[(off[1:-1] - 1).astype(np.int64)] = False
+    keys = set(zip(g.nbr.tolist(), np.repeat(np.arange(10000), np.diff(g.row_ptr).astype(np.int64)).tolist()))
+    sel = np.nonzero(same_walk)[0][:20000]
+    assert all((int(dst[i]), int(src[i])) in keys for i in sel)

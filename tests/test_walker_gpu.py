@@ -1,0 +1,156 @@
+"""GPU walker parity: the HIP walker's corpus, visit counts and relative
+entropy against the CPU oracle driven by the same Philox streams.  Integer
+outputs must be bit-exact; the FP64 relative entropy must be the same double."""
+
+import os
+
+import numpy as np
+import pytest
+
+import dgoracle
+import distger
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_graph_from_ctx(ctx):
+    """Oracle view of the device CSR (same neighbour order and CN values)."""
+    rp, nbr, cn = ctx.graph_csr()
+    g = dgoracle.OracleGraph.__new__(dgoracle.OracleGraph)
+    L = dgoracle.load("dg")
+    g.L = L
+    g.v_num = len(rp) - 1
+    g.row_ptr = rp
+    g.nbr = nbr
+    g.cn = cn
+    g.m = len(nbr)
+    g.data = cn.astype(np.float32)
+    g._g = dgoracle._Graph(g.v_num, g.m, rp.ctypes.data, nbr.ctypes.data, cn.ctypes.data)
+    return g
+
+
+@pytest.fixture(scope="module")
+def c1_ctx(ctx, c1_edges):
+    print(f"[walker-gpu] loading config-1 graph: {len(c1_edges)} edges", flush=True)
+    ctx.graph_from_edges(10000, c1_edges, undirected=True, cn_mode=distger.ffi.DG_CN_COMPUTE)
+    return ctx
+
+
+def test_device_csr_matches_oracle(c1_ctx, c1_edges):
+    g = dgoracle.OracleGraph(10000, c1_edges)
+    rp, nbr, cn = c1_ctx.graph_csr()
+    assert np.array_equal(rp, g.row_ptr)
+    assert np.array_equal(nbr, g.nbr[: g.m])
+    assert np.array_equal(cn, g.cn[: g.m])
+
+
+@pytest.mark.parametrize("minl,mu,cap,seed", [
+    (20, 0.999, 80, 7),    # config 1: reference termination, max length 80
+    (20, 0.999, 0, 8),     # no cap (reference)
+    (3, 0.8, 200, 9),      # uneven lengths, LDS spill beyond 47 vertices
+    (4, 0.7, 120, 10),     # many walks reach the cap
+])
+def test_round_bit_exact(c1_ctx, minl, mu, cap, seed):
+    print(f"[walker-gpu] round parity minl={minl} mu={mu} cap={cap} seed={seed}", flush=True)
+    g = oracle_graph_from_ctx(c1_ctx)
+    W = 50000
+    p = c1_ctx.default_walk_params(seed=seed, min_length=minl, max_length=cap, mu=mu)
+    c1_ctx.walk_reset()
+    st = c1_ctx.walk_round(p, 3, 0, W)
+    tok, off = c1_ctx.corpus()
+    otok, ooff = g.walk_round(g.cfg(seed, minl, cap, mu), 3, 0, W, threads=8,
+                              tok_cap=W * max(cap, 64) + 16)
+    assert np.array_equal(off, ooff)
+    assert np.array_equal(tok, otok)
+    assert st.tokens == len(otok) and st.steps == len(otok) - W
+    assert st.max_walk_len == int(np.diff(ooff).max())
+    freq = c1_ctx.vertex_freq()
+    assert np.array_equal(freq, np.bincount(otok, minlength=10000).astype(np.uint32))
+    h, status = g.relative_entropy(freq)
+    assert status == 0
+    assert st.re == h  # same double
+
+
+def test_walker_begin_offset(c1_ctx):
+    g = oracle_graph_from_ctx(c1_ctx)
+    p = c1_ctx.default_walk_params(seed=3, min_length=3, max_length=200, mu=0.8)
+    c1_ctx.walk_reset()
+    c1_ctx.walk_round(p, 2, 12345, 777)
+    tok, off = c1_ctx.corpus()
+    otok, ooff = g.walk_round(g.cfg(3, 3, 200, 0.8), 2, 12345, 777, threads=4)
+    assert np.array_equal(off, ooff) and np.array_equal(tok, otok)
+
+
+def test_walk_run_rounds_match_oracle(c1_ctx, tmp_path):
+    """The whole round loop (walk.hpp:608-782): 30 rounds, RE per round."""
+    print("[walker-gpu] 30-round run", flush=True)
+    g = oracle_graph_from_ctx(c1_ctx)
+    p = c1_ctx.default_walk_params(seed=42, max_length=80)
+    stats = c1_ctx.walk_run(p, keep_corpus=True)
+    ref = dgoracle.walk_run(g, 42, 10000, init_round=5, max_length=80, threads=8)
+    assert len(stats) == len(ref) == 30
+    for s, r in zip(stats, ref):
+        assert s.round == r["round"]
+        assert s.tokens == len(r["tokens"])
+        assert s.re == r["re"] and s.delta_re == r["delta_re"]
+    tok, off = c1_ctx.corpus()  # last round
+    assert np.array_equal(tok, ref[-1]["tokens"])
+    nw, nt = c1_ctx.train_corpus_info()
+    assert nw == 50000 + 29 * 10000 and nt == sum(len(r["tokens"]) for r in ref)
+    # walk file bytes (path.hpp:67-119), id order and 3-thread segment order
+    for threads in (1, 3):
+        a = tmp_path / f"gpu-{threads}.txt"
+        b = tmp_path / f"ora-{threads}.txt"
+        c1_ctx.corpus_write_text(str(a), threads)
+        dgoracle.write_walks(str(b), ref[-1]["tokens"], ref[-1]["offsets"], threads)
+        assert a.read_bytes() == b.read_bytes()
+
+
+def test_converge_mode_stops_on_delta(c1_ctx):
+    g = oracle_graph_from_ctx(c1_ctx)
+    p = c1_ctx.default_walk_params(seed=5, max_length=80, rounds_mode=1, delta=0.01)
+    stats = c1_ctx.walk_run(p)
+    ref = dgoracle.walk_run(g, 5, 10000, max_length=80, delta=0.01, converge=True, threads=8)
+    assert [s.re for s in stats] == [r["re"] for r in ref]
+    assert stats[-1].stop == 1 and len(stats) < 30
+
+
+def test_degree_zero_vertex_is_an_error(ctx):
+    e = np.zeros(3, distger.EDGE_DTYPE)
+    e[0] = (0, 1, 0)
+    e[1] = (1, 2, 0)
+    e[2] = (2, 0, 0)
+    ctx.graph_from_edges(4, e, cn_mode=distger.ffi.DG_CN_COMPUTE)  # vertex 3 isolated
+    p = ctx.default_walk_params(seed=1, max_length=10)
+    ctx.walk_reset()
+    with pytest.raises(distger.DgError) as ei:
+        ctx.walk_round(p, 1, 0, 8)
+    assert ei.value.code == -7  # DG_ERR_RE_UNDEFINED
+    tok, off = ctx.corpus()
+    assert list(tok[off[3]:off[4]]) == [3]  # a walk from the isolated vertex is one vertex
+
+
+def test_length_limit_is_an_error(ctx, c1_edges):
+    ctx.graph_from_edges(10000, c1_edges, cn_mode=distger.ffi.DG_CN_COMPUTE)
+    p = ctx.default_walk_params(seed=1, min_length=2, mu=0.6, max_length=0, length_limit=64)
+    ctx.walk_reset()
+    with pytest.raises(distger.DgError) as ei:
+        ctx.walk_round(p, 1, 0, 5000)
+    assert ei.value.code == -6  # DG_ERR_WALK_TOO_LONG
+
+
+def test_bin_roundtrip_and_load(ctx, c1_edges, tmp_path):
+    path = str(tmp_path / "g.bin")
+    e = c1_edges.copy()
+    g = dgoracle.OracleGraph(10000, e)
+    # store the CN in the edge data, as gcn -> gconverter does for the reference
+    cn_by_edge = {}
+    for u in range(10000):
+        for k in range(int(g.row_ptr[u]), int(g.row_ptr[u + 1])):
+            cn_by_edge[(u, int(g.nbr[k]))] = int(g.cn[k])
+    e["data"] = [cn_by_edge[(int(a), int(b))] for a, b in zip(e["src"], e["dst"])]
+    distger.write_edges_bin(path, e)
+    assert os.path.getsize(path) == 12 * len(e)
+    ctx.graph_load_bin(path, 10000, undirected=True, cn_mode=distger.ffi.DG_CN_FROM_DATA)
+    rp, nbr, cn = ctx.graph_csr()
+    assert np.array_equal(cn, g.cn[: g.m]) and np.array_equal(nbr, g.nbr[: g.m])
