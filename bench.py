@@ -45,14 +45,16 @@ def build_workload(args):
     return train, pos, neg
 
 
-def walker_bytes(stats_list, draws):
-    """Algorithmic HBM bytes of the walker kernel (walker
-    roofline): per walk the start row (16 B), per candidate draw the chosen
-    8-byte adjacency unit and the candidate's row-offset pair (16 B), per
-    vertex written 4 B."""
+def walker_bytes(stats_list):
+    """Algorithmic HBM bytes of the walker kernel: 16 B per row-offset pair
+    read (walk start and every move), 8 B per adjacency unit drawn, 4 B per
+    vertex written, 8 B per visit-count update (read-modify-write of a
+    uint32), 12 B per walk record (offset + length)."""
     walks = sum(s.walkers for s in stats_list)
     tokens = sum(s.tokens for s in stats_list)
-    return 16 * walks + 24 * draws + 4 * tokens
+    draws = sum(s.draws for s in stats_list)
+    moves = sum(s.moves for s in stats_list)
+    return 16 * moves + 8 * draws + 4 * tokens + 8 * tokens + 12 * walks
 
 
 def cpu_baseline(ctx, seed, sample_walks, threads):
@@ -113,10 +115,9 @@ def main():
         _, pack_ms = ctx.timing("pack")
         _, re_ms = ctx.timing("re")
         steps = sum(s.steps for s in stats)
-        draws = steps  # one candidate draw per appended vertex that is not a bounce-back
         value = steps / wall
         # algorithmic bytes of the dominant kernel, per launch
-        per_launch_bytes = walker_bytes(stats, draws) / max(n_walk, 1)
+        per_launch_bytes = walker_bytes(stats) / max(n_walk, 1)
         avg_walk_s = walk_ms / 1e3 / max(n_walk, 1)
         achieved = per_launch_bytes / avg_walk_s / 1e9
         cpu = None

@@ -22,6 +22,8 @@ struct dg_walk_queue {
   unsigned int zero_terms;    // RE terms with p_i == 0 or q_i == 0
   unsigned int pad;
   double re;                  // relative entropy of the round
+  unsigned long long draws;   // candidate draws (adjacency units read)
+  unsigned long long moves;   // row-offset pairs read (walk starts + moves)
 };
 
 #define DG_QFLAG_TOKEN_OVERFLOW 1u
@@ -46,11 +48,13 @@ struct dg_ctx {
   hipStream_t stream = nullptr;
   std::string err;
 
-  // graph (device-resident CSR; adjacency units are {neighbour, common-neighbour count})
+  // graph (device-resident CSR)
   uint32_t v_num = 0;
   uint64_t m = 0;
   uint64_t* d_row_ptr = nullptr;
-  uint2* d_adj = nullptr;
+  uint32_t* d_nbr = nullptr;  // neighbour ids, reference order
+  uint32_t* d_cn = nullptr;   // common-neighbour counts
+  uint2* d_adj = nullptr;     // walker view {neighbour, bounce threshold}
   uint32_t* d_deg = nullptr;
 
   // walker buffers
@@ -72,6 +76,7 @@ struct dg_ctx {
   size_t spill_cap = 0;
   uint64_t* d_scan_tmp = nullptr;    // block sums for scans
   size_t scan_tmp_cap = 0;
+  double* d_term_table = nullptr;    // tabulated entropy terms for short walks
   double* d_re_terms = nullptr;      // v_num
   double* d_re_chunks = nullptr;
   size_t re_cap = 0;

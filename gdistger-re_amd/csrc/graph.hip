@@ -1,4 +1,5 @@
-// graph.hip - device-resident CSR for the walker.
+// graph.hip - device-resident CSR for the walker, common-neighbour counts and
+// per-entry acceptance thresholds.
 //
 // Replaces GraphEngine::load_graph (include/graph.hpp:642-824) for a single
 // partition: the whole graph lives in one GPU's HBM, so the MPI edge shuffle
@@ -9,112 +10,267 @@
 // (build_edge_container, :509-540).
 //
 // HBM layout
-//   d_row_ptr : uint64[V+1]   row offsets (edge_id_t)
-//   d_adj     : uint2[m]      {neighbour, common-neighbour count}; 8 bytes per
-//                             entry like the reference AdjUnit<real_t>
-//                             (include/graph.hpp:32-37), with the count stored
-//                             as an integer instead of a float
-//   d_deg     : uint32[V]     degree (= row length), read once per step
+//   d_row_ptr : uint64[V+1]  row offsets (edge_id_t)
+//   d_nbr     : uint32[m]    neighbour ids, reference order
+//   d_cn      : uint32[m]    common-neighbour count |N(u) & N(v) \ {u,v}|
+//   d_adj     : uint2[m]     walker view {neighbour, acceptance threshold}:
+//                            8 bytes per entry like the reference
+//                            AdjUnit<real_t> (include/graph.hpp:32-37)
+//   d_deg     : uint32[V]    degree (= row length)
+//
+// Common neighbours (src/tools/gcn.cpp:250-373, the offline tool whose
+// output the reference walker reads back as edge data, walk.hpp:920) are
+// computed on the device: one wavefront per source vertex hashes N(u) into
+// LDS, then probes each neighbour's list 64 entries at a time and counts hits
+// with a ballot; sources with more than kSmallDeg neighbours use a whole
+// workgroup and a larger LDS table, in chunks if needed.
+//
+// Acceptance threshold: the walker bounces back when tanh(pa) <= p with
+// p = u / (2^31 - 1), u the 31-bit draw (walk.hpp:928-938).  pa depends only
+// on (deg u, deg v, cn), so per entry the smallest u that satisfies the test
+// is found once by bisection (u / M is monotone in u under correctly rounded
+// division), and the walker tests `u >= threshold` - the same decision bit
+// for bit, without tanh or the neighbour's degree in the step loop.
 #include <algorithm>
 #include <cstring>
 #include <thread>
 
 #include "dg_internal.h"
+#include "dg_math.h"
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr int kSmallDeg = 1024;          // wave path: hash of 2048 slots per wave
+constexpr int kSmallSlots = 2048;
+constexpr int kBigSlots = 16384;         // block path: 64 KiB table, chunks of 8192
+constexpr int kBigChunk = 8192;
+
+__device__ __forceinline__ uint32_t slot_hash(uint32_t w, uint32_t mask) {
+  return (w * 2654435761u) & mask;
+}
+
+__device__ __forceinline__ void hash_insert(uint32_t* tbl, uint32_t mask, uint32_t w) {
+  uint32_t s = slot_hash(w, mask);
+  while (true) {
+    uint32_t prev = atomicCAS(&tbl[s], kEmpty, w);
+    if (prev == kEmpty || prev == w) return;
+    s = (s + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ bool hash_find(const uint32_t* tbl, uint32_t mask, uint32_t w) {
+  uint32_t s = slot_hash(w, mask);
+  while (true) {
+    uint32_t x = tbl[s];
+    if (x == w) return true;
+    if (x == kEmpty) return false;
+    s = (s + 1) & mask;
+  }
+}
+
+// One wavefront per source vertex with degree <= kSmallDeg.
+__global__ __launch_bounds__(256) void cn_small_kernel(const uint64_t* rp, const uint32_t* nbr,
+                                                       uint32_t v_num, uint32_t* cn) {
+  __shared__ uint32_t s_tbl[4][kSmallSlots];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t* tbl = s_tbl[wave];
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t u = blockIdx.x * 4 + wave; u < v_num; u += waves) {
+    const uint64_t b = rp[u], e = rp[u + 1];
+    const uint32_t du = (uint32_t)(e - b);
+    if (du > (uint32_t)kSmallDeg || du == 0) continue;  // wave-uniform
+    uint32_t slots = 64;
+    while (slots < 2 * du) slots <<= 1;
+    const uint32_t mask = slots - 1;
+    for (uint32_t i = lane; i < slots; i += 64) tbl[i] = kEmpty;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (uint32_t i = lane; i < du; i += 64) {
+      const uint32_t w = nbr[b + i];
+      if (w != u) hash_insert(tbl, mask, w);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint64_t k = b; k < e; ++k) {
+      const uint32_t v = nbr[k];
+      const uint64_t vb = rp[v], ve = rp[v + 1];
+      uint32_t c = 0;
+      for (uint64_t j0 = vb; j0 < ve; j0 += 64) {  // wave-uniform trip count
+        bool hit = false;
+        if (j0 + lane < ve) {
+          const uint32_t w = nbr[j0 + lane];
+          hit = (w != v) && (w != u) && hash_find(tbl, mask, w);
+        }
+        c += (uint32_t)__popcll(__ballot(hit));
+      }
+      if (lane == 0) cn[k] = c;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// One workgroup per source vertex with degree > kSmallDeg; N(u) is hashed in
+// chunks of kBigChunk entries and the counts of all chunks are added.
+__global__ __launch_bounds__(256) void cn_big_kernel(const uint64_t* rp, const uint32_t* nbr,
+                                                     const uint32_t* big, uint32_t n_big,
+                                                     uint32_t* cn) {
+  __shared__ uint32_t tbl[kBigSlots];
+  __shared__ uint32_t s_cnt[4];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
+    const uint32_t u = big[bi];
+    const uint64_t b = rp[u], e = rp[u + 1];
+    for (uint64_t c0 = b; c0 < e; c0 += kBigChunk) {
+      const uint64_t c1 = c0 + kBigChunk < e ? c0 + kBigChunk : e;
+      for (uint32_t i = threadIdx.x; i < (uint32_t)kBigSlots; i += 256) tbl[i] = kEmpty;
+      __syncthreads();
+      for (uint64_t i = c0 + threadIdx.x; i < c1; i += 256) {
+        const uint32_t w = nbr[i];
+        if (w != u) hash_insert(tbl, kBigSlots - 1, w);
+      }
+      __syncthreads();
+      // each wave takes neighbours k = b + wave, b + wave + 4, ...
+      for (uint64_t k = b + wave; k < e; k += 4) {
+        const uint32_t v = nbr[k];
+        const uint64_t vb = rp[v], ve = rp[v + 1];
+        uint32_t c = 0;
+        for (uint64_t j = vb; j < ve; j += 64) {
+          bool hit = false;
+          if (j + lane < ve) {
+            const uint32_t w = nbr[j + lane];
+            hit = (w != v) && (w != u) && hash_find(tbl, kBigSlots - 1, w);
+          }
+          c += (uint32_t)__popcll(__ballot(hit));
+        }
+        if (lane == 0) cn[k] = (c0 == b ? 0u : cn[k]) + c;
+      }
+      __syncthreads();
+    }
+  }
+  (void)s_cnt;
+}
+
+__global__ void degree_kernel(const uint64_t* rp, uint32_t v_num, uint32_t* deg) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < v_num; v += gridDim.x * blockDim.x)
+    deg[v] = (uint32_t)(rp[v + 1] - rp[v]);
+}
+
+// Smallest u in [0, 2^31 - 1] with tanh(pa) <= u / (2^31 - 1), pa as in
+// walk.hpp:933; 2^31 (never reached by a 31-bit draw) if no u qualifies.
+__device__ __forceinline__ uint32_t bounce_threshold(int sd, int dd, int cnv) {
+  const double M = 2147483647.0;
+  const double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) /
+                    (double)(sd < dd ? sd : dd);
+  const double pn = dg_tanh(pa);
+  if (pn != pn) return 0x80000000u;
+  if (!(pn <= 1.0)) return 0x80000000u;
+  uint32_t lo = 0, hi = 2147483647u;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (pn <= (double)mid / M)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ void threshold_kernel(const uint64_t* rp, const uint32_t* nbr, const uint32_t* cn,
+                                 const uint32_t* deg, uint32_t v_num, uint64_t m, uint2* adj) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    // source vertex of entry e: last u with rp[u] <= e
+    uint32_t lo = 0, hi = v_num - 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (rp[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint32_t v = nbr[e];
+    adj[e] = make_uint2(v, bounce_threshold((int)deg[lo], (int)deg[v], (int)cn[e]));
+  }
+}
+
+int grid_for(dg_ctx* ctx, uint64_t n, int block) {
+  uint64_t g = (n + block - 1) / block;
+  uint64_t cap = (uint64_t)ctx->num_cus * 32;
+  if (g > cap) g = cap;
+  return g == 0 ? 1 : (int)g;
+}
+
+}  // namespace
 
 void dg_graph_free(dg_ctx* ctx) {
-  if (ctx->d_row_ptr) hipFree(ctx->d_row_ptr);
-  if (ctx->d_adj) hipFree(ctx->d_adj);
-  if (ctx->d_deg) hipFree(ctx->d_deg);
+  void* bufs[] = {ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, ctx->d_adj, ctx->d_deg};
+  for (void* p : bufs)
+    if (p) hipFree(p);
   ctx->d_row_ptr = nullptr;
+  ctx->d_nbr = nullptr;
+  ctx->d_cn = nullptr;
   ctx->d_adj = nullptr;
   ctx->d_deg = nullptr;
   ctx->v_num = 0;
   ctx->m = 0;
 }
 
-static unsigned host_threads() {
-  unsigned n = std::thread::hardware_concurrency();
-  if (n == 0) n = 1;
-  return n > 16 ? 16 : n;
-}
-
-template <typename F>
-static void parallel_for(uint64_t n, F f) {
-  unsigned t = host_threads();
-  if (n < 4096 || t == 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (unsigned i = 0; i < t; ++i) {
-    uint64_t lo = n * i / t, hi = n * (i + 1) / t;
-    th.emplace_back([=] { f(lo, hi); });
-  }
-  for (auto& x : th) x.join();
-}
-
-// Common-neighbour counts |(N(u) & N(v)) \ {u, v}| per CSR entry, over
-// distinct neighbours (the quantity src/tools/gcn.cpp:304-357 writes into the
-// edge data).  Sorted unique copies of every row, then a merge per entry.
-static void host_common_neighbors(uint32_t v_num, const std::vector<uint64_t>& rp,
-                                  const std::vector<uint32_t>& nbr, std::vector<uint32_t>& cn) {
-  uint64_t m = rp[v_num];
-  std::vector<uint32_t> srt(nbr);
-  std::vector<uint32_t> ulen(v_num);
-  parallel_for(v_num, [&](uint64_t lo, uint64_t hi) {
-    for (uint64_t v = lo; v < hi; ++v) {
-      uint32_t* b = srt.data() + rp[v];
-      uint32_t* e = srt.data() + rp[v + 1];
-      std::sort(b, e);
-      ulen[v] = (uint32_t)(std::unique(b, e) - b);
-    }
-  });
-  cn.assign(m, 0);
-  parallel_for(v_num, [&](uint64_t lo, uint64_t hi) {
-    for (uint64_t u = lo; u < hi; ++u) {
-      const uint32_t* a = srt.data() + rp[u];
-      uint32_t na = ulen[u];
-      for (uint64_t e = rp[u]; e < rp[u + 1]; ++e) {
-        uint32_t v = nbr[e];
-        const uint32_t* b = srt.data() + rp[v];
-        uint32_t nb = ulen[v];
-        uint32_t i = 0, j = 0, c = 0;
-        while (i < na && j < nb) {
-          if (a[i] < b[j]) {
-            ++i;
-          } else if (a[i] > b[j]) {
-            ++j;
-          } else {
-            c += (a[i] != (uint32_t)u && a[i] != v);
-            ++i;
-            ++j;
-          }
-        }
-        cn[e] = c;
-      }
-    }
-  });
-}
-
-static int upload_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>& rp,
-                        const std::vector<uint32_t>& nbr, const std::vector<uint32_t>& cn) {
+// Device CSR from the host arrays; CN from the edge data or computed here.
+static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>& rp,
+                        const std::vector<uint32_t>& nbr, const std::vector<uint32_t>& cn_data,
+                        int cn_mode) {
   dg_graph_free(ctx);
-  uint64_t m = rp[v_num];
-  std::vector<uint2> adj(m);
-  for (uint64_t e = 0; e < m; ++e) adj[e] = make_uint2(nbr[e], cn[e]);
-  std::vector<uint32_t> deg(v_num);
-  for (uint32_t v = 0; v < v_num; ++v) deg[v] = (uint32_t)(rp[v + 1] - rp[v]);
+  const uint64_t m = rp[v_num];
   DG_HIP(ctx, hipMalloc((void**)&ctx->d_row_ptr, sizeof(uint64_t) * (v_num + 1)));
+  DG_HIP(ctx, hipMalloc((void**)&ctx->d_nbr, sizeof(uint32_t) * (m ? m : 1)));
+  DG_HIP(ctx, hipMalloc((void**)&ctx->d_cn, sizeof(uint32_t) * (m ? m : 1)));
   DG_HIP(ctx, hipMalloc((void**)&ctx->d_adj, sizeof(uint2) * (m ? m : 1)));
-  DG_HIP(ctx, hipMalloc((void**)&ctx->d_deg, sizeof(uint32_t) * (v_num ? v_num : 1)));
-  DG_HIP(ctx, hipMemcpy(ctx->d_row_ptr, rp.data(), sizeof(uint64_t) * (v_num + 1),
-                        hipMemcpyHostToDevice));
+  DG_HIP(ctx, hipMalloc((void**)&ctx->d_deg, sizeof(uint32_t) * v_num));
+  DG_HIP(ctx, hipMemcpyAsync(ctx->d_row_ptr, rp.data(), sizeof(uint64_t) * (v_num + 1),
+                             hipMemcpyHostToDevice, ctx->stream));
   if (m)
-    DG_HIP(ctx, hipMemcpy(ctx->d_adj, adj.data(), sizeof(uint2) * m, hipMemcpyHostToDevice));
-  if (v_num)
-    DG_HIP(ctx, hipMemcpy(ctx->d_deg, deg.data(), sizeof(uint32_t) * v_num, hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpyAsync(ctx->d_nbr, nbr.data(), sizeof(uint32_t) * m,
+                               hipMemcpyHostToDevice, ctx->stream));
   ctx->v_num = v_num;
   ctx->m = m;
+  degree_kernel<<<grid_for(ctx, v_num, 256), 256, 0, ctx->stream>>>(ctx->d_row_ptr, v_num,
+                                                                    ctx->d_deg);
+  DG_HIP(ctx, hipGetLastError());
+  if (cn_mode == DG_CN_COMPUTE) {
+    std::vector<uint32_t> big;
+    for (uint32_t v = 0; v < v_num; ++v)
+      if (rp[v + 1] - rp[v] > (uint64_t)kSmallDeg) big.push_back(v);
+    size_t tc;
+    DG_CHECK(dg_time_begin(ctx, "cn", &tc));
+    cn_small_kernel<<<grid_for(ctx, (uint64_t)v_num, 4), 256, 0, ctx->stream>>>(
+        ctx->d_row_ptr, ctx->d_nbr, v_num, ctx->d_cn);
+    DG_HIP(ctx, hipGetLastError());
+    if (!big.empty()) {
+      uint32_t* d_big = nullptr;
+      DG_HIP(ctx, hipMalloc((void**)&d_big, sizeof(uint32_t) * big.size()));
+      DG_HIP(ctx, hipMemcpyAsync(d_big, big.data(), sizeof(uint32_t) * big.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+      const unsigned g = (unsigned)std::min<size_t>(big.size(), (size_t)ctx->num_cus * 4);
+      cn_big_kernel<<<g, 256, 0, ctx->stream>>>(ctx->d_row_ptr, ctx->d_nbr, d_big,
+                                                (uint32_t)big.size(), ctx->d_cn);
+      DG_HIP(ctx, hipGetLastError());
+      DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      hipFree(d_big);
+    }
+    DG_CHECK(dg_time_end(ctx, tc));
+  } else if (m) {
+    DG_HIP(ctx, hipMemcpyAsync(ctx->d_cn, cn_data.data(), sizeof(uint32_t) * m,
+                               hipMemcpyHostToDevice, ctx->stream));
+  }
+  size_t tt;
+  DG_CHECK(dg_time_begin(ctx, "threshold", &tt));
+  if (m)
+    threshold_kernel<<<grid_for(ctx, m, 256), 256, 0, ctx->stream>>>(
+        ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, ctx->d_deg, v_num, m, ctx->d_adj);
+  DG_HIP(ctx, hipGetLastError());
+  DG_CHECK(dg_time_end(ctx, tt));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DG_OK;
 }
 
@@ -128,7 +284,8 @@ extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* e
     if (edges[i].src >= v_num || edges[i].dst >= v_num)
       return dg_fail(ctx, DG_ERR_ARG, "edge %llu (%u, %u) out of range for %u vertices",
                      (unsigned long long)i, edges[i].src, edges[i].dst, v_num);
-  uint64_t m = e_num * (undirected ? 2 : 1);
+  // include/graph.hpp:671-699 then :509-540: doubled list, counted, placed in order
+  const uint64_t m = e_num * (undirected ? 2 : 1);
   std::vector<uint64_t> rp(v_num + 1, 0);
   for (uint64_t i = 0; i < e_num; ++i) {
     rp[edges[i].src + 1]++;
@@ -136,23 +293,23 @@ extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* e
   }
   for (uint32_t v = 0; v < v_num; ++v) rp[v + 1] += rp[v];
   std::vector<uint64_t> fill(rp.begin(), rp.end() - 1);
-  std::vector<uint32_t> nbr(m), cn(m);
+  std::vector<uint32_t> nbr(m);
+  std::vector<uint32_t> cn;
+  if (cn_mode == DG_CN_FROM_DATA) cn.resize(m);
   for (uint64_t i = 0; i < e_num; ++i) {
-    // walk.hpp:920 converts the float edge data to int
-    uint32_t c = (uint32_t)(int)edges[i].data;
-    uint64_t p = fill[edges[i].src]++;
+    const uint64_t p = fill[edges[i].src]++;
     nbr[p] = edges[i].dst;
-    cn[p] = c;
+    // walk.hpp:920 converts the float edge data to int
+    const uint32_t c = cn_mode == DG_CN_FROM_DATA ? (uint32_t)(int)edges[i].data : 0u;
+    if (cn_mode == DG_CN_FROM_DATA) cn[p] = c;
     if (undirected) {
-      uint64_t q = fill[edges[i].dst]++;
+      const uint64_t q = fill[edges[i].dst]++;
       nbr[q] = edges[i].src;
-      cn[q] = c;
+      if (cn_mode == DG_CN_FROM_DATA) cn[q] = c;
     }
   }
-  if (cn_mode == DG_CN_COMPUTE) host_common_neighbors(v_num, rp, nbr, cn);
-  // reset walker state bound to the previous graph
-  dg_walk_free(ctx);
-  return upload_graph(ctx, v_num, rp, nbr, cn);
+  dg_walk_free(ctx);  // walker state belongs to the previous graph
+  return finish_graph(ctx, v_num, rp, nbr, cn, cn_mode);
 }
 
 extern "C" int dg_graph_load_bin(dg_ctx* ctx, const char* path, uint32_t v_num, int undirected,
@@ -168,9 +325,9 @@ extern "C" int dg_graph_load_bin(dg_ctx* ctx, const char* path, uint32_t v_num, 
     return dg_fail(ctx, DG_ERR_IO, "%s: size %ld is not a multiple of %zu", path, sz,
                    sizeof(dg_edge));
   }
-  uint64_t e_num = (uint64_t)sz / sizeof(dg_edge);
+  const uint64_t e_num = (uint64_t)sz / sizeof(dg_edge);
   std::vector<dg_edge> edges(e_num);
-  size_t got = e_num ? fread(edges.data(), sizeof(dg_edge), e_num, f) : 0;
+  const size_t got = e_num ? fread(edges.data(), sizeof(dg_edge), e_num, f) : 0;
   fclose(f);
   if (got != e_num) return dg_fail(ctx, DG_ERR_IO, "%s: short read", path);
   return dg_graph_from_edges(ctx, v_num, edges.data(), e_num, undirected, cn_mode);
@@ -191,13 +348,21 @@ extern "C" int dg_graph_get_csr(dg_ctx* ctx, uint64_t* row_ptr, uint32_t* nbr, u
   if (row_ptr)
     DG_HIP(ctx, hipMemcpy(row_ptr, ctx->d_row_ptr, sizeof(uint64_t) * (ctx->v_num + 1),
                           hipMemcpyDeviceToHost));
-  if ((nbr || cn) && ctx->m) {
-    std::vector<uint2> adj(ctx->m);
+  if (nbr && ctx->m)
+    DG_HIP(ctx, hipMemcpy(nbr, ctx->d_nbr, sizeof(uint32_t) * ctx->m, hipMemcpyDeviceToHost));
+  if (cn && ctx->m)
+    DG_HIP(ctx, hipMemcpy(cn, ctx->d_cn, sizeof(uint32_t) * ctx->m, hipMemcpyDeviceToHost));
+  return DG_OK;
+}
+
+extern "C" int dg_graph_get_thresholds(dg_ctx* ctx, uint32_t* thr) {
+  if (!ctx || !thr) return DG_ERR_ARG;
+  if (!ctx->d_adj) return dg_fail(ctx, DG_ERR_STATE, "no graph loaded");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint2> adj(ctx->m);
+  if (ctx->m)
     DG_HIP(ctx, hipMemcpy(adj.data(), ctx->d_adj, sizeof(uint2) * ctx->m, hipMemcpyDeviceToHost));
-    for (uint64_t e = 0; e < ctx->m; ++e) {
-      if (nbr) nbr[e] = adj[e].x;
-      if (cn) cn[e] = adj[e].y;
-    }
-  }
+  for (uint64_t e = 0; e < ctx->m; ++e) thr[e] = adj[e].y;
   return DG_OK;
 }

@@ -29,13 +29,16 @@
 namespace {
 
 constexpr int kWalkBlock = 256;
-constexpr int kPathLds = 47;  // vertices per lane kept in LDS (odd)
+constexpr int kPathLds = 23;    // vertices per lane kept in LDS (odd); reference walks hold 22
+constexpr int kTermTable = 32;  // entropy terms tabulated for walks shorter than 32
+constexpr int kTermStride = kTermTable + 1;
 constexpr uint32_t kPool = 64;
 constexpr uint32_t kDefaultLengthLimit = 1024;
 
 struct WalkArgs {
   const uint64_t* row_ptr;
-  const uint2* adj;
+  const uint2* adj;  // {neighbour, bounce threshold}
+  const double* term_table;
   uint32_t v_num;
   uint32_t round;
   uint64_t seed;
@@ -51,33 +54,36 @@ struct WalkArgs {
   uint64_t* walk_off;
   uint32_t* walk_len;
   uint32_t* spill;
+  uint32_t* freq;
   dg_walk_queue* q;
 };
 
+// Running statistics of Correlation (include/walk.hpp:260-387).  Sn and N
+// both equal the number of vertices on the walk, which the kernel keeps as an
+// integer, so only the seven FP64 moments live in registers.
 struct WalkState {
-  double H, S, EH, ES, EHS, EHH, ESS, N, R;
+  double H, EH, ES, EHS, EHH, ESS, R;
 };
 
-// Correlation::iterate with step_k = 1 (include/walk.hpp:284-354): the
-// incremental Shannon entropy H of the walk's visit histogram after appending
-// a vertex seen fi times before, running means of H, S, HS, HH, SS and the
-// Pearson correlation R of (H, S).  Same operation order as the reference.
-__device__ __forceinline__ void state_iterate(WalkState& w, double fi) {
+// log2 term of Correlation::iterateHn with step_k = 1 (include/walk.hpp:
+// 284-304) for a walk of Sn vertices appending a vertex seen fi times before:
+// log2(pow(Sn/(Sn+1), Sn) * pow(1/(Sn+1), 1) [* pow((fi+1)/fi, fi) * pow(fi+1, 1)]).
+__device__ __forceinline__ double entropy_term(double Sn, double fi) {
   const double one = 1.0;
-  const double Sn = w.S;
   const double Sn1 = Sn + one;
-  double prod = dg_pow(Sn / Sn1, Sn) * dg_pow(one / Sn1, one);
-  double t;
-  if (fi == 0.0) {
-    t = dg_log2(prod);
-  } else {
-    const double fj = fi + one;
-    const double fac = dg_pow(fj / fi, fi) * dg_pow(fj, one);
-    t = dg_log2(prod * fac);
-  }
+  const double prod = dg_pow(Sn / Sn1, Sn) * dg_pow(one / Sn1, one);
+  if (fi == 0.0) return dg_log2(prod);
+  const double fj = fi + one;
+  const double fac = dg_pow(fj / fi, fi) * dg_pow(fj, one);
+  return dg_log2(prod * fac);
+}
+
+// Rest of Correlation::iterate (include/walk.hpp:317-354), same operation order.
+__device__ __forceinline__ void state_iterate(WalkState& w, double Sn, double t) {
+  const double one = 1.0;
   const double H1 = (Sn * w.H - t) / (Sn + one);
-  const double S1 = w.S + one;
-  const double N1 = w.N + 1.0;
+  const double S1 = Sn + one;
+  const double N1 = Sn + 1.0;
   const double EH = w.EH + (H1 - w.EH) / N1;
   const double ES = w.ES + (S1 - w.ES) / N1;
   const double EHS = w.EHS + (H1 * S1 - w.EHS) / N1;
@@ -88,13 +94,20 @@ __device__ __forceinline__ void state_iterate(WalkState& w, double fi) {
   const double cov = EHS - EH * ES;
   w.R = cov == 0.0 ? 0.0 : cov / dg_sqrt(DH * DS);
   w.H = H1;
-  w.S = S1;
   w.EH = EH;
   w.ES = ES;
   w.EHS = EHS;
   w.EHH = EHH;
   w.ESS = ESS;
-  w.N = N1;
+}
+
+// term_table[S * kTermStride + f] = entropy_term(S, f) for 1 <= S < kTermTable,
+// 0 <= f <= S; computed by the same device code the walker would run inline.
+__global__ void term_table_kernel(double* tbl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kTermTable * kTermStride) return;
+  const int S = i / kTermStride, f = i % kTermStride;
+  tbl[i] = (S >= 1 && f <= S) ? entropy_term((double)S, (double)f) : 0.0;
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -102,21 +115,31 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
   __shared__ uint32_t s_path[kWalkBlock * kPathLds];
+  __shared__ double s_term[kTermTable * kTermStride];
+  for (int i = threadIdx.x; i < kTermTable * kTermStride; i += kWalkBlock) s_term[i] = a.term_table[i];
+  __syncthreads();
+
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t* path = s_path + threadIdx.x * kPathLds;
   const uint64_t slot = (uint64_t)blockIdx.x * kWalkBlock + threadIdx.x;
   uint32_t* spill = a.spill + slot * a.spill_per_slot;
 
   bool active = false;
-  bool drained = false;  // wave-uniform
-  uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  bool drained = false;                   // wave-uniform
+  uint64_t pool_next = 0, pool_end = 0;   // wave-uniform
 
-  uint32_t wid = 0, cur = 0, cur_deg = 0, len = 0, step = 0;
+  uint32_t wid = 0, cur = 0, cur_deg = 0, len = 0;
   uint64_t cur_rb = 0;
+  uint32_t ndraw = 0, nmove = 0;
   bool pending_back = false;
-  WalkState ws = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  WalkState ws = {0, 0, 0, 0, 0, 0, 0};
   dg_stream rs;
   dg_stream_init(&rs, a.seed, 0, 0, DG_TAG_WALK);
 
@@ -150,16 +173,15 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
           cur_deg = (uint32_t)(a.row_ptr[cur + 1] - cur_rb);
           path[0] = cur;
           len = 1;
-          step = 0;
+          ndraw = 0;
+          nmove = 1;
           pending_back = false;
           ws.H = 0.0;
-          ws.S = 1.0;
           ws.EH = 0.0;
           ws.ES = 1.0;
           ws.EHS = 0.0;
           ws.EHH = 0.0;
           ws.ESS = 1.0;
-          ws.N = 1.0;
           ws.R = 1.0;
           dg_stream_init(&rs, a.seed, w, a.round, DG_TAG_WALK);
           active = true;
@@ -180,32 +202,22 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
       } else {
         bool append = false;
         bool trace = false;
-        uint32_t v_app = 0, cand = 0, cand_deg = 0;
-        uint64_t cand_rb = 0;
+        uint32_t v_app = 0;
         if (pending_back) {
-          v_app = cur;  // walk.hpp:986-1017: return to the current vertex
+          v_app = cur;  // walk.hpp:986-1017: back to the current vertex
           append = true;
         } else if (cur_deg == 0) {
           finish = true;  // walk.hpp:887-892
         } else {
           const uint32_t i = dg_stream_uniform(&rs, cur_deg);  // walk.hpp:899
           const uint2 e = a.adj[cur_rb + i];
+          ++ndraw;
           if (cur_deg == 2 && a.adj[cur_rb].x == cur) {
             finish = true;  // walk.hpp:900-902
-          } else if (!(step == 0 && e.x == cur)) {  // walk.hpp:904-906 redraw
-            const uint32_t u = dg_stream_u31(&rs);   // walk.hpp:928
-            cand = e.x;
-            cand_rb = a.row_ptr[cand];
-            cand_deg = (uint32_t)(a.row_ptr[cand + 1] - cand_rb);
-            const int sd = (int)cur_deg;
-            const int dd = (int)cand_deg;
-            const int cnv = (int)e.y;  // walk.hpp:920
-            const double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) /
-                              (double)(sd < dd ? sd : dd);  // walk.hpp:933
-            const double pn = dg_tanh(pa);                  // walk.hpp:934
-            const double p = (double)u / 2147483647.0;
-            trace = (pn <= p);  // walk.hpp:938
-            v_app = cand;
+          } else if (!(len == 1 && e.x == cur)) {  // walk.hpp:904-906 redraw at step 0
+            const uint32_t u = dg_stream_u31(&rs);  // walk.hpp:928
+            trace = (u >= e.y);                    // walk.hpp:933-938 via the threshold
+            v_app = e.x;
             append = true;
           }
         }
@@ -215,13 +227,15 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
           const uint32_t n1 = len < (uint32_t)kPathLds ? len : (uint32_t)kPathLds;
           for (uint32_t j = 0; j < n1; ++j) fi += (path[j] == v_app);
           for (uint32_t j = kPathLds; j < len; ++j) fi += (spill[j - kPathLds] == v_app);
+          const double t = len < (uint32_t)kTermTable ? s_term[len * kTermStride + fi]
+                                                       : entropy_term((double)len, (double)fi);
+          state_iterate(ws, (double)len, t);
           if (len < (uint32_t)kPathLds)
             path[len] = v_app;
           else
             spill[len - kPathLds] = v_app;
           ++len;
-          ++step;
-          state_iterate(ws, (double)fi);
+          const uint32_t step = len - 1;
           if (step > a.min_length && (ws.R * ws.R < a.mu || ws.R < 0.0)) {
             finish = true;
           } else if (a.max_length != 0 && len >= a.max_length) {
@@ -234,9 +248,10 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
           } else if (trace) {
             pending_back = true;
           } else {
-            cur = cand;  // walk.hpp:1018-1023
-            cur_rb = cand_rb;
-            cur_deg = cand_deg;
+            cur = v_app;  // walk.hpp:1018-1023
+            cur_rb = a.row_ptr[cur];
+            cur_deg = (uint32_t)(a.row_ptr[cur + 1] - cur_rb);
+            ++nmove;
           }
         }
       }
@@ -252,8 +267,14 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
         if ((int)lane >= d) incl += t;
       }
       const uint32_t total = __shfl(incl, 63, 64);
+      const uint64_t draws = wave_sum_u64(finish ? ndraw : 0u);
+      const uint64_t moves = wave_sum_u64(finish ? nmove : 0u);
       unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&a.q->cursor, (unsigned long long)total);
+      if (lane == 0) {
+        base = atomicAdd(&a.q->cursor, (unsigned long long)total);
+        atomicAdd(&a.q->draws, (unsigned long long)draws);
+        atomicAdd(&a.q->moves, (unsigned long long)moves);
+      }
       base = __shfl(base, 0, 64);
       const bool fits = base + total <= a.tok_cap;
       const uint64_t my_off = base + incl - my_len;
@@ -275,10 +296,18 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
           const unsigned long long off = __shfl((unsigned long long)my_off, L, 64);
           const uint32_t nl = n < (uint32_t)kPathLds ? n : (uint32_t)kPathLds;
           const uint32_t* src = wave_path + L * kPathLds;
-          for (uint32_t j = lane; j < nl; j += 64) a.tokens[off + j] = src[j];
+          for (uint32_t j = lane; j < nl; j += 64) {
+            const uint32_t v = src[j];
+            a.tokens[off + j] = v;
+            atomicAdd(&a.freq[v], 1u);  // path.hpp:91,95 (context_map_freq, vertex_cn)
+          }
         }
         if (finish)  // the spilled tail, by its own lane
-          for (uint32_t j = kPathLds; j < len; ++j) a.tokens[my_off + j] = spill[j - kPathLds];
+          for (uint32_t j = kPathLds; j < len; ++j) {
+            const uint32_t v = spill[j - kPathLds];
+            a.tokens[my_off + j] = v;
+            atomicAdd(&a.freq[v], 1u);
+          }
       }
       if (finish) active = false;
     }
@@ -293,6 +322,8 @@ __global__ void queue_reset_kernel(dg_walk_queue* q) {
     q->max_len = 0;
     q->zero_terms = 0;
     q->re = 0.0;
+    q->draws = 0;
+    q->moves = 0;
   }
 }
 
@@ -378,18 +409,37 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_apply(const uint32_t* le
 }
 
 // ---- corpus packing: walk-id order, plus cumulative visit counts -----------
-__global__ void pack_kernel(const uint32_t* raw, const uint64_t* walk_off, const uint64_t* out_off,
-                            uint64_t n_walks, uint32_t* tokens, uint32_t* freq) {
-  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
-       w += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t src = walk_off[w];
-    const uint64_t dst = out_off[w];
-    const uint64_t n = out_off[w + 1] - dst;
-    for (uint64_t j = 0; j < n; ++j) {
-      const uint32_t v = raw[src + j];
-      tokens[dst + j] = v;
-      atomicAdd(&freq[v], 1u);  // path.hpp:91,95 (context_map_freq, vertex_cn)
+// Each block takes 256 consecutive walks; their source and destination
+// offsets go to LDS and every thread copies single tokens, finding its walk by
+// bisection over the block's offsets, so loads and stores are coalesced.
+__global__ __launch_bounds__(256) void pack_kernel(const uint32_t* raw, const uint64_t* walk_off,
+                                                   const uint64_t* out_off, uint64_t n_walks,
+                                                   uint32_t* tokens) {
+  __shared__ uint64_t s_out[257];
+  __shared__ uint64_t s_src[256];
+  for (uint64_t w0 = (uint64_t)blockIdx.x * 256; w0 < n_walks; w0 += (uint64_t)gridDim.x * 256) {
+    const uint32_t nw = (uint32_t)(n_walks - w0 < 256 ? n_walks - w0 : 256);
+    if (threadIdx.x < nw) {
+      s_out[threadIdx.x] = out_off[w0 + threadIdx.x];
+      s_src[threadIdx.x] = walk_off[w0 + threadIdx.x];
     }
+    if (threadIdx.x == 0) s_out[nw] = out_off[w0 + nw];
+    __syncthreads();
+    const uint64_t base = s_out[0];
+    const uint64_t total = s_out[nw] - base;
+    for (uint64_t t = threadIdx.x; t < total; t += 256) {
+      const uint64_t pos = base + t;
+      uint32_t lo = 0, hi = nw - 1;  // last walk k with s_out[k] <= pos
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (s_out[mid] <= pos)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      tokens[pos] = raw[s_src[lo] + (pos - s_out[lo])];
+    }
+    __syncthreads();
   }
 }
 
@@ -453,7 +503,8 @@ int grid_for(dg_ctx* ctx, uint64_t n, int block) {
 void dg_walk_free(dg_ctx* ctx) {
   void* bufs[] = {ctx->d_freq,   ctx->d_tok_raw,  ctx->d_walk_off, ctx->d_walk_len,
                   ctx->d_tokens, ctx->d_offsets,  ctx->d_spill,    ctx->d_scan_tmp,
-                  ctx->d_re_terms, ctx->d_re_chunks, ctx->d_train_tokens, ctx->d_train_offsets};
+                  ctx->d_re_terms, ctx->d_re_chunks, ctx->d_train_tokens, ctx->d_train_offsets,
+                  ctx->d_term_table};
   for (void* p : bufs)
     if (p) hipFree(p);
   ctx->d_freq = nullptr;
@@ -468,6 +519,7 @@ void dg_walk_free(dg_ctx* ctx) {
   ctx->d_re_chunks = nullptr;
   ctx->d_train_tokens = nullptr;
   ctx->d_train_offsets = nullptr;
+  ctx->d_term_table = nullptr;
   ctx->tok_raw_cap = ctx->walk_cap = ctx->walk_len_cap = ctx->tokens_cap = ctx->offsets_cap = 0;
   ctx->spill_cap = ctx->scan_tmp_cap = ctx->re_cap = 0;
   ctx->train_tokens_cap = ctx->train_offsets_cap = 0;
@@ -515,6 +567,12 @@ extern "C" int dg_walk_reset(dg_ctx* ctx) {
   if (!ctx->d_freq) {
     DG_HIP(ctx, hipMalloc((void**)&ctx->d_freq, sizeof(uint32_t) * ctx->v_num));
   }
+  if (!ctx->d_term_table) {
+    DG_HIP(ctx, hipMalloc((void**)&ctx->d_term_table, sizeof(double) * kTermTable * kTermStride));
+    term_table_kernel<<<(kTermTable * kTermStride + 255) / 256, 256, 0, ctx->stream>>>(
+        ctx->d_term_table);
+    DG_HIP(ctx, hipGetLastError());
+  }
   fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq, ctx->v_num,
                                                                            0u);
   DG_HIP(ctx, hipGetLastError());
@@ -557,6 +615,8 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   WalkArgs a;
   a.row_ptr = ctx->d_row_ptr;
   a.adj = ctx->d_adj;
+  a.term_table = ctx->d_term_table;
+  a.freq = ctx->d_freq;
   a.v_num = ctx->v_num;
   a.round = round;
   a.seed = p->seed;
@@ -595,14 +655,16 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   }
   const uint64_t n_tok = ctx->h_queue->cursor;
   const uint32_t max_len = ctx->h_queue->max_len;
+  const uint64_t draws = ctx->h_queue->draws;
+  const uint64_t moves = ctx->h_queue->moves;
 
   // pack in walk-id order and count visits
   DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, (size_t)n_tok));
   size_t tp;
   DG_CHECK(dg_time_begin(ctx, "pack", &tp));
   DG_CHECK(dg_scan_lengths(ctx, ctx->d_walk_len, walker_num, ctx->d_offsets));
-  pack_kernel<<<grid_for(ctx, walker_num, 256), 256, 0, ctx->stream>>>(
-      ctx->d_tok_raw, ctx->d_walk_off, ctx->d_offsets, walker_num, ctx->d_tokens, ctx->d_freq);
+  pack_kernel<<<grid_for(ctx, (walker_num + 255) / 256, 1), 256, 0, ctx->stream>>>(
+      ctx->d_tok_raw, ctx->d_walk_off, ctx->d_offsets, walker_num, ctx->d_tokens);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tp));
   ctx->freq_total += n_tok;
@@ -651,6 +713,8 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
     out->walk_ms = walk_ms;
     out->total_ms = total_ms;
     out->max_walk_len = max_len;
+    out->draws = draws;
+    out->moves = moves;
     out->stop = 0;
   }
   if (ctx->h_queue->zero_terms != 0)

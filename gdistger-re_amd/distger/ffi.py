@@ -59,6 +59,8 @@ class RoundStats(ctypes.Structure):
         ("total_ms", ctypes.c_double),
         ("max_walk_len", ctypes.c_uint32),
         ("stop", ctypes.c_uint32),
+        ("draws", ctypes.c_uint64),
+        ("moves", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -96,6 +98,7 @@ def load_library():
         "dg_graph_load_bin": (c.c_int, [vp, c.c_char_p, c.c_uint32, c.c_int, c.c_int]),
         "dg_graph_info": (c.c_int, [vp, c.POINTER(c.c_uint32), c.POINTER(c.c_uint64)]),
         "dg_graph_get_csr": (c.c_int, [vp, vp, vp, vp]),
+        "dg_graph_get_thresholds": (c.c_int, [vp, vp]),
         "dg_walk_params_default": (None, [c.POINTER(WalkParams), c.c_uint32]),
         "dg_walk_round": (c.c_int, [vp, c.POINTER(WalkParams), c.c_uint32, c.c_uint32,
                                     c.c_uint32, c.POINTER(RoundStats)]),
@@ -196,6 +199,12 @@ class Context:
         cn = np.zeros(m, np.uint32)
         self._check(self.L.dg_graph_get_csr(self.h, _vp(rp), _vp(nbr), _vp(cn)))
         return rp, nbr, cn
+
+    def graph_thresholds(self):
+        _, m = self.graph_info()
+        thr = np.zeros(m, np.uint32)
+        self._check(self.L.dg_graph_get_thresholds(self.h, _vp(thr)))
+        return thr
 
     # ---- walker
     def default_walk_params(self, v_num=None, **kw):

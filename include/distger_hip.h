@@ -89,6 +89,11 @@ int dg_graph_info(dg_ctx* ctx, uint32_t* v_num, uint64_t* m);
 /* Copy the device CSR back: row_ptr[v_num+1], nbr[m], cn[m] (any may be NULL). */
 int dg_graph_get_csr(dg_ctx* ctx, uint64_t* row_ptr, uint32_t* nbr, uint32_t* cn);
 
+/* Per-entry bounce thresholds thr[m]: the walker moving u -> v bounces back
+ * (walk.hpp:938, tanh(pa) <= rand()/RAND_MAX) exactly when its 31-bit draw is
+ * >= thr.  Exported for verification. */
+int dg_graph_get_thresholds(dg_ctx* ctx, uint32_t* thr);
+
 /* ---- walker (W4-W9) ---------------------------------------------------- */
 
 #define DG_ROUNDS_REFERENCE 0 /* stop after max_rounds rounds (walk.hpp:740-744) */
@@ -124,6 +129,8 @@ typedef struct {
   double total_ms;       /* device time of the whole round */
   uint32_t max_walk_len; /* longest walk of the round */
   uint32_t stop;         /* 1 if the round driver stops after this round */
+  uint64_t draws;        /* candidate draws (8-byte adjacency units read) */
+  uint64_t moves;        /* row-offset pairs read (walk starts + moves) */
 } dg_round_stats;
 
 /* Run one round: walks walker_begin .. walker_begin+walker_num-1 (start vertex

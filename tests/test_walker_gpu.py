@@ -158,3 +158,48 @@ def test_bin_roundtrip_and_load(ctx, c1_edges, tmp_path):
     ctx.graph_load_bin(path, 10000, undirected=True, cn_mode=distger.ffi.DG_CN_FROM_DATA)
     rp, nbr, cn = ctx.graph_csr()
     assert np.array_equal(cn, g.cn[: g.m]) and np.array_equal(nbr, g.nbr[: g.m])
+
+
+def _hub_graph(n_hub_nbrs, v, extra, seed):
+    rng = np.random.default_rng(seed)
+    pairs = set((0, i) for i in range(1, n_hub_nbrs + 1))
+    while len(pairs) < n_hub_nbrs + extra:
+        a, b = rng.integers(0, v, 2)
+        if a != b:
+            pairs.add((int(min(a, b)), int(max(a, b))))
+    pairs = sorted(pairs)
+    rng.shuffle(pairs)
+    e = np.zeros(len(pairs), distger.EDGE_DTYPE)
+    e["src"] = [p[0] for p in pairs]
+    e["dst"] = [p[1] for p in pairs]
+    return e
+
+
+@pytest.mark.parametrize("hub,v,extra", [(3000, 5000, 20000), (9000, 12000, 30000)])
+def test_common_neighbours_with_hubs(ctx, hub, v, extra):
+    """Hub rows above 1024 entries take the workgroup path; above 8192 the
+    chunked one.  Counts must equal the oracle's sorted-merge counts."""
+    e = _hub_graph(hub, v, extra, hub)
+    ctx.graph_from_edges(v, e, cn_mode=distger.ffi.DG_CN_COMPUTE)
+    g = dgoracle.OracleGraph(v, e)
+    rp, nbr, cn = ctx.graph_csr()
+    assert np.array_equal(nbr, g.nbr[: g.m])
+    assert np.array_equal(cn, g.cn[: g.m])
+
+
+def test_bounce_thresholds_definition(c1_ctx):
+    """thr = min{u : tanh(pa) <= u / (2^31-1)} with the oracle's tanh."""
+    L = dgoracle.load("dg")
+    rp, nbr, cn = c1_ctx.graph_csr()
+    thr = c1_ctx.graph_thresholds()
+    deg = np.diff(rp).astype(np.int64)
+    src = np.repeat(np.arange(len(deg)), deg)
+    M = 2147483647.0
+    rng = np.random.default_rng(0)
+    for e in rng.choice(len(nbr), 4000, replace=False):
+        sd, dd, c = int(deg[src[e]]), int(deg[nbr[e]]), int(cn[e])
+        pa = (1.0 / float(sd - c)) * float(max(sd, dd)) / float(min(sd, dd))
+        pn = L.dgo_tanh(pa)
+        t = int(thr[e])
+        assert pn <= t / M
+        assert t == 0 or not (pn <= (t - 1) / M)
