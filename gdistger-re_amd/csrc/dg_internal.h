@@ -62,6 +62,8 @@ struct dg_ctx {
   dg_walk_queue* h_queue = nullptr;  // pinned mirror
   uint32_t* d_freq = nullptr;        // cumulative visit counts
   uint64_t freq_total = 0;           // sum of d_freq
+  uint32_t* d_freq_round = nullptr;  // visits of the round being walked
+  size_t freq_round_cap = 0;
   uint32_t* d_tok_raw = nullptr;     // tokens in completion order
   size_t tok_raw_cap = 0;
   uint64_t* d_walk_off = nullptr;    // per-walk offset into d_tok_raw

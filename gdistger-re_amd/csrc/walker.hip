@@ -327,6 +327,12 @@ __global__ void queue_reset_kernel(dg_walk_queue* q) {
   }
 }
 
+__global__ void add_u32_kernel(uint32_t* dst, const uint32_t* src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] += src[i];
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -504,7 +510,7 @@ void dg_walk_free(dg_ctx* ctx) {
   void* bufs[] = {ctx->d_freq,   ctx->d_tok_raw,  ctx->d_walk_off, ctx->d_walk_len,
                   ctx->d_tokens, ctx->d_offsets,  ctx->d_spill,    ctx->d_scan_tmp,
                   ctx->d_re_terms, ctx->d_re_chunks, ctx->d_train_tokens, ctx->d_train_offsets,
-                  ctx->d_term_table};
+                  ctx->d_term_table, ctx->d_freq_round};
   for (void* p : bufs)
     if (p) hipFree(p);
   ctx->d_freq = nullptr;
@@ -520,6 +526,8 @@ void dg_walk_free(dg_ctx* ctx) {
   ctx->d_train_tokens = nullptr;
   ctx->d_train_offsets = nullptr;
   ctx->d_term_table = nullptr;
+  ctx->d_freq_round = nullptr;
+  ctx->freq_round_cap = 0;
   ctx->tok_raw_cap = ctx->walk_cap = ctx->walk_len_cap = ctx->tokens_cap = ctx->offsets_cap = 0;
   ctx->spill_cap = ctx->scan_tmp_cap = ctx->re_cap = 0;
   ctx->train_tokens_cap = ctx->train_offsets_cap = 0;
@@ -616,7 +624,11 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   a.row_ptr = ctx->d_row_ptr;
   a.adj = ctx->d_adj;
   a.term_table = ctx->d_term_table;
-  a.freq = ctx->d_freq;
+  // this round's visits go to a separate count array, added to the cumulative
+  // counts only once the round succeeded (a retry after a token-buffer
+  // overflow must not count twice)
+  DG_CHECK(dg_reserve(ctx, &ctx->d_freq_round, &ctx->freq_round_cap, (size_t)ctx->v_num));
+  a.freq = ctx->d_freq_round;
   a.v_num = ctx->v_num;
   a.round = round;
   a.seed = p->seed;
@@ -636,6 +648,8 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
     a.tokens = ctx->d_tok_raw;
     a.tok_cap = ctx->tok_raw_cap;
     queue_reset_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
+    fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq_round,
+                                                                             ctx->v_num, 0u);
     size_t tk;
     DG_CHECK(dg_time_begin(ctx, "walk", &tk));
     walk_kernel<<<(unsigned)blocks, kWalkBlock, 0, ctx->stream>>>(a);
@@ -667,6 +681,9 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
       ctx->d_tok_raw, ctx->d_walk_off, ctx->d_offsets, walker_num, ctx->d_tokens);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tp));
+  add_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
+      ctx->d_freq, ctx->d_freq_round, ctx->v_num);
+  DG_HIP(ctx, hipGetLastError());
   ctx->freq_total += n_tok;
   ctx->cur_walks = walker_num;
   ctx->cur_tokens = n_tok;

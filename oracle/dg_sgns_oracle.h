@@ -1,0 +1,37 @@
+/*
+ * dg_sgns_oracle.h - CPU oracle for the SGNS trainer.  TEST INFRASTRUCTURE
+ * ONLY: loaded by the tests, the smoke check and the benchmark's CPU-baseline
+ * leg as the checker, never by the product library.
+ */
+#ifndef DG_SGNS_ORACLE_H
+#define DG_SGNS_ORACLE_H
+
+#include <stdint.h>
+
+int dgo_vocab(uint32_t v_num, const uint64_t* cn, uint32_t min_count, uint32_t* vertex_of_rank,
+              uint32_t* rank_of_vertex, uint64_t* cn_of_rank, uint32_t* vocab_size,
+              uint64_t* train_words);
+void dgo_init_syn0(uint32_t vocab, uint32_t dim, int float_ops, float* syn0);
+void dgo_unigram_table(const uint64_t* cn_of_rank, uint32_t vocab, uint32_t table_size,
+                       int32_t* table);
+void dgo_exp_table(float* t);
+void dgo_keep_limits(const uint64_t* cn_of_rank, uint32_t vocab, float sample, uint64_t train_words,
+                     uint32_t* keep);
+uint64_t dgo_subsample(const uint32_t* tokens, const uint64_t* offsets, uint64_t n_walks,
+                       const uint32_t* rank_of_vertex, const uint32_t* keep, uint64_t seed,
+                       uint32_t epoch, int32_t* sen, uint64_t* sen_off);
+void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_sen,
+                        uint32_t window, uint32_t negative, const int32_t* table,
+                        uint32_t table_size, uint32_t vocab, float alpha, const float* expt,
+                        uint32_t dim, float* syn0, float* syn1);
+uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,
+                           uint64_t cap);
+void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
+                    uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
+                    const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
+                    const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
+                    uint32_t dim, float* syn0, float* syn1);
+double dgo_tuple_loss(const uint32_t* tuples, uint64_t n, uint32_t K, const double* syn0,
+                      const double* syn1, uint32_t dim);
+
+#endif

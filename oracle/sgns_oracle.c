@@ -1,0 +1,437 @@
+/*
+ * sgns_oracle.c - CPU restatement of the skip-gram negative-sampling trainer
+ * (word2vec.cu path and the batched shared-negative form).
+ *
+ * TEST INFRASTRUCTURE ONLY (see dg_oracle.h).  Single-threaded, sequential
+ * order; float arithmetic kept in the same operation order as the kernels it
+ * checks, so deterministic GPU modes can be compared bit for bit.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../gdistger-re_amd/csrc/dg_philox.h"
+#include "dg_sgns_oracle.h"
+
+#define EXP_TABLE_SIZE 1000
+#define MAX_EXP 6
+
+/* ------------------------------------------------------------------------
+ * S1: vocabulary.  src/word2vec.cu:791-809 (words = vertex ids, counts =
+ * degrees) and :632-659 (SortVocab: descending count, words below min_count
+ * dropped except index 0).  Ties are broken by vertex id (qsort is unstable).
+ * ---------------------------------------------------------------------- */
+static const uint64_t* g_sort_cn;
+static int cmp_vocab(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  if (g_sort_cn[x] != g_sort_cn[y]) return g_sort_cn[x] > g_sort_cn[y] ? -1 : 1;
+  return (x > y) - (x < y);
+}
+
+int dgo_vocab(uint32_t v_num, const uint64_t* cn, uint32_t min_count, uint32_t* vertex_of_rank,
+              uint32_t* rank_of_vertex, uint64_t* cn_of_rank, uint32_t* vocab_size,
+              uint64_t* train_words) {
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (v_num ? v_num : 1));
+  if (!order) return -1;
+  for (uint32_t i = 0; i < v_num; ++i) order[i] = i;
+  g_sort_cn = cn;
+  qsort(order, v_num, sizeof(uint32_t), cmp_vocab);
+  uint32_t n = 0;
+  uint64_t tw = 0;
+  for (uint32_t i = 0; i < v_num; ++i) rank_of_vertex[i] = 0xffffffffu;
+  for (uint32_t a = 0; a < v_num; ++a) {
+    uint32_t v = order[a];
+    if (cn[v] < min_count && a != 0) continue;
+    vertex_of_rank[n] = v;
+    cn_of_rank[n] = cn[v];
+    rank_of_vertex[v] = n;
+    tw += cn[v];
+    ++n;
+  }
+  free(order);
+  *vocab_size = n;
+  *train_words = tw;
+  return 0;
+}
+
+/* S2: src/word2vec.cu:868-873 - LCG from 1, syn0 = ((r & 0xFFFF)/65536f - 0.5)/d
+ * evaluated in double and rounded once (float_ops = 0), or the float
+ * expression of include/dsgl.hpp:452-456 (float_ops = 1). */
+void dgo_init_syn0(uint32_t vocab, uint32_t dim, int float_ops, float* syn0) {
+  unsigned long long next_random = 1;
+  for (uint64_t a = 0; a < vocab; ++a)
+    for (uint64_t b = 0; b < dim; ++b) {
+      next_random = next_random * (unsigned long long)25214903917ULL + 11;
+      if (float_ops)
+        syn0[a * dim + b] = (((next_random & 0xFFFF) / 65536.f) - 0.5f) / (float)dim;
+      else
+        syn0[a * dim + b] = (float)((((double)((float)(next_random & 0xFFFF) / (float)65536)) - 0.5) /
+                                    (double)(long long)dim);
+    }
+}
+
+/* S3: src/word2vec.cu:537-552 unigram table ~ count^0.75 (double d1).  The
+ * reference reads vocab[vocab_size] when the index runs past the end; here
+ * that increment adds nothing. */
+void dgo_unigram_table(const uint64_t* cn_of_rank, uint32_t vocab, uint32_t table_size,
+                       int32_t* table) {
+  double train_words_pow = 0;
+  for (uint32_t a = 0; a < vocab; ++a) train_words_pow += pow((double)cn_of_rank[a], 0.75);
+  uint32_t i = 0;
+  double d1 = pow((double)cn_of_rank[0], 0.75) / train_words_pow;
+  for (uint32_t a = 0; a < table_size; ++a) {
+    table[a] = (int32_t)i;
+    if (a / (double)table_size > d1) {
+      i++;
+      if (i < vocab) d1 += pow((double)cn_of_rank[i], 0.75) / train_words_pow;
+    }
+    if (i >= vocab) i = vocab - 1;
+  }
+}
+
+/* src/word2vec.cu:1569-1577: sigmoid table, exp in double of a float argument. */
+void dgo_exp_table(float* t) {
+  for (int i = 0; i < EXP_TABLE_SIZE; ++i) {
+    t[i] = (float)exp((double)((i / (float)EXP_TABLE_SIZE * 2 - 1) * MAX_EXP));
+    t[i] = t[i] / (t[i] + 1);
+  }
+  t[EXP_TABLE_SIZE] = t[EXP_TABLE_SIZE - 1];
+}
+
+/* Subsampling keep limit per word (src/word2vec.cu:1015-1020 with float
+ * arithmetic): the token is dropped when ran < (r & 0xFFFF)/65536, i.e. kept
+ * iff (r & 0xFFFF) <= floor(ran * 65536); 65536 means always kept. */
+void dgo_keep_limits(const uint64_t* cn_of_rank, uint32_t vocab, float sample, uint64_t train_words,
+                     uint32_t* keep) {
+  for (uint32_t w = 0; w < vocab; ++w) {
+    if (!(sample > 0)) {
+      keep[w] = 65536;
+      continue;
+    }
+    float st = sample * (float)train_words;
+    float c = (float)cn_of_rank[w];
+    float ran = (sqrtf(c / st) + 1) * st / c;
+    float lim = ran * 65536.0f;
+    keep[w] = lim >= 65536.0f ? 65536u : (uint32_t)lim;
+  }
+}
+
+/* Subsampled sentences: each walk of the corpus becomes one sentence (tokens
+ * mapped to vocabulary ranks; vertices outside the vocabulary skipped, as
+ * SearchVocab == -1 at src/word2vec.cu:1012).  The draw for corpus token i
+ * in epoch e is word 0 of Philox(ctr = (0, i lo, i hi ^ e<<24, SAMPLE)). */
+uint64_t dgo_subsample(const uint32_t* tokens, const uint64_t* offsets, uint64_t n_walks,
+                       const uint32_t* rank_of_vertex, const uint32_t* keep, uint64_t seed,
+                       uint32_t epoch, int32_t* sen, uint64_t* sen_off) {
+  uint64_t n = 0;
+  sen_off[0] = 0;
+  for (uint64_t w = 0; w < n_walks; ++w) {
+    for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
+      uint32_t r = rank_of_vertex[tokens[i]];
+      if (r == 0xffffffffu) continue;
+      dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32) ^ (epoch << 24),
+                                    DG_TAG_SAMPLE, (uint32_t)seed, (uint32_t)(seed >> 32));
+      if ((o.x0 & 0xFFFFu) > keep[r]) continue;
+      sen[n++] = (int32_t)r;
+    }
+    sen_off[w + 1] = n;
+  }
+  return n;
+}
+
+static float table_sigmoid_grad(float f, int label, float alpha, const float* expt) {
+  /* src/word2vec.cu:384-389 */
+  if (f > MAX_EXP) return (label - 1) * alpha;
+  if (f < -MAX_EXP) return (label - 0) * alpha;
+  return (label - expt[(int)((f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))]) * alpha;
+}
+
+/* Dot product in the order of the reference block reduction
+ * (src/word2vec.cu:373-381 for d = 2 * FSIZE; generalised to d = 64 * k):
+ * s[l] = sum over j of x[l + 64 j] * y[l + 64 j] taken in order j = 0, 1, ...
+ * (pairs folded as f[x] += f[x + FSIZE] per halving), then the 64 partials
+ * folded with offsets 32, 16, 8, 4, 2, 1. */
+static float tree_dot(const float* x, const float* y, uint32_t dim) {
+  float s[64];
+  uint32_t per = dim / 64;
+  for (uint32_t l = 0; l < 64; ++l) {
+    /* per-lane values p[j] = x[l + 64 j] * y[l + 64 j]; halving folds */
+    float p[16];
+    for (uint32_t j = 0; j < per; ++j) p[j] = x[l + 64 * j] * y[l + 64 * j];
+    uint32_t n = per;
+    while (n > 1) {
+      uint32_t h = n / 2;
+      for (uint32_t j = 0; j < h; ++j) p[j] = p[j] + p[j + h];
+      n = h;
+    }
+    s[l] = p[0];
+  }
+  for (uint32_t off = 32; off >= 1; off /= 2)
+    for (uint32_t l = 0; l < off; ++l) s[l] = s[l] + s[l + off];
+  return s[0];
+}
+
+/* ------------------------------------------------------------------------
+ * S5: skip_gram_kernel (src/word2vec.cu:305-399), negative sampling only,
+ * sentences of one batch in order (block index = sentence index in batch).
+ * ---------------------------------------------------------------------- */
+void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_sen,
+                        uint32_t window, uint32_t negative, const int32_t* table,
+                        uint32_t table_size, uint32_t vocab, float alpha, const float* expt,
+                        uint32_t dim, float* syn0, float* syn1) {
+  float* neu1e = (float*)malloc(sizeof(float) * dim);
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    unsigned long next_random = (unsigned long)s;
+    for (uint64_t pos = sen_off[s]; pos < sen_off[s + 1]; ++pos) {
+      int word = sen[pos];
+      next_random = next_random * (unsigned long)2514903917UL + 11;
+      int b = (int)(next_random % window);
+      for (int a = b; a < (int)window * 2 + 1 - b; a++) {
+        if (a == (int)window) continue;
+        int64_t c = (int64_t)pos - (int64_t)window + a;
+        if (c < (int64_t)sen_off[s] || c >= (int64_t)sen_off[s + 1]) continue;
+        int last_word = sen[c];
+        float* x = syn0 + (uint64_t)last_word * dim;
+        for (uint32_t i = 0; i < dim; ++i) neu1e[i] = 0;
+        for (uint32_t d = 0; d < negative + 1; d++) {
+          int target, label;
+          if (d == 0) {
+            target = word;
+            label = 1;
+          } else {
+            next_random = next_random * (unsigned long)25214903917UL + 11;
+            target = table[(next_random >> 16) % table_size];
+            if (target == 0) target = (int)(next_random % (vocab - 1) + 1);
+            if (target == word) continue;
+            label = 0;
+          }
+          float* y = syn1 + (uint64_t)target * dim;
+          float f = tree_dot(x, y, dim);
+          float g = table_sigmoid_grad(f, label, alpha, expt);
+          for (uint32_t i = 0; i < dim; ++i) {
+            neu1e[i] += g * y[i];
+            y[i] += g * x[i];
+          }
+        }
+        for (uint32_t i = 0; i < dim; ++i) x[i] += neu1e[i];
+      }
+    }
+  }
+  free(neu1e);
+}
+
+/* Reference batch boundaries (src/word2vec.cu:999-1041): at most 15000
+ * sentences, and a batch closes once it holds >= (15000 - 1) * 20 tokens. */
+uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,
+                           uint64_t cap) {
+  uint64_t nb = 0, s = 0;
+  while (s < n_sen && nb < cap) {
+    batch_first[nb++] = s;
+    uint64_t cnt = 0;
+    uint64_t base = sen_off[s];
+    while (s < n_sen && cnt < 15000) {
+      ++cnt;
+      ++s;
+      if (sen_off[s] - base >= (15000 - 1) * 20) break;
+    }
+  }
+  if (nb < cap) batch_first[nb] = n_sen;
+  return nb;
+}
+
+/* ------------------------------------------------------------------------
+ * Batched form: sentence minibatches with per-sentence shared negatives
+ * (the semantics of __sgNegReuse, src/word2vec.cu:231-303, every window of a
+ * sentence sharing K negatives), all gradients of a minibatch taken from the
+ * table snapshot at its start and applied as one sum per row.
+ *
+ * Per sentence s of length L (>= 2): the Philox stream (ctr = (blk, s lo,
+ * s hi ^ epoch<<24, SGNS)) gives, in order, b_p = next % window for p = 0..L-1
+ * and K raw words x_k, negative n_k = table[x_k % table_size] (0 -> x_k %
+ * (vocab-1) + 1).
+ *   A[c][p] = x_c . y_{t_p}  (c in win(p)),  B[c][k] = x_c . y_{n_k}
+ *   gp[c][p] = (1 - sig(A)) alpha,  gn[c][k] = -sig(B) alpha * m[c][k],
+ *   m[c][k] = #{p : c in win(p), n_k != t_p}
+ *   dX[c] = sum_p gp[c][p] y_{t_p} + sum_k gn[c][k] y_{n_k}   (p, then k ascending)
+ *   dYt[p] = sum_c gp[c][p] x_c ; dYn[k] = sum_c gn[c][k] x_c  (c ascending)
+ * Row sums: contributions in (sentence, slot) order, slots dX[0..L),
+ * dYt[0..L), dYn[0..K); W[r] = W0[r] + (((c1 + c2) + c3) ...).
+ * sig is the reference table sigmoid (exact_sigmoid = 0) or 1/(1+e^-f).
+ * ---------------------------------------------------------------------- */
+static float sig_value(float f, const float* expt, int exact) {
+  if (exact) return (float)(1.0 / (1.0 + exp(-(double)f)));
+  if (f > MAX_EXP) return 1.0f;
+  if (f < -MAX_EXP) return 0.0f;
+  return expt[(int)((f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))];
+}
+
+typedef struct {
+  uint32_t row;
+  uint32_t table; /* 0 = input (syn0), 1 = output (syn1) */
+  float* vec;
+} contrib_t;
+
+static int cmp_contrib(const void* a, const void* b) {
+  const contrib_t* x = (const contrib_t*)a;
+  const contrib_t* y = (const contrib_t*)b;
+  if (x->table != y->table) return x->table < y->table ? -1 : 1;
+  if (x->row != y->row) return x->row < y->row ? -1 : 1;
+  return (x->vec > y->vec) - (x->vec < y->vec); /* slot order = allocation order */
+}
+
+void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
+                    uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
+                    const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
+                    const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
+                    uint32_t dim, float* syn0, float* syn1) {
+  uint32_t K = negative;
+  for (uint64_t mb0 = 0; mb0 < n_sen; mb0 += minibatch) {
+    uint64_t mb1 = mb0 + minibatch < n_sen ? mb0 + minibatch : n_sen;
+    /* count slots */
+    uint64_t slots = 0;
+    for (uint64_t s = mb0; s < mb1; ++s) {
+      uint64_t L = sen_off[s + 1] - sen_off[s];
+      if (L >= 2) slots += 2 * L + K;
+    }
+    contrib_t* cs = (contrib_t*)malloc(sizeof(contrib_t) * (slots ? slots : 1));
+    float* vecs = (float*)calloc(slots ? slots * dim : 1, sizeof(float));
+    uint64_t ns = 0;
+    for (uint64_t s = mb0; s < mb1; ++s) {
+      const int32_t* t = sen + sen_off[s];
+      uint32_t L = (uint32_t)(sen_off[s + 1] - sen_off[s]);
+      if (L < 2) continue;
+      uint64_t gs = first_sen + s;
+      dg_stream st;
+      dg_stream_init(&st, seed, (uint32_t)gs, (uint32_t)(gs >> 32) ^ (epoch << 24), DG_TAG_SGNS);
+      uint32_t b[4096];
+      int32_t neg[64];
+      for (uint32_t p = 0; p < L; ++p) b[p] = dg_stream_next(&st) % window;
+      for (uint32_t k = 0; k < K; ++k) {
+        uint32_t x = dg_stream_next(&st);
+        int32_t n = table[x % table_size];
+        if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
+        neg[k] = n;
+      }
+      float* gp = (float*)calloc((size_t)L * L, sizeof(float));
+      float* gn = (float*)calloc((size_t)L * K + 1, sizeof(float));
+      for (uint32_t p = 0; p < L; ++p) {
+        int64_t lo = (int64_t)p - (int64_t)window + b[p], hi = (int64_t)p + (int64_t)window - b[p];
+        for (int64_t c = lo; c <= hi; ++c) {
+          if (c < 0 || c >= L || c == p) continue;
+          float f = tree_dot(syn0 + (uint64_t)t[c] * dim, syn1 + (uint64_t)t[p] * dim, dim);
+          gp[(uint64_t)c * L + p] = (1.0f - sig_value(f, expt, exact_sigmoid)) * alpha;
+        }
+      }
+      for (uint32_t c = 0; c < L; ++c) {
+        for (uint32_t k = 0; k < K; ++k) {
+          uint32_t m = 0;
+          for (uint32_t p = 0; p < L; ++p) {
+            int64_t lo = (int64_t)p - (int64_t)window + b[p], hi = (int64_t)p + (int64_t)window - b[p];
+            if ((int64_t)c >= lo && (int64_t)c <= hi && c != p && neg[k] != t[p]) ++m;
+          }
+          if (m == 0) continue;
+          float f = tree_dot(syn0 + (uint64_t)t[c] * dim, syn1 + (uint64_t)neg[k] * dim, dim);
+          gn[(uint64_t)c * K + k] = (0.0f - sig_value(f, expt, exact_sigmoid)) * alpha * (float)m;
+        }
+      }
+      /* dX */
+      for (uint32_t c = 0; c < L; ++c) {
+        float* v = vecs + (ns) * dim;
+        for (uint32_t i = 0; i < dim; ++i) {
+          float acc = 0.0f;
+          for (uint32_t p = 0; p < L; ++p) {
+            float g = gp[(uint64_t)c * L + p];
+            if (g != 0.0f) acc = acc + g * syn1[(uint64_t)t[p] * dim + i];
+          }
+          for (uint32_t k = 0; k < K; ++k) {
+            float g = gn[(uint64_t)c * K + k];
+            if (g != 0.0f) acc = acc + g * syn1[(uint64_t)neg[k] * dim + i];
+          }
+          v[i] = acc;
+        }
+        cs[ns].row = (uint32_t)t[c];
+        cs[ns].table = 0;
+        cs[ns].vec = v;
+        ++ns;
+      }
+      /* dYt */
+      for (uint32_t p = 0; p < L; ++p) {
+        float* v = vecs + (ns) * dim;
+        for (uint32_t i = 0; i < dim; ++i) {
+          float acc = 0.0f;
+          for (uint32_t c = 0; c < L; ++c) {
+            float g = gp[(uint64_t)c * L + p];
+            if (g != 0.0f) acc = acc + g * syn0[(uint64_t)t[c] * dim + i];
+          }
+          v[i] = acc;
+        }
+        cs[ns].row = (uint32_t)t[p];
+        cs[ns].table = 1;
+        cs[ns].vec = v;
+        ++ns;
+      }
+      /* dYn */
+      for (uint32_t k = 0; k < K; ++k) {
+        float* v = vecs + (ns) * dim;
+        for (uint32_t i = 0; i < dim; ++i) {
+          float acc = 0.0f;
+          for (uint32_t c = 0; c < L; ++c) {
+            float g = gn[(uint64_t)c * K + k];
+            if (g != 0.0f) acc = acc + g * syn0[(uint64_t)t[c] * dim + i];
+          }
+          v[i] = acc;
+        }
+        cs[ns].row = (uint32_t)neg[k];
+        cs[ns].table = 1;
+        cs[ns].vec = v;
+        ++ns;
+      }
+      free(gp);
+      free(gn);
+    }
+    /* apply: per row, contributions in slot order, summed then added once */
+    qsort(cs, ns, sizeof(contrib_t), cmp_contrib);
+    float* acc = (float*)malloc(sizeof(float) * dim);
+    uint64_t i = 0;
+    while (i < ns) {
+      uint64_t j = i;
+      for (uint32_t d = 0; d < dim; ++d) acc[d] = 0.0f;
+      while (j < ns && cs[j].row == cs[i].row && cs[j].table == cs[i].table) {
+        for (uint32_t d = 0; d < dim; ++d) acc[d] = acc[d] + cs[j].vec[d];
+        ++j;
+      }
+      float* w = (cs[i].table == 0 ? syn0 : syn1) + (uint64_t)cs[i].row * dim;
+      for (uint32_t d = 0; d < dim; ++d) w[d] = w[d] + acc[d];
+      i = j;
+    }
+    free(acc);
+    free(cs);
+    free(vecs);
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Tuple loss for the differentiation checks (FP64):
+ *   L = sum_t [ -log sig(x_ctx . y_center) - sum_k log sig(-x_ctx . y_negk) ]
+ * with x = input rows (syn0), y = output rows (syn1), tuples laid out as
+ * (center, context, neg_1..neg_K).
+ * ---------------------------------------------------------------------- */
+double dgo_tuple_loss(const uint32_t* tuples, uint64_t n, uint32_t K, const double* syn0,
+                      const double* syn1, uint32_t dim) {
+  double loss = 0.0;
+  uint32_t w = 2 + K;
+  for (uint64_t t = 0; t < n; ++t) {
+    const double* x = syn0 + (uint64_t)tuples[t * w + 1] * dim;
+    for (uint32_t j = 0; j < 1 + K; ++j) {
+      uint32_t r = j == 0 ? tuples[t * w] : tuples[t * w + 1 + j];
+      const double* y = syn1 + (uint64_t)r * dim;
+      double z = 0.0;
+      for (uint32_t d = 0; d < dim; ++d) z += x[d] * y[d];
+      /* -log sig(s z) = log(1 + exp(-s z)), s = +1 for the centre, -1 for negatives */
+      double a = j == 0 ? -z : z;
+      loss += a > 0 ? a + log1p(exp(-a)) : log1p(exp(a));
+    }
+  }
+  return loss;
+}
