@@ -132,3 +132,50 @@ extern "C" int dg_timing_reset(dg_ctx* ctx) {
   ctx->timing.clear();
   return DG_OK;
 }
+
+// ---- HBM copy calibration ----------------------------------------------------
+// A plain streaming copy of known size; its measured bandwidth and its hardware
+// byte counters calibrate the counters of the other kernels.
+__global__ __launch_bounds__(256) void dg_copy_kernel(const uint4* __restrict__ src,
+                                                      uint4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" int dg_copy_bench(dg_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps) {
+  if (!ctx || bytes < 16 || iters == 0) return DG_ERR_ARG;
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t n = bytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  DG_HIP(ctx, hipMalloc((void**)&a, n * 16));
+  hipError_t e = hipMalloc((void**)&b, n * 16);
+  if (e != hipSuccess) {
+    hipFree(a);
+    return dg_fail(ctx, DG_ERR_NOMEM, "copy buffer: %s", hipGetErrorString(e));
+  }
+  hipMemsetAsync(a, 1, n * 16, ctx->stream);
+  const unsigned grid = (unsigned)(ctx->num_cus * 8);
+  size_t tok;
+  int rc = DG_OK;
+  for (uint32_t it = 0; it < iters && rc == DG_OK; ++it) {
+    rc = dg_time_begin(ctx, "copy", &tok);
+    if (rc == DG_OK) {
+      dg_copy_kernel<<<grid, 256, 0, ctx->stream>>>(a, b, n);
+      rc = dg_time_end(ctx, tok);
+    }
+  }
+  uint64_t launches = 0;
+  double ms = 0.0;
+  if (rc == DG_OK) rc = dg_time_resolve(ctx);
+  if (rc == DG_OK) {
+    auto& t = ctx->timing["copy"];
+    launches = t.launches;
+    ms = t.total_ms;
+  }
+  hipFree(a);
+  hipFree(b);
+  if (rc != DG_OK) return rc;
+  if (gbps) *gbps = launches ? (2.0 * (double)(n * 16) * (double)launches) / (ms * 1e-3) / 1e9 : 0.0;
+  return DG_OK;
+}

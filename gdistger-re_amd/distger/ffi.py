@@ -67,6 +67,51 @@ class RoundStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+DG_SGNS_PLAIN_FP32 = 0
+DG_SGNS_BATCHED_FP32 = 1
+DG_SGNS_BATCHED_BF16 = 2
+DG_VOCAB_DEGREE = 0
+DG_VOCAB_VISITS = 1
+DG_ALPHA_LINEAR = 0
+DG_ALPHA_FIXED = 1
+DG_CORPUS_ROUND = 0
+DG_CORPUS_TRAIN = 1
+
+
+class SgnsParams(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_uint32),
+        ("window", ctypes.c_uint32),
+        ("negative", ctypes.c_uint32),
+        ("min_count", ctypes.c_uint32),
+        ("iter", ctypes.c_uint32),
+        ("batch_size", ctypes.c_uint32),
+        ("sample", ctypes.c_float),
+        ("alpha", ctypes.c_float),
+        ("form", ctypes.c_uint32),
+        ("vocab_source", ctypes.c_uint32),
+        ("alpha_schedule", ctypes.c_uint32),
+        ("deterministic", ctypes.c_uint32),
+        ("exact_sigmoid", ctypes.c_uint32),
+        ("minibatch", ctypes.c_uint32),
+        ("table_size", ctypes.c_uint32),
+        ("init_float_ops", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class SgnsStats(ctypes.Structure):
+    _fields_ = [
+        ("words", ctypes.c_uint64),
+        ("kept", ctypes.c_uint64),
+        ("sentences", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("ms", ctypes.c_double),
+        ("alpha_end", ctypes.c_float),
+        ("reserved", ctypes.c_float),
+    ]
+
+
 _lib = None
 
 
@@ -121,6 +166,21 @@ def load_library():
         "dg_lp_split": (c.c_int, [c.c_uint32, vp, c.c_uint64, c.c_double, c.c_uint64,
                                   c.POINTER(c.c_uint64), vp, c.c_uint64, c.POINTER(c.c_uint64)]),
         "dg_auc": (c.c_double, [vp, c.c_uint64, vp, c.c_uint64]),
+        "dg_copy_bench": (c.c_int, [vp, c.c_uint64, c.c_uint32, c.POINTER(c.c_double)]),
+        "dg_sgns_params_default": (None, [c.POINTER(SgnsParams)]),
+        "dg_sgns_init": (c.c_int, [vp, c.POINTER(SgnsParams)]),
+        "dg_sgns_train": (c.c_int, [vp, c.c_int, c.POINTER(SgnsParams), c.c_float,
+                                    c.POINTER(SgnsStats)]),
+        "dg_sgns_vocab": (c.c_int, [vp, vp, c.POINTER(c.c_uint32)]),
+        "dg_sgns_get_tables": (c.c_int, [vp, vp, vp]),
+        "dg_sgns_set_tables": (c.c_int, [vp, vp, vp]),
+        "dg_sgns_score_pairs": (c.c_int, [vp, vp, vp, c.c_uint64, vp]),
+        "dg_emb_write": (c.c_int, [vp, c.c_char_p, c.c_int]),
+        "dg_sgns_tuples_forward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, vp,
+                                             c.POINTER(c.c_double)]),
+        "dg_sgns_tuples_backward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, c.c_float,
+                                              c.c_int]),
+        "dg_sgns_grad_rows": (c.c_int, [vp, c.c_int, vp, c.c_uint64, vp]),
         "dg_write_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64]),
         "dg_read_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64, c.POINTER(c.c_uint64)]),
     }
@@ -261,6 +321,77 @@ class Context:
 
     def train_corpus_clear(self):
         self._check(self.L.dg_train_corpus_clear(self.h))
+
+    # ---- SGNS trainer
+    def default_sgns_params(self, **kw):
+        p = SgnsParams()
+        self.L.dg_sgns_params_default(ctypes.byref(p))
+        for k, val in kw.items():
+            setattr(p, k, val)
+        return p
+
+    def sgns_init(self, params):
+        self._check(self.L.dg_sgns_init(self.h, ctypes.byref(params)))
+
+    def sgns_train(self, params, corpus=DG_CORPUS_TRAIN, alpha=0.0):
+        st = SgnsStats()
+        self._check(self.L.dg_sgns_train(self.h, corpus, ctypes.byref(params), float(alpha),
+                                         ctypes.byref(st)))
+        return st
+
+    def sgns_vocab(self):
+        n = ctypes.c_uint32()
+        self._check(self.L.dg_sgns_vocab(self.h, None, ctypes.byref(n)))
+        v = np.zeros(n.value, np.uint32)
+        self._check(self.L.dg_sgns_vocab(self.h, _vp(v), ctypes.byref(n)))
+        return v
+
+    def sgns_tables(self, dim):
+        n = len(self.sgns_vocab())
+        s0 = np.zeros((n, dim), np.float32)
+        s1 = np.zeros((n, dim), np.float32)
+        self._check(self.L.dg_sgns_get_tables(self.h, _vp(s0), _vp(s1)))
+        return s0, s1
+
+    def sgns_set_tables(self, syn0, syn1):
+        s0 = np.ascontiguousarray(syn0, dtype=np.float32)
+        s1 = np.ascontiguousarray(syn1, dtype=np.float32)
+        self._check(self.L.dg_sgns_set_tables(self.h, _vp(s0), _vp(s1)))
+
+    def sgns_score_pairs(self, src, dst):
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        out = np.zeros(len(src), np.float32)
+        self._check(self.L.dg_sgns_score_pairs(self.h, _vp(src), _vp(dst), len(src), _vp(out)))
+        return out
+
+    def emb_write(self, path, binary=False):
+        self._check(self.L.dg_emb_write(self.h, path.encode(), int(binary)))
+
+    def tuples_forward(self, tuples, K, bf16=False):
+        t = np.ascontiguousarray(tuples, dtype=np.uint32)
+        n = t.shape[0]
+        logits = np.zeros(n * (1 + K), np.float32)
+        loss = ctypes.c_double()
+        self._check(self.L.dg_sgns_tuples_forward(self.h, _vp(t), n, K, int(bf16), _vp(logits),
+                                                  ctypes.byref(loss)))
+        return logits.reshape(n, 1 + K), loss.value
+
+    def tuples_backward(self, tuples, K, bf16=False, lr=0.0, apply=False):
+        t = np.ascontiguousarray(tuples, dtype=np.uint32)
+        self._check(self.L.dg_sgns_tuples_backward(self.h, _vp(t), t.shape[0], K, int(bf16),
+                                                   float(lr), int(apply)))
+
+    def grad_rows(self, table, rows, dim):
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        out = np.zeros((len(r), dim), np.float32)
+        self._check(self.L.dg_sgns_grad_rows(self.h, table, _vp(r), len(r), _vp(out)))
+        return out
+
+    def copy_bench(self, nbytes, iters=3):
+        g = ctypes.c_double()
+        self._check(self.L.dg_copy_bench(self.h, int(nbytes), iters, ctypes.byref(g)))
+        return g.value
 
     # ---- timing
     def timing(self, name):

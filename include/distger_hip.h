@@ -174,6 +174,98 @@ int dg_corpus_append(dg_ctx* ctx);
 int dg_train_corpus_info(dg_ctx* ctx, uint64_t* n_walks, uint64_t* n_tokens);
 int dg_train_corpus_clear(dg_ctx* ctx);
 
+/* ---- skip-gram negative-sampling trainer (S1-S9) ------------------------ */
+
+#define DG_SGNS_PLAIN_FP32 0   /* skip_gram_kernel semantics (word2vec.cu:305-399), FP32 */
+#define DG_SGNS_BATCHED_FP32 1 /* shared-negative position tiles, FP32 rows */
+#define DG_SGNS_BATCHED_BF16 2 /* same on BF16 rows and BF16 MFMA, FP32 master rows */
+
+#define DG_VOCAB_DEGREE 0 /* word counts = in + out degree (word2vec.cu:791-809, huge_walk.cpp:43-46) */
+#define DG_VOCAB_VISITS 1 /* word counts = walk visit counts (dsgl.hpp:314-382) */
+
+#define DG_ALPHA_LINEAR 0 /* alpha0 * max(1e-4, 1 - words_done / (iter * words + 1)) (dsgl.hpp:886-897) */
+#define DG_ALPHA_FIXED 1  /* alpha for the whole call (word2vec.cu:1290 sets it per walk file) */
+
+typedef struct {
+  uint32_t dim;            /* -size (multiple of 64, <= 512) */
+  uint32_t window;         /* -window */
+  uint32_t negative;       /* -negative (<= 16) */
+  uint32_t min_count;      /* -min-count */
+  uint32_t iter;           /* -iter (epochs over the corpus) */
+  uint32_t batch_size;     /* -batch-size: DSGL's flag, forced to 2*window+1 (dsgl.hpp:832); recorded */
+  float sample;            /* -sample */
+  float alpha;             /* -alpha (starting learning rate) */
+  uint32_t form;           /* DG_SGNS_* */
+  uint32_t vocab_source;   /* DG_VOCAB_* */
+  uint32_t alpha_schedule; /* DG_ALPHA_* */
+  uint32_t deterministic;  /* plain: sentences one after another in one wavefront;
+                              batched: sorted-key scatter-add per minibatch instead of
+                              concurrent atomic updates */
+  uint32_t exact_sigmoid;  /* 0: the reference's 1000-entry sigmoid table; 1: 1/(1+e^-f) */
+  uint32_t minibatch;      /* batched: position tiles per launch (0 = tuned default) */
+  uint32_t table_size;     /* unigram table entries (1e8 in the reference) */
+  uint32_t init_float_ops; /* 0: word2vec.cu init rounding (double), 1: dsgl.hpp (float) */
+  uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
+} dg_sgns_params;
+
+void dg_sgns_params_default(dg_sgns_params* p);
+
+/* Build the vocabulary (ranked by count, ties by vertex id), initialise
+ * syn0 (the reference LCG) and syn1 = 0, the unigram table and the sigmoid
+ * table.  Replaces TrainModel's set-up (word2vec.cu:1246-1283) and
+ * dsgl's preparatory_work (dsgl.hpp:511-519). */
+int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p);
+
+typedef struct {
+  uint64_t words;       /* corpus tokens consumed, subsampled ones included (word2vec.cu:1011-1020) */
+  uint64_t kept;        /* tokens left after subsampling */
+  uint64_t sentences;   /* walks */
+  uint64_t launches;    /* trainer kernel launches */
+  double ms;            /* device time of the training kernels */
+  float alpha_end;      /* learning rate of the last launch */
+  float reserved;
+} dg_sgns_stats;
+
+#define DG_CORPUS_ROUND 0 /* the current round's corpus (one walk file of the reference) */
+#define DG_CORPUS_TRAIN 1 /* the appended training corpus */
+
+/* Train on a device corpus for p->iter epochs.  `alpha` overrides p->alpha
+ * when > 0 (the reference sets a learning rate per walk file). */
+int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* p, float alpha,
+                  dg_sgns_stats* out);
+
+/* Vocabulary: vertex id of each rank, and its size. */
+int dg_sgns_vocab(dg_ctx* ctx, uint32_t* vertex_of_rank, uint32_t* vocab_size);
+
+/* Copy or replace the FP32 (master) tables, rows in rank order, vocab x dim. */
+int dg_sgns_get_tables(dg_ctx* ctx, float* syn0, float* syn1);
+int dg_sgns_set_tables(dg_ctx* ctx, const float* syn0, const float* syn1);
+
+/* Link-prediction scores: dot(syn0[u], syn0[v]) for vertex pairs (rank
+ * lookup on the device; vertices outside the vocabulary score 0). */
+int dg_sgns_score_pairs(dg_ctx* ctx, const uint32_t* src, const uint32_t* dst, uint64_t n,
+                        float* scores);
+
+/* Embedding file (dsgl.hpp:1443-1461): "V d\n", then per rank "id " and d
+ * values "%f " (text) or d raw floats (binary), then "\n". */
+int dg_emb_write(dg_ctx* ctx, const char* path, int binary);
+
+/* ---- differentiable tuple interface (forward / hand-written adjoint) ----
+ * A tuple is (center, context, neg_1..neg_K) in vocabulary ranks, K =
+ * p->negative.  Loss per tuple: -log sig(x_ctx . y_center) - sum_k log
+ * sig(-x_ctx . y_negk), x = syn0 rows, y = syn1 rows, exact sigmoid.
+ * forward: logits[n*(1+K)] and the summed loss.
+ * backward: dL/dlogit = sig(z) - label, then row gradients reduced by a
+ * sorted-key segmented sum (deterministic); with apply != 0 the rows are
+ * updated W -= lr * grad, else the gradients are kept for dg_sgns_grad_rows. */
+int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K,
+                           int bf16, float* logits, double* loss);
+int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
+                            float lr, int apply);
+/* Gradient rows of the last backward: `table` 0 = syn0, 1 = syn1, rows given
+ * by rank; out[n_rows * dim] (rows that got no gradient read 0). */
+int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, uint64_t n_rows, float* out);
+
 /* ---- timing ------------------------------------------------------------ */
 
 /* Per-kernel device time accumulated with HIP events on the context's stream
@@ -181,6 +273,11 @@ int dg_train_corpus_clear(dg_ctx* ctx);
  * "cn", "sgns_plain", "sgns_fwd", ...).  Returns launches and total ms. */
 int dg_timing_get(dg_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
 int dg_timing_reset(dg_ctx* ctx);
+
+/* Streaming device-to-device copy of `bytes` (kernel dg_copy_kernel), `iters`
+ * times; returns read+write GB/s.  Calibrates hardware byte counters of the
+ * other kernels.  No reference counterpart. */
+int dg_copy_bench(dg_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps);
 
 /* ---- host data utilities (synthetic inputs, file formats) -------------- */
 

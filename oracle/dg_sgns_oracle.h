@@ -26,6 +26,7 @@ void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_
                         uint32_t dim, float* syn0, float* syn1);
 uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,
                            uint64_t cap);
+uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen);
 void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
                     uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
                     const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,

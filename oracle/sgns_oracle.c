@@ -241,35 +241,37 @@ uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* ba
 }
 
 /* ------------------------------------------------------------------------
- * Batched form: sentence minibatches with per-sentence shared negatives
- * (the semantics of __sgNegReuse, src/word2vec.cu:231-303, every window of a
- * sentence sharing K negatives), all gradients of a minibatch taken from the
- * table snapshot at its start and applied as one sum per row.
+ * Batched form: tiles of DGO_TILE consecutive positions of a sentence share
+ * K negatives (the sharing of __sgNegReuse, src/word2vec.cu:231-303, applied
+ * per tile), and every gradient of a launch of `minibatch` tiles is taken
+ * from the tables as they were at its start; the row sums are then added
+ * once.  Arithmetic in double, rows stored as float (the FP64 reference the
+ * FP32 and BF16 device forms are measured against).
  *
- * Per sentence s of length L (>= 2): the Philox stream (ctr = (blk, s lo,
- * s hi ^ epoch<<24, SGNS)) gives, in order, b_p = next % window for p = 0..L-1
- * and K raw words x_k, negative n_k = table[x_k % table_size] (0 -> x_k %
- * (vocab-1) + 1).
- *   A[c][p] = x_c . y_{t_p}  (c in win(p)),  B[c][k] = x_c . y_{n_k}
- *   gp[c][p] = (1 - sig(A)) alpha,  gn[c][k] = -sig(B) alpha * m[c][k],
- *   m[c][k] = #{p : c in win(p), n_k != t_p}
- *   dX[c] = sum_p gp[c][p] y_{t_p} + sum_k gn[c][k] y_{n_k}   (p, then k ascending)
- *   dYt[p] = sum_c gp[c][p] x_c ; dYn[k] = sum_c gn[c][k] x_c  (c ascending)
- * Row sums: contributions in (sentence, slot) order, slots dX[0..L),
- * dYt[0..L), dYn[0..K); W[r] = W0[r] + (((c1 + c2) + c3) ...).
- * sig is the reference table sigmoid (exact_sigmoid = 0) or 1/(1+e^-f).
+ * Tile (s, j): positions p0 = 22 j .. p1 = min(p0 + 22, L) - 1 of sentence s
+ * (L >= 2).  Philox stream (ctr = (blk, gs, j | epoch << 16, SGNS)), gs =
+ * first_sen + s: b_p = next % window for p = p0..p1-1, then K words x_k,
+ * negative n_k = table[x_k % table_size] (0 -> x_k % (vocab-1) + 1).
+ * Contexts c in [max(0, p0-W), min(L, p1+W)).
+ *   gp[c][p] = (1 - sig(x_c . y_{t_p})) alpha          for c in win(p)
+ *   gn[c][k] = (0 - sig(x_c . y_{n_k})) alpha m[c][k],  m = #{p : c in win(p), n_k != t_p}
+ *   dX[c] = sum_p gp y_{t_p} + sum_k gn y_{n_k};  dYt[p] = sum_c gp x_c;  dYn[k] = sum_c gn x_c
+ * win(p) = {c != p : |c - p| <= W - b_p}.
  * ---------------------------------------------------------------------- */
-static float sig_value(float f, const float* expt, int exact) {
-  if (exact) return (float)(1.0 / (1.0 + exp(-(double)f)));
-  if (f > MAX_EXP) return 1.0f;
-  if (f < -MAX_EXP) return 0.0f;
-  return expt[(int)((f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))];
+#define DGO_TILE 22
+
+static double sig_d(double f, const float* expt, int exact) {
+  if (exact) return 1.0 / (1.0 + exp(-f));
+  if (f > MAX_EXP) return 1.0;
+  if (f < -MAX_EXP) return 0.0;
+  return (double)expt[(int)((float)(f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))];
 }
 
 typedef struct {
   uint32_t row;
   uint32_t table; /* 0 = input (syn0), 1 = output (syn1) */
-  float* vec;
+  uint64_t order;
+  double* vec;
 } contrib_t;
 
 static int cmp_contrib(const void* a, const void* b) {
@@ -277,7 +279,22 @@ static int cmp_contrib(const void* a, const void* b) {
   const contrib_t* y = (const contrib_t*)b;
   if (x->table != y->table) return x->table < y->table ? -1 : 1;
   if (x->row != y->row) return x->row < y->row ? -1 : 1;
-  return (x->vec > y->vec) - (x->vec < y->vec); /* slot order = allocation order */
+  return (x->order > y->order) - (x->order < y->order);
+}
+
+static double ddot(const float* x, const float* y, uint32_t dim) {
+  double s = 0.0;
+  for (uint32_t d = 0; d < dim; ++d) s += (double)x[d] * (double)y[d];
+  return s;
+}
+
+uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen) {
+  uint64_t n = 0;
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    uint64_t L = sen_off[s + 1] - sen_off[s];
+    if (L >= 2) n += (L + DGO_TILE - 1) / DGO_TILE;
+  }
+  return n;
 }
 
 void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
@@ -285,130 +302,118 @@ void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_
                     const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
                     const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
                     uint32_t dim, float* syn0, float* syn1) {
-  uint32_t K = negative;
-  for (uint64_t mb0 = 0; mb0 < n_sen; mb0 += minibatch) {
-    uint64_t mb1 = mb0 + minibatch < n_sen ? mb0 + minibatch : n_sen;
-    /* count slots */
-    uint64_t slots = 0;
-    for (uint64_t s = mb0; s < mb1; ++s) {
-      uint64_t L = sen_off[s + 1] - sen_off[s];
-      if (L >= 2) slots += 2 * L + K;
+  const int K = (int)negative, W = (int)window;
+  /* tile list in order */
+  uint64_t n_tiles = dgo_tile_count(sen_off, n_sen);
+  uint64_t* ts = (uint64_t*)malloc(sizeof(uint64_t) * (n_tiles ? n_tiles : 1));
+  uint32_t* tj = (uint32_t*)malloc(sizeof(uint32_t) * (n_tiles ? n_tiles : 1));
+  uint64_t nt = 0;
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    uint64_t L = sen_off[s + 1] - sen_off[s];
+    if (L < 2) continue;
+    for (uint32_t j = 0; j * DGO_TILE < L; ++j) {
+      ts[nt] = s;
+      tj[nt] = j;
+      ++nt;
     }
-    contrib_t* cs = (contrib_t*)malloc(sizeof(contrib_t) * (slots ? slots : 1));
-    float* vecs = (float*)calloc(slots ? slots * dim : 1, sizeof(float));
+  }
+  for (uint64_t t0 = 0; t0 < n_tiles; t0 += minibatch) {
+    uint64_t t1 = t0 + minibatch < n_tiles ? t0 + minibatch : n_tiles;
+    uint64_t cap = (t1 - t0) * (2 * DGO_TILE + 2 * (uint64_t)W + (uint64_t)K);
+    contrib_t* cs = (contrib_t*)malloc(sizeof(contrib_t) * (cap ? cap : 1));
+    double* vecs = (double*)calloc(cap ? cap * dim : 1, sizeof(double));
     uint64_t ns = 0;
-    for (uint64_t s = mb0; s < mb1; ++s) {
+    for (uint64_t ti = t0; ti < t1; ++ti) {
+      uint64_t s = ts[ti];
       const int32_t* t = sen + sen_off[s];
-      uint32_t L = (uint32_t)(sen_off[s + 1] - sen_off[s]);
-      if (L < 2) continue;
+      int L = (int)(sen_off[s + 1] - sen_off[s]);
+      int p0 = (int)tj[ti] * DGO_TILE, p1 = p0 + DGO_TILE < L ? p0 + DGO_TILE : L;
       uint64_t gs = first_sen + s;
       dg_stream st;
-      dg_stream_init(&st, seed, (uint32_t)gs, (uint32_t)(gs >> 32) ^ (epoch << 24), DG_TAG_SGNS);
-      uint32_t b[4096];
+      dg_stream_init(&st, seed, (uint32_t)gs, tj[ti] | (epoch << 16), DG_TAG_SGNS);
+      int b[DGO_TILE];
       int32_t neg[64];
-      for (uint32_t p = 0; p < L; ++p) b[p] = dg_stream_next(&st) % window;
-      for (uint32_t k = 0; k < K; ++k) {
+      for (int p = p0; p < p1; ++p) b[p - p0] = (int)(dg_stream_next(&st) % window);
+      for (int k = 0; k < K; ++k) {
         uint32_t x = dg_stream_next(&st);
         int32_t n = table[x % table_size];
         if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
         neg[k] = n;
       }
-      float* gp = (float*)calloc((size_t)L * L, sizeof(float));
-      float* gn = (float*)calloc((size_t)L * K + 1, sizeof(float));
-      for (uint32_t p = 0; p < L; ++p) {
-        int64_t lo = (int64_t)p - (int64_t)window + b[p], hi = (int64_t)p + (int64_t)window - b[p];
-        for (int64_t c = lo; c <= hi; ++c) {
-          if (c < 0 || c >= L || c == p) continue;
-          float f = tree_dot(syn0 + (uint64_t)t[c] * dim, syn1 + (uint64_t)t[p] * dim, dim);
-          gp[(uint64_t)c * L + p] = (1.0f - sig_value(f, expt, exact_sigmoid)) * alpha;
+      int c_lo = p0 - W > 0 ? p0 - W : 0, c_hi = p1 + W < L ? p1 + W : L;
+      int nc = c_hi - c_lo, np = p1 - p0;
+      double* gp = (double*)calloc((size_t)nc * np + 1, sizeof(double));
+      double* gn = (double*)calloc((size_t)nc * (K + 1), sizeof(double));
+      for (int c = c_lo; c < c_hi; ++c) {
+        const float* x = syn0 + (uint64_t)t[c] * dim;
+        int m[64] = {0};
+        for (int p = p0; p < p1; ++p) {
+          int w = W - b[p - p0];
+          if (c == p || c < p - w || c > p + w) continue;
+          for (int k = 0; k < K; ++k) m[k] += (neg[k] != t[p]);
+          double f = ddot(x, syn1 + (uint64_t)t[p] * dim, dim);
+          gp[(size_t)(c - c_lo) * np + (p - p0)] = (1.0 - sig_d(f, expt, exact_sigmoid)) * alpha;
+        }
+        for (int k = 0; k < K; ++k) {
+          if (!m[k]) continue;
+          double f = ddot(x, syn1 + (uint64_t)neg[k] * dim, dim);
+          gn[(size_t)(c - c_lo) * K + k] = (0.0 - sig_d(f, expt, exact_sigmoid)) * alpha * m[k];
         }
       }
-      for (uint32_t c = 0; c < L; ++c) {
-        for (uint32_t k = 0; k < K; ++k) {
-          uint32_t m = 0;
-          for (uint32_t p = 0; p < L; ++p) {
-            int64_t lo = (int64_t)p - (int64_t)window + b[p], hi = (int64_t)p + (int64_t)window - b[p];
-            if ((int64_t)c >= lo && (int64_t)c <= hi && c != p && neg[k] != t[p]) ++m;
-          }
-          if (m == 0) continue;
-          float f = tree_dot(syn0 + (uint64_t)t[c] * dim, syn1 + (uint64_t)neg[k] * dim, dim);
-          gn[(uint64_t)c * K + k] = (0.0f - sig_value(f, expt, exact_sigmoid)) * alpha * (float)m;
+      for (int c = c_lo; c < c_hi; ++c) {
+        double* v = vecs + ns * dim;
+        for (int p = 0; p < np; ++p) {
+          double g = gp[(size_t)(c - c_lo) * np + p];
+          if (g != 0.0)
+            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn1[(uint64_t)t[p0 + p] * dim + d];
         }
-      }
-      /* dX */
-      for (uint32_t c = 0; c < L; ++c) {
-        float* v = vecs + (ns) * dim;
-        for (uint32_t i = 0; i < dim; ++i) {
-          float acc = 0.0f;
-          for (uint32_t p = 0; p < L; ++p) {
-            float g = gp[(uint64_t)c * L + p];
-            if (g != 0.0f) acc = acc + g * syn1[(uint64_t)t[p] * dim + i];
-          }
-          for (uint32_t k = 0; k < K; ++k) {
-            float g = gn[(uint64_t)c * K + k];
-            if (g != 0.0f) acc = acc + g * syn1[(uint64_t)neg[k] * dim + i];
-          }
-          v[i] = acc;
+        for (int k = 0; k < K; ++k) {
+          double g = gn[(size_t)(c - c_lo) * K + k];
+          if (g != 0.0)
+            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn1[(uint64_t)neg[k] * dim + d];
         }
         cs[ns].row = (uint32_t)t[c];
         cs[ns].table = 0;
+        cs[ns].order = ns;
         cs[ns].vec = v;
         ++ns;
       }
-      /* dYt */
-      for (uint32_t p = 0; p < L; ++p) {
-        float* v = vecs + (ns) * dim;
-        for (uint32_t i = 0; i < dim; ++i) {
-          float acc = 0.0f;
-          for (uint32_t c = 0; c < L; ++c) {
-            float g = gp[(uint64_t)c * L + p];
-            if (g != 0.0f) acc = acc + g * syn0[(uint64_t)t[c] * dim + i];
-          }
-          v[i] = acc;
+      for (int p = 0; p < np + K; ++p) {
+        double* v = vecs + ns * dim;
+        for (int c = 0; c < nc; ++c) {
+          double g = p < np ? gp[(size_t)c * np + p] : gn[(size_t)c * K + (p - np)];
+          if (g != 0.0)
+            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn0[(uint64_t)t[c_lo + c] * dim + d];
         }
-        cs[ns].row = (uint32_t)t[p];
+        cs[ns].row = p < np ? (uint32_t)t[p0 + p] : (uint32_t)neg[p - np];
         cs[ns].table = 1;
-        cs[ns].vec = v;
-        ++ns;
-      }
-      /* dYn */
-      for (uint32_t k = 0; k < K; ++k) {
-        float* v = vecs + (ns) * dim;
-        for (uint32_t i = 0; i < dim; ++i) {
-          float acc = 0.0f;
-          for (uint32_t c = 0; c < L; ++c) {
-            float g = gn[(uint64_t)c * K + k];
-            if (g != 0.0f) acc = acc + g * syn0[(uint64_t)t[c] * dim + i];
-          }
-          v[i] = acc;
-        }
-        cs[ns].row = (uint32_t)neg[k];
-        cs[ns].table = 1;
+        cs[ns].order = ns;
         cs[ns].vec = v;
         ++ns;
       }
       free(gp);
       free(gn);
     }
-    /* apply: per row, contributions in slot order, summed then added once */
     qsort(cs, ns, sizeof(contrib_t), cmp_contrib);
-    float* acc = (float*)malloc(sizeof(float) * dim);
+    double* acc = (double*)malloc(sizeof(double) * dim);
     uint64_t i = 0;
     while (i < ns) {
       uint64_t j = i;
-      for (uint32_t d = 0; d < dim; ++d) acc[d] = 0.0f;
+      for (uint32_t d = 0; d < dim; ++d) acc[d] = 0.0;
       while (j < ns && cs[j].row == cs[i].row && cs[j].table == cs[i].table) {
-        for (uint32_t d = 0; d < dim; ++d) acc[d] = acc[d] + cs[j].vec[d];
+        for (uint32_t d = 0; d < dim; ++d) acc[d] += cs[j].vec[d];
         ++j;
       }
       float* w = (cs[i].table == 0 ? syn0 : syn1) + (uint64_t)cs[i].row * dim;
-      for (uint32_t d = 0; d < dim; ++d) w[d] = w[d] + acc[d];
+      for (uint32_t d = 0; d < dim; ++d) w[d] = (float)((double)w[d] + acc[d]);
       i = j;
     }
     free(acc);
     free(cs);
     free(vecs);
   }
+  free(ts);
+  free(tj);
 }
 
 /* ------------------------------------------------------------------------
