@@ -31,7 +31,9 @@ extern "C" int dg_open(int device, dg_ctx** out) {
     return DG_ERR_HIP;
   }
   ctx->num_cus = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  // A blocking stream: work the library queues here is ordered with the
+  // synchronous hipMemcpy/hipMemset calls it makes on the null stream.
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamDefault) != hipSuccess) {
     delete ctx;
     return DG_ERR_HIP;
   }
