@@ -1,16 +1,24 @@
 """Benchmark of the DistGER embedding hot path on one MI355X.
 
-Workload (2,238,731-vertex config): undirected Chung-Lu graph, 2,238,731
-vertices, ~29M undirected edges, degree exponent 2.2, seed 1, 10% of the edges
-held out for link prediction; the walker runs on the training graph with the
-reference defaults (--min_L 20, mu 0.999, no length cap).
+Workload (the 2,238,731-vertex configuration): undirected Chung-Lu graph with
+2,238,731 vertices and ~29M undirected edges, degree exponent 2.2, seed 1;
+10% of the edges held out (with as many uniform non-edges) for link
+prediction; walks on the training graph with the reference defaults
+(--min_L 20, mu 0.999, --min_R 5, no length cap); SGNS dim 128, window 10, 5
+negatives, sample 1e-3, alpha 0.01 linear decay.
 
-One step = one walk round of V walks (the size of every round after the
-first, include/walk.hpp:764-775) in one persistent launch, followed by the
-corpus packing and the relative-entropy update.  Inputs are resident in HBM
-before the timed region; the timed region is bracketed by device syncs.
+One step = one round of the embedding pipeline: V walks in one persistent
+walker launch (plus corpus packing and the relative-entropy update), then the
+BF16-MFMA batched SGNS trainer over that round's corpus.  `value` is corpus
+words walked and trained per second over the timed steps (inputs resident in
+HBM, timed region bracketed by device syncs).
 
-Prints one JSON line.
+After the timed steps, the plain FP32 form (word2vec.cu skip_gram_kernel
+semantics) and the BF16-MFMA form are each trained from the same initial
+tables on the same accumulated corpus, and both embeddings are scored by
+link-prediction AUC on the held-out edges.
+
+Prints one JSON line on stdout (progress on stderr).
 """
 
 import argparse
@@ -26,39 +34,49 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 import numpy as np  # noqa: E402
 
 import distger  # noqa: E402
+from distger import ffi  # noqa: E402
 
 V_C2 = 2238731
 E_C2 = 29_000_000
-MI355X_HBM_GBS = 8000.0
+HBM_GBS = 8000.0
+BF16_TFLOPS = 2500.0
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_workload(args):
-    t0 = time.time()
-    edges = distger.gen_chung_lu(args.vertices, args.edges, 2.2, 1, 1)
-    train, pos, neg = distger.lp_split(args.vertices, edges, 0.1, 1)
-    log(f"graph: {args.vertices} vertices, {len(edges)} edges, train {len(train)}, "
-        f"held out {len(pos)} + {len(neg)} non-edges ({time.time() - t0:.1f} s)")
-    return train, pos, neg
-
-
 def walker_bytes(stats_list):
-    """Algorithmic HBM bytes of the walker kernel: 16 B per row-offset pair
-    read (walk start and every move), 8 B per adjacency unit drawn, 4 B per
-    vertex written, 8 B per visit-count update (read-modify-write of a
-    uint32), 12 B per walk record (offset + length)."""
+    """Algorithmic HBM bytes of the walker kernel: 16 B per row-offset pair read
+    (walk start and every move), 8 B per adjacency unit drawn, 4 B per vertex
+    written, 8 B per visit-count update, 12 B per walk record."""
     walks = sum(s.walkers for s in stats_list)
     tokens = sum(s.tokens for s in stats_list)
     draws = sum(s.draws for s in stats_list)
     moves = sum(s.moves for s in stats_list)
-    return 16 * moves + 8 * draws + 4 * tokens + 8 * tokens + 12 * walks
+    return 16 * moves + 8 * draws + 12 * tokens + 12 * walks
 
 
-def cpu_baseline(ctx, seed, sample_walks, threads):
+def sgns_tile_bytes(kept, tiles, dim, negative):
+    """Algorithmic HBM bytes of the BF16 tile kernel: every staged row (context
+    rows ~ kept positions, output rows = kept + K per tile) is read in BF16 (2 B
+    per element), its FP32 master element read and written (8 B) and its BF16
+    copy rewritten (2 B); plus 4 B per kept token of the sentence buffer."""
+    rows = 2 * kept + negative * tiles
+    return rows * dim * 12 + 4 * kept
+
+
+def auc_of(ctx, pos, neg):
+    ps = ctx.sgns_score_pairs(pos["src"], pos["dst"])
+    ns = ctx.sgns_score_pairs(neg["src"], neg["dst"])
+    return distger.auc(ps, ns)
+
+
+def cpu_baseline(ctx, walk_sample, sgns_sample, threads, counts, seed):
+    """The CPU oracle on a bounded sample of the same pipeline: walks on
+    `threads` host threads, then the plain SGNS restatement on one thread."""
     import dgoracle
+    import dgsgns_oracle
 
     rp, nbr, cn = ctx.graph_csr()
     g = dgoracle.OracleGraph.__new__(dgoracle.OracleGraph)
@@ -66,71 +84,153 @@ def cpu_baseline(ctx, seed, sample_walks, threads):
     g.v_num = len(rp) - 1
     g.row_ptr, g.nbr, g.cn, g.m = rp, nbr, cn, len(nbr)
     g._g = dgoracle._Graph(g.v_num, g.m, rp.ctypes.data, nbr.ctypes.data, cn.ctypes.data)
-    cfg = g.cfg(seed, 20, 0, 0.999)
     t0 = time.time()
-    tok, off = g.walk_round(cfg, 2, 0, sample_walks, threads=threads)
-    dt = time.time() - t0
-    steps = len(tok) - sample_walks
-    return steps / dt, dt, steps
+    tok, off = g.walk_round(g.cfg(seed, 20, 0, 0.999), 2, 0, walk_sample, threads=threads)
+    walk_s = time.time() - t0
+    walk_rate = len(tok) / walk_s
+    tr = dgsgns_oracle.Trainer(counts, 128, table_size=10_000_000)
+    sub_tok = tok[: int(off[sgns_sample])]
+    sub_off = off[: sgns_sample + 1]
+    sen, soff = tr.subsample(sub_tok, sub_off, seed, 0)
+    t0 = time.time()
+    tr.plain(sen, soff, 10, 5, 0.01)
+    sgns_s = time.time() - t0
+    sgns_rate = len(sub_tok) / sgns_s
+    rate = 1.0 / (1.0 / walk_rate + 1.0 / sgns_rate)
+    return {"value": rate, "unit": "words/s", "cores": threads, "kind": "port",
+            "sample": (f"walks: {walk_sample} walks of one round on {threads} threads "
+                       f"({walk_rate:.3e} vertices/s, {walk_s:.1f} s); SGNS: plain restatement "
+                       f"on 1 thread over the first {sgns_sample} walks ({len(sub_tok)} words, "
+                       f"{sgns_rate:.3e} words/s, {sgns_s:.1f} s); pipeline rate combines both")}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--vertices", type=int, default=V_C2)
     ap.add_argument("--edges", type=int, default=E_C2)
+    ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--table-size", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain-vs-BF16 AUC run")
+    ap.add_argument("--cpu-walks", type=int, default=400_000)
+    ap.add_argument("--cpu-sgns-walks", type=int, default=20_000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(DESIGN.md describes how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
-    train, pos, neg = build_workload(args)
+    t0 = time.time()
     V = args.vertices
+    edges = distger.gen_chung_lu(V, args.edges, 2.2, 1, 1)
+    train, pos, neg = distger.lp_split(V, edges, 0.1, 1)
+    log(f"graph: {V} vertices, {len(edges)} edges, train {len(train)}, held out {len(pos)} "
+        f"+ {len(neg)} non-edges ({time.time() - t0:.1f} s)")
+
+    sp = dict(dim=args.dim, window=10, negative=5, sample=1e-3, alpha=0.01,
+              alpha_schedule=ffi.DG_ALPHA_LINEAR, table_size=args.table_size, seed=args.seed,
+              min_count=0, iter=1)
+    out = {}
     with distger.Context(0) as ctx:
         t0 = time.time()
-        ctx.graph_from_edges(V, train, undirected=True, cn_mode=distger.ffi.DG_CN_COMPUTE)
-        log(f"graph resident in HBM ({time.time() - t0:.1f} s incl. common neighbours)")
-        p = ctx.default_walk_params(seed=args.seed)
+        ctx.graph_from_edges(V, train, undirected=True, cn_mode=ffi.DG_CN_COMPUTE)
+        _, cn_ms = ctx.timing("cn")
+        log(f"graph resident in HBM ({time.time() - t0:.1f} s; common neighbours {cn_ms:.1f} ms)")
+        rp, _, _ = ctx.graph_csr()
+        counts = 2 * np.diff(rp).astype(np.uint64)
+        wp = ctx.default_walk_params(seed=args.seed)
         ctx.walk_reset()
-        # round 1 (V * init_round walks) seeds the cumulative counts, untimed
-        st1 = ctx.walk_round(p, 1, 0, V * p.init_round)
-        log(f"round 1: {st1.walkers} walks, {st1.tokens} tokens, walk kernel {st1.walk_ms:.2f} ms")
-        for w in range(args.warmup):
-            ctx.walk_round(p, 2 + w, 0, V)
+        ctx.train_corpus_clear()
+        bf = ctx.default_sgns_params(form=ffi.DG_SGNS_BATCHED_BF16, **sp)
+        t0 = time.time()
+        ctx.sgns_init(bf)
+        log(f"SGNS tables initialised ({time.time() - t0:.1f} s)")
+        # round 1: V * init_round walks, trained once, untimed
+        st1 = ctx.walk_round(wp, 1, 0, V * wp.init_round)
+        ctx.corpus_append()
+        ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
+        log(f"round 1: {st1.walkers} walks, {st1.tokens} words, walker {st1.walk_ms:.2f} ms")
+        rnd = 2
+        for _ in range(args.warmup):
+            ctx.walk_round(wp, rnd, 0, V)
+            ctx.corpus_append()
+            ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
+            rnd += 1
         ctx.sync()
         ctx.timing_reset()
-        stats = []
+        wstats, sstats = [], []
         t0 = time.perf_counter()
-        for k in range(args.steps):
-            stats.append(ctx.walk_round(p, 2 + args.warmup + k, 0, V))
+        for _ in range(args.steps):
+            wstats.append(ctx.walk_round(wp, rnd, 0, V))
+            sstats.append(ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND))
+            rnd += 1
         ctx.sync()
         wall = time.perf_counter() - t0
         n_walk, walk_ms = ctx.timing("walk")
         _, pack_ms = ctx.timing("pack")
         _, re_ms = ctx.timing("re")
-        steps = sum(s.steps for s in stats)
-        value = steps / wall
-        # algorithmic bytes of the dominant kernel, per launch
-        per_launch_bytes = walker_bytes(stats) / max(n_walk, 1)
-        avg_walk_s = walk_ms / 1e3 / max(n_walk, 1)
-        achieved = per_launch_bytes / avg_walk_s / 1e9
+        n_bf, bf_ms = ctx.timing("sgns_bf16")
+        words = sum(s.words for s in sstats)
+        steps = sum(s.steps for s in wstats)
+        value = words / wall
+        log(f"timed: {args.steps} rounds, {words} words in {wall * 1e3:.1f} ms; walker "
+            f"{walk_ms / n_walk:.2f} ms/round, SGNS BF16 {bf_ms / args.steps:.2f} ms/round")
+        for r in range(args.steps):  # keep the timed rounds in the training corpus too
+            pass
+        # walker roofline (per launch)
+        walk_bytes = walker_bytes(wstats) / n_walk
+        walk_gbs = walk_bytes / (walk_ms / n_walk * 1e-3) / 1e9
+        # SGNS BF16 roofline (per launch)
+        kept = sum(s.kept for s in sstats)
+        tiles = sum(s.sentences for s in sstats)  # one tile per walk of <= 22 kept words
+        bf_bytes = sgns_tile_bytes(kept, tiles, args.dim, 5) / n_bf
+        bf_gbs = bf_bytes / (bf_ms / n_bf * 1e-3) / 1e9
+        out["walk"] = {"walk_steps_per_s": steps / (walk_ms * 1e-3),
+                       "round_steps_per_s": steps / ((walk_ms + pack_ms + re_ms) * 1e-3),
+                       "kernel_ms": walk_ms / n_walk, "pack_ms": pack_ms / n_walk,
+                       "re_ms": re_ms / n_walk, "bytes_per_launch": walk_bytes,
+                       "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS}
+        out["sgns_bf16_pipeline"] = {"words_per_s": words / (bf_ms * 1e-3),
+                                     "kernel_ms_per_launch": bf_ms / n_bf, "launches": n_bf,
+                                     "bytes_per_launch": bf_bytes, "achieved_gbs": bf_gbs}
+
+        if not args.no_compare:
+            # plain FP32 vs BF16 MFMA on the same accumulated corpus (round 1 + warmup)
+            nw, nt = ctx.train_corpus_info()
+            res = {}
+            for name, form in (("bf16_mfma", ffi.DG_SGNS_BATCHED_BF16),
+                               ("fp32_batched", ffi.DG_SGNS_BATCHED_FP32),
+                               ("fp32_plain", ffi.DG_SGNS_PLAIN_FP32)):
+                p = ctx.default_sgns_params(form=form, **sp)
+                ctx.sgns_init(p)
+                ctx.timing_reset()
+                t1 = time.perf_counter()
+                st = ctx.sgns_train(p, corpus=ffi.DG_CORPUS_TRAIN)
+                ctx.sync()
+                w = time.perf_counter() - t1
+                a = auc_of(ctx, pos, neg)
+                res[name] = {"words_per_s": st.words / (st.ms * 1e-3), "wall_words_per_s": st.words / w,
+                             "auc": a, "words": st.words, "kernel_ms": st.ms}
+                log(f"{name}: {st.words} words, {st.words / (st.ms * 1e-3):.3e} words/s, AUC {a:.4f}")
+            res["bf16_vs_plain_speedup"] = res["bf16_mfma"]["words_per_s"] / res["fp32_plain"]["words_per_s"]
+            res["auc_gap_bf16_vs_plain"] = res["bf16_mfma"]["auc"] - res["fp32_plain"]["auc"]
+            out["sgns_forms"] = res
+            out["sgns_forms"]["corpus"] = f"{nw} walks, {nt} words (round 1 + {args.warmup} round(s))"
         cpu = None
         if not args.no_cpu:
-            threads = min(16, os.cpu_count() or 1)
-            rate, dt, cs = cpu_baseline(ctx, args.seed, args.cpu_sample, threads)
-            cpu = {"value": rate, "unit": "walk steps/s", "cores": threads, "kind": "port",
-                   "sample": f"{args.cpu_sample} walks of one round on the same graph "
-                             f"({cs} steps, {dt:.1f} s)"}
-    out = {
+            cpu = cpu_baseline(ctx, args.cpu_walks, args.cpu_sgns_walks, min(16, os.cpu_count() or 1),
+                               counts, args.seed)
+        copy_gbs = ctx.copy_bench(1 << 30, 3)
+        out["copy_kernel_gbs"] = copy_gbs
+
+    line = {
         "metric": "walk steps/s + SGNS words/s at equal AUC; BF16-MFMA vs FP32 plain, 1 GPU",
         "value": value,
-        "unit": "walk steps/s",
+        "unit": "words/s (walk + BF16-MFMA SGNS pipeline)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -138,18 +238,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": f"Chung-Lu V={V} E~{args.edges} gamma=2.2 seed=1, 90% train; "
-                               f"walk rounds of V walks, min_L 20, mu 0.999",
+        "config": {"workload": (f"Chung-Lu V={V} E~{args.edges} gamma=2.2 seed=1, 90% train; "
+                                f"per step one walk round of V walks (min_L 20, mu 0.999) + SGNS "
+                                f"BF16-MFMA dim {args.dim} w10 K5 sample 1e-3 on that round"),
                    "global_batch": V, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / MI355X_HBM_GBS, "traffic": None},
-        "kernels_ms": {"walk": walk_ms / max(n_walk, 1), "pack": pack_ms / max(n_walk, 1),
-                       "re": re_ms / max(n_walk, 1)},
+        "roofline": {"bound": "hbm", "kernel": "sg_mfma_kernel<bf16>",
+                     "achieved": bf_gbs, "peak": HBM_GBS, "unit": "GB/s",
+                     "frac": bf_gbs / HBM_GBS, "traffic": None},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    line.update(out)
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":
