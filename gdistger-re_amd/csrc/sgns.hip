@@ -13,11 +13,13 @@
 //     reference's block reduction order (pairs at distance 64, then a fold
 //     by 32, 16, ..., 1), so a single-wavefront run in sentence order gives
 //     the oracle's bits.
-//   * batched (DG_SGNS_BATCHED_*): a sentence is cut into tiles of kTile
-//     consecutive positions; all (context, target) pairs of a tile share K
-//     negatives (as __sgNegReuse shares them per sentence, word2vec.cu:231-303),
-//     the tile's output rows stay in registers, and each context row is read
-//     once per tile instead of once per pair.  All gradients of a launch come
+//   * batched (DG_SGNS_BATCHED_*): a sentence is cut into tiles of T
+//     consecutive positions (16, or 8 when K > 5).  Each position draws its
+//     own K negatives, shared by the contexts of its window (the per-position
+//     minibatch of DSGL, dsgl.hpp:1100-1159); the tile's rows are staged in
+//     LDS once and the (contexts x outputs) logits and both rank-1 update
+//     products run on MFMA, so a context row is read once per tile instead of
+//     once per (context, target) pair.  All gradients of a launch come
 //     from the tables as they were at its start ("snapshot" semantics) when
 //     deterministic; they are then summed per row by a sorted-key segmented
 //     scatter-add in a fixed order.  Non-deterministic mode adds them with
@@ -35,8 +37,8 @@ namespace {
 
 constexpr int kExpTable = 1000;
 constexpr int kMaxExp = 6;
-constexpr int kMaxNeg = 10;     // negatives per tile (tile positions + negatives <= 32)
-constexpr int kMaxWindow = 21;  // tile positions + 2 * window <= 64 staged context rows
+constexpr int kMaxNeg = 16;     // -negative
+constexpr int kMaxWindow = 24;  // tile positions + 2 * window <= 64 staged context rows
 constexpr uint32_t kNoRank = 0xffffffffu;
 
 struct SgnsDev {
@@ -185,22 +187,33 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
   }
 }
 
-// ---- batched form: shared-negative position tiles on MFMA --------------------
-// A tile is kTile consecutive positions of a sentence.  Its context rows X
-// (positions p0-W .. p1+W-1, up to CTX rows) and output rows Y (the kTile
-// targets followed by the K shared negatives, up to kOut rows) are staged in
-// LDS; then
-//   S = X Y^T                           (logits, MFMA)
-//   G = alpha * (label - sig(S)) * mask (window mask; negative multiplicity)
-//   dX = G Y,   dY = G^T X              (MFMA)
-// and the rows are updated by atomics (Hogwild) or staged for the sorted
-// scatter-add.  FP32 rows use v_mfma_f32_16x16x4f32; BF16 rows use
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (G is rounded to BF16 for
-// the two update products).
-constexpr int kTile = 22;
-constexpr int kOut = 32;
-
+// ---- batched form: position tiles on MFMA ----------------------------------
+// A tile is T consecutive positions p0 .. p1-1 of a sentence.  Its context
+// rows X (positions c_lo .. c_hi-1, c_lo = max(0, p0-W), c_hi = min(L, p1+W))
+// and output rows Y (per position j, the columns j(1+K) .. j(1+K)+K hold its
+// target and its K negatives) are staged in LDS, then
+//   S  = X Y^T,  Gx = X X^T                                (MFMA)
+//   per output column n (position p, label l, weight w), over the contexts
+//   c_1 < c_2 < ... of the shrunk window of p:
+//     s_k = S[c_k][n] + sum_{j<k} g_j Gx[c_k][c_j],   g_k = alpha w (l - sig(s_k))
+//   dX = G Y,  dY = G^T X                                  (MFMA)
+// s_k equals x_{c_k} . (y_n + sum_{j<k} g_j x_{c_j}): the output row as it
+// stands after the updates of the earlier contexts of the same position,
+// which is the order of skip_gram_kernel (word2vec.cu:360-396), where every
+// (context, target) pair updates the target row before the next context is
+// read.  The Gram matrix turns that dependency chain into a scalar
+// recursion, so the row products stay on MFMA.  dX is taken with the rows of
+// Y as staged (the context side of the reference update differs from this
+// by terms of order alpha^2).
+// Rows live in HBM as FP32 only.  The FP32 form stages them as they are and
+// runs v_mfma_f32_16x16x4f32; the BF16 form rounds them to BF16 while staging
+// and runs v_mfma_f32_16x16x32_bf16 (logits, Gram) and 16x16x16_bf16 (the
+// update products, G rounded to BF16), both with FP32 accumulation, adding
+// its FP32 results to the FP32 rows.  Every staged row is also updated by the tile that staged it, so a
+// BF16 copy in HBM would need a refresh (4 B read + 2 B written per element)
+// for each 2 B it saves on the gather, and would be stale in between.
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -213,30 +226,51 @@ struct Mfma;
 
 template <>
 struct Mfma<float> {
+  // logits / Gram products (k = dim) and update products (k = rows) both on
+  // v_mfma_f32_16x16x4f32
   static constexpr int KS = 4;
+  static constexpr int KU = 4;
   typedef float frag_t;
+  typedef float ufrag_t;
   // M[r0 + lane%16][k0 + lane/16]
   __device__ static frag_t row_frag(const float* M, int ld, int r0, int k0, int lane) {
     return M[(r0 + (lane & 15)) * ld + k0 + (lane >> 4)];
   }
-  // M[k0 + lane/16][c0 + lane%16]
-  __device__ static frag_t col_frag(const float* M, int ld, int k0, int c0, int lane) {
-    return M[(k0 + (lane >> 4)) * ld + c0 + (lane & 15)];
-  }
   __device__ static f32x4_t mma(frag_t a, frag_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  __device__ static float store(float g) { return g; }
+  // update products: A from the FP32 gradient matrix, B from staged rows
+  __device__ static ufrag_t urow_f(const float* M, int ld, int r0, int k0, int lane) {
+    return M[(r0 + (lane & 15)) * ld + k0 + (lane >> 4)];
+  }
+  __device__ static ufrag_t ucol_f(const float* M, int ld, int k0, int c0, int lane) {
+    return M[(k0 + (lane >> 4)) * ld + c0 + (lane & 15)];
+  }
+  __device__ static ufrag_t ucol(const float* M, int ld, int k0, int c0, int lane) {
+    return M[(k0 + (lane >> 4)) * ld + c0 + (lane & 15)];
+  }
+  __device__ static f32x4_t mmau(ufrag_t a, ufrag_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
 };
 
 template <>
 struct Mfma<uint16_t> {
+  // logits / Gram products on v_mfma_f32_16x16x32_bf16 (k = dim); update
+  // products on v_mfma_f32_16x16x16_bf16, so row counts pad to 16 only
   static constexpr int KS = 32;
+  static constexpr int KU = 16;
   typedef bf16x8_t frag_t;
+  typedef bf16x4_t ufrag_t;
   union U {
     uint4 q;
     uint16_t h[8];
     bf16x8_t v;
+  };
+  union U4 {
+    uint2 q;
+    uint16_t h[4];
+    bf16x4_t v;
   };
   // M[r0 + lane%16][k0 + 8 (lane/16) .. +7]
   __device__ static frag_t row_frag(const uint16_t* M, int ld, int r0, int k0, int lane) {
@@ -244,36 +278,63 @@ struct Mfma<uint16_t> {
     u.q = *reinterpret_cast<const uint4*>(M + (r0 + (lane & 15)) * ld + k0 + 8 * (lane >> 4));
     return u.v;
   }
-  // M[k0 + 8 (lane/16) + i][c0 + lane%16], i = 0..7
-  __device__ static frag_t col_frag(const uint16_t* M, int ld, int k0, int c0, int lane) {
-    U u;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) u.h[i] = M[(k0 + 8 * (lane >> 4) + i) * ld + c0 + (lane & 15)];
-    return u.v;
-  }
   __device__ static f32x4_t mma(frag_t a, frag_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
-  __device__ static uint16_t store(float g) { return f2bf(g); }
+  // FP32 M[r0 + lane%16][k0 + 4 (lane/16) .. +3] rounded to BF16 (16-byte aligned rows)
+  __device__ static ufrag_t urow_f(const float* M, int ld, int r0, int k0, int lane) {
+    const float4 a = *reinterpret_cast<const float4*>(M + (r0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4));
+    U4 u;
+    u.h[0] = f2bf(a.x);
+    u.h[1] = f2bf(a.y);
+    u.h[2] = f2bf(a.z);
+    u.h[3] = f2bf(a.w);
+    return u.v;
+  }
+  // FP32 M[k0 + 4 (lane/16) + i][c0 + lane%16], i = 0..3, rounded to BF16
+  __device__ static ufrag_t ucol_f(const float* M, int ld, int k0, int c0, int lane) {
+    U4 u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u.h[i] = f2bf(M[(k0 + 4 * (lane >> 4) + i) * ld + c0 + (lane & 15)]);
+    return u.v;
+  }
+  // BF16 M[k0 + 4 (lane/16) + i][c0 + lane%16], i = 0..3
+  __device__ static ufrag_t ucol(const uint16_t* M, int ld, int k0, int c0, int lane) {
+    U4 u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u.h[i] = M[(k0 + 4 * (lane >> 4) + i) * ld + c0 + (lane & 15)];
+    return u.v;
+  }
+  __device__ static f32x4_t mmau(ufrag_t a, ufrag_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
 };
 
-struct TileArgs {
-  SgnsDev a;
-  const uint16_t* bf0;  // BF16 copies (BF16 form)
-  const uint16_t* bf1;
-  uint16_t* wbf0;
-  uint16_t* wbf1;
+// Tile descriptor, built by tile_desc_kernel for every tile of a launch:
+//   [0] p0, [1] c_lo, [2] nc (context rows), [3] np (positions),
+//   [4 .. 4+T)            window shrink b of each position,
+//   [4+T .. 4+T+CTX)      context rows (vocabulary ranks, -1 past nc),
+//   [4+T+CTX .. +NOUT)    output rows per column (-1 past np (1+K)).
+constexpr int kDescHead = 4;
+
+struct DescArgs {
+  const int32_t* sen;
+  const uint64_t* sen_off;
+  const int32_t* table;
   const uint2* tiles;  // {sentence index, tile index within the sentence}
   uint64_t n_tiles;
   uint64_t first_sentence;
   uint64_t seed;
   uint32_t epoch;
-  int exact;
-  int deterministic;
-  float* stage;     // deterministic: per tile (CTX + kOut) slots of dim floats
-  uint32_t* keys;   // key = row | table << row_bits; 0xffffffff = unused slot
-  uint32_t* vals;
-  uint32_t row_bits;
+  uint32_t table_size;
+  uint32_t vocab;
+  int window;
+  int kd;  // negatives drawn per position
+  int T;
+  int ctx;
+  int nout;
+  uint32_t stride;  // ints per descriptor
+  int32_t* desc;
 };
 
 // word j of a Philox stream (block j/4, word j%4)
@@ -284,176 +345,252 @@ __device__ __forceinline__ uint32_t stream_word(uint64_t seed, uint32_t c1, uint
   return r == 0 ? o.x0 : (r == 1 ? o.x1 : (r == 2 ? o.x2 : o.x3));
 }
 
-template <typename E, int DIM, int CTX>
-__global__ __launch_bounds__(64) void sg_mfma_kernel(TileArgs t) {
-  typedef Mfma<E> M;
-  constexpr int PAD = 16 / (int)sizeof(E);
-  constexpr int LD = DIM + PAD;
-  constexpr int GLD = kOut + PAD;
-  constexpr int CPR = DIM * (int)sizeof(E) / 16;  // 16-byte chunks per row
-  __shared__ __attribute__((aligned(16))) E sX[CTX * LD];
-  __shared__ __attribute__((aligned(16))) E sY[kOut * LD];
-  __shared__ __attribute__((aligned(16))) E sG[CTX * GLD];
-  __shared__ int s_b[kTile];
-  __shared__ int s_ctx[CTX];
-  __shared__ int s_out[kOut];
-  const int lane = threadIdx.x;
-  const SgnsDev& a = t.a;
-  const int W = (int)a.window, K = (int)a.negative;
-  const E* src0 = sizeof(E) == 4 ? (const E*)a.syn0 : (const E*)t.bf0;
-  const E* src1 = sizeof(E) == 4 ? (const E*)a.syn1 : (const E*)t.bf1;
-  for (uint64_t ti = blockIdx.x; ti < t.n_tiles; ti += gridDim.x) {
-    const uint2 tl = t.tiles[ti];
-    const uint64_t s = tl.x;
-    const uint64_t sb = a.sen_off[s];
-    const int L = (int)(a.sen_off[s + 1] - sb);
-    const int p0 = (int)tl.y * kTile;
-    const int p1 = p0 + kTile < L ? p0 + kTile : L;
+// One wavefront per tile.  Draws: word q < np of the tile's stream gives
+// b = word % window of position p0+q; word np + q K + k gives negative k of
+// position q: table[word % table_size], 0 -> word % (vocab-1) + 1.
+__global__ __launch_bounds__(256) void tile_desc_kernel(DescArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const int K = a.kd, W = a.window;
+  for (uint64_t ti = wave; ti < a.n_tiles; ti += nwaves) {
+    const uint2 tl = a.tiles[ti];
+    const uint64_t sb = a.sen_off[tl.x];
+    const int L = (int)(a.sen_off[tl.x + 1] - sb);
+    const int p0 = (int)tl.y * a.T;
+    const int p1 = p0 + a.T < L ? p0 + a.T : L;
     const int np = p1 - p0;
-    const int32_t* tok = a.sen + sb;
     const int c_lo = p0 - W > 0 ? p0 - W : 0;
     const int c_hi = p1 + W < L ? p1 + W : L;
     const int nc = c_hi - c_lo;
-    const uint64_t gs = t.first_sentence + s;
-    // draws: b_p (words 0..np-1), then K negative words
-    if (lane < np + K) {
-      const uint32_t w = stream_word(t.seed, (uint32_t)gs, tl.y | (t.epoch << 16), DG_TAG_SGNS,
-                                     (uint32_t)lane);
-      if (lane < np) {
-        s_b[lane] = (int)(w % a.window);
-        s_out[lane] = tok[p0 + lane];
+    const uint32_t c1 = (uint32_t)(a.first_sentence + tl.x), c2 = tl.y | (a.epoch << 16);
+    int32_t* d = a.desc + ti * a.stride;
+    const int n_ent = kDescHead + a.T + a.ctx + a.nout;
+    for (int e = lane; e < n_ent; e += 64) {
+      int v;
+      if (e < kDescHead) {
+        v = e == 0 ? p0 : (e == 1 ? c_lo : (e == 2 ? nc : np));
+      } else if (e < kDescHead + a.T) {
+        const int q = e - kDescHead;
+        v = q < np ? (int)(stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)q) % (uint32_t)W) : 0;
+      } else if (e < kDescHead + a.T + a.ctx) {
+        const int m = e - kDescHead - a.T;
+        v = m < nc ? a.sen[sb + c_lo + m] : -1;
       } else {
-        int n = a.table[w % a.table_size];
-        if (n == 0) n = (int)(w % (a.vocab - 1) + 1);
-        s_out[lane] = n;
+        const int o = e - kDescHead - a.T - a.ctx;
+        const int j = o / (1 + K), r = o % (1 + K);
+        if (j >= np) {
+          v = -1;
+        } else if (r == 0) {
+          v = a.sen[sb + p0 + j];
+        } else {
+          const uint32_t w = stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)(np + j * K + r - 1));
+          v = a.table[w % a.table_size];
+          if (v == 0) v = (int)(w % (a.vocab - 1) + 1);
+        }
       }
-    } else if (lane < kOut) {
-      s_out[lane] = -1;
+      d[e] = v;
     }
-    if (lane < CTX) s_ctx[lane] = lane < nc ? tok[c_lo + lane] : -1;
+  }
+}
+
+struct TileArgs {
+  SgnsDev a;
+  const int32_t* desc;
+  uint32_t desc_stride;
+  uint64_t n_tiles;
+  int kd;      // negatives drawn per position
+  float wneg;  // negative / kd: weight of one drawn negative
+  int exact;
+  int deterministic;
+  float* stage;     // deterministic: per tile (CTX + NOUT) slots of dim floats
+  uint32_t* keys;   // key = row | table << row_bits; 0xffffffff = unused slot
+  uint32_t* vals;
+  uint32_t row_bits;
+};
+
+template <typename E, int DIM, int T, int CTX, int NOUT>
+__global__ __launch_bounds__(256) void sg_tile_kernel(TileArgs t) {
+  typedef Mfma<E> M;
+  constexpr int PAD = 16 / (int)sizeof(E);
+  constexpr int LD = DIM + PAD;   // staged rows (elements)
+  constexpr int SLD = NOUT + 4;   // S / G rows (floats, 16-byte aligned)
+  constexpr int XLD = CTX + 1;    // Gx rows (floats)
+  constexpr int C4 = DIM / 4;     // four-element chunks per row
+  constexpr int NCHUNK = (CTX + NOUT) * C4;
+  constexpr int NLD = (NCHUNK + 255) / 256;  // chunks per thread
+  constexpr int BATCH = NLD < 12 ? NLD : 12; // loads in flight per thread
+  constexpr int NDESC = kDescHead + T + CTX + NOUT;
+  static_assert(CTX % 16 == 0 && NOUT % 16 == 0 && NDESC <= 256, "tile shape");
+  __shared__ __attribute__((aligned(16))) E sX[CTX * LD];
+  __shared__ __attribute__((aligned(16))) E sY[NOUT * LD];
+  __shared__ __attribute__((aligned(16))) float sS[CTX * SLD];
+  __shared__ float sGx[CTX * XLD];
+  __shared__ float s_expt[kExpTable + 1];
+  __shared__ int s_d[NDESC];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const SgnsDev& a = t.a;
+  const int W = (int)a.window, K1 = t.kd + 1;
+  const int* s_b = s_d + kDescHead;
+  const int* s_ctx = s_d + kDescHead + T;
+  const int* s_out = s_d + kDescHead + T + CTX;
+  constexpr int NT_OUT = NOUT / 16;
+  constexpr int NT_DIM = DIM / 16;
+  for (int i = tid; i <= kExpTable; i += 256) s_expt[i] = a.expt[i];
+  // descriptor of the first tile; later ones are loaded one tile ahead
+  int dnext = 0;
+  if (blockIdx.x < t.n_tiles && tid < NDESC) dnext = t.desc[(uint64_t)blockIdx.x * t.desc_stride + tid];
+  for (uint64_t ti = blockIdx.x; ti < t.n_tiles; ti += gridDim.x) {
+    if (tid < NDESC) s_d[tid] = dnext;
     __syncthreads();
-    // stage rows (16-byte chunks)
-    for (int i = lane; i < CTX * CPR; i += 64) {
-      const int r = i / CPR, c = i % CPR;
-      const int row = s_ctx[r];
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (row >= 0) v = reinterpret_cast<const uint4*>(src0 + (uint64_t)row * DIM)[c];
-      *reinterpret_cast<uint4*>(sX + r * LD + c * (16 / (int)sizeof(E))) = v;
+    const int p0 = s_d[0], c_lo = s_d[1], nc = s_d[2], np = s_d[3];
+    {
+      const uint64_t tn = ti + gridDim.x;
+      if (tn < t.n_tiles && tid < NDESC) dnext = t.desc[tn * t.desc_stride + tid];
     }
-    for (int i = lane; i < kOut * CPR; i += 64) {
-      const int r = i / CPR, c = i % CPR;
-      const int row = s_out[r];
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (row >= 0) v = reinterpret_cast<const uint4*>(src1 + (uint64_t)row * DIM)[c];
-      *reinterpret_cast<uint4*>(sY + r * LD + c * (16 / (int)sizeof(E))) = v;
+    // stage the rows: four FP32 elements per load, BATCH loads in flight, each
+    // rounded to E as stored
+#pragma unroll
+    for (int b0 = 0; b0 < NLD; b0 += BATCH) {
+      float4 v[BATCH];
+#pragma unroll
+      for (int q = 0; q < BATCH; ++q) {
+        const int i = (b0 + q) * 256 + tid;
+        v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (b0 + q < NLD && i < NCHUNK) {
+          const int r = i / C4, c = i % C4;
+          const int row = r < CTX ? s_ctx[r] : s_out[r - CTX];
+          if (row >= 0)
+            v[q] = reinterpret_cast<const float4*>((r < CTX ? a.syn0 : a.syn1) + (uint64_t)row * DIM)[c];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < BATCH; ++q) {
+        const int i = (b0 + q) * 256 + tid;
+        if (b0 + q < NLD && i < NCHUNK) {
+          const int r = i / C4, c = i % C4;
+          E* dst = (r < CTX ? sX + r * LD : sY + (r - CTX) * LD) + 4 * c;
+          if constexpr (sizeof(E) == 4) {
+            *reinterpret_cast<float4*>(dst) = v[q];
+          } else {
+            uint2 h;
+            h.x = (uint32_t)f2bf(v[q].x) | ((uint32_t)f2bf(v[q].y) << 16);
+            h.y = (uint32_t)f2bf(v[q].z) | ((uint32_t)f2bf(v[q].w) << 16);
+            *reinterpret_cast<uint2*>(dst) = h;
+          }
+        }
+      }
     }
     __syncthreads();
     const int nM = (nc + 15) / 16;
-    const int kc = ((nM * 16 + M::KS - 1) / M::KS) * M::KS;  // context extent of dY's k loop
-    // S and G
-    for (int mt = 0; mt < nM; ++mt) {
-      for (int nt = 0; nt < 2; ++nt) {
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const int kc = nM * 16;  // <= CTX, a multiple of the update k step
+    // S = X Y^T (nM x NT_OUT tiles) and Gx = X X^T (nM x nM tiles)
+    for (int tt = wave; tt < nM * NT_OUT + nM * nM; tt += 4) {
+      const bool is_s = tt < nM * NT_OUT;
+      const int u = is_s ? tt : tt - nM * NT_OUT;
+      const int mt = is_s ? u / NT_OUT : u / nM;
+      const int nt = is_s ? u % NT_OUT : u % nM;
+      const E* B = is_s ? sY : sX;
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
-        for (int k0 = 0; k0 < DIM; k0 += M::KS)
-          acc = M::mma(M::row_frag(sX, LD, mt * 16, k0, lane), M::row_frag(sY, LD, nt * 16, k0, lane), acc);
+      for (int k0 = 0; k0 < DIM; k0 += M::KS)
+        acc = M::mma(M::row_frag(sX, LD, mt * 16, k0, lane), M::row_frag(B, LD, nt * 16, k0, lane), acc);
+      float* dst = is_s ? sS : sGx;
+      const int ld = is_s ? SLD : XLD;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int m = mt * 16 + 4 * (lane >> 4) + i;
-          const int n = nt * 16 + (lane & 15);
-          float g = 0.0f;
-          if (m < nc && n < np + K) {
-            const int c = c_lo + m;
-            if (n < np) {
-              const int p = p0 + n, w = W - s_b[n];
-              if (c != p && c >= p - w && c <= p + w)
-                g = (1.0f - sigmoid_of(acc[i], a.expt, t.exact)) * a.alpha;
-            } else {
-              int mult = 0;
-              for (int j = 0; j < np; ++j) {
-                const int p = p0 + j, w = W - s_b[j];
-                mult += (c != p && c >= p - w && c <= p + w && s_out[n] != s_out[j]) ? 1 : 0;
-              }
-              if (mult) g = (0.0f - sigmoid_of(acc[i], a.expt, t.exact)) * a.alpha * (float)mult;
-            }
-          }
-          sG[m * GLD + n] = M::store(g);
-        }
-      }
+      for (int i = 0; i < 4; ++i)
+        dst[(mt * 16 + 4 * (lane >> 4) + i) * ld + nt * 16 + (lane & 15)] = acc[i];
     }
-    for (int i = lane; i < (kc - nM * 16) * kOut; i += 64)  // rows between nM*16 and kc
-      sG[(nM * 16 + i / kOut) * GLD + i % kOut] = M::store(0.0f);
     __syncthreads();
-    const uint64_t slot0 = ti * (uint64_t)(CTX + kOut);
-    const uint32_t tb = 1u << t.row_bits;
-    // dX = G Y  -> input rows
-    for (int mt = 0; mt < nM; ++mt) {
-      for (int nt = 0; nt < DIM / 16; ++nt) {
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int k0 = 0; k0 < kOut; k0 += M::KS)
-          acc = M::mma(M::row_frag(sG, GLD, mt * 16, k0, lane), M::col_frag(sY, LD, k0, nt * 16, lane), acc);
-        const int d = nt * 16 + (lane & 15);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int m = mt * 16 + 4 * (lane >> 4) + i;
-          if (m >= nc) continue;
-          const uint64_t row = (uint64_t)s_ctx[m];
-          if (t.deterministic) {
-            t.stage[(slot0 + m) * DIM + d] = acc[i];
-          } else if (sizeof(E) == 4) {
-            atomicAdd(&a.syn0[row * DIM + d], acc[i]);
-          } else {
-            const float old = atomicAdd(&a.syn0[row * DIM + d], acc[i]);
-            t.wbf0[row * DIM + d] = f2bf(old + acc[i]);
+    // the per-column recursion; columns spread over the four waves
+    {
+      const int n = (tid & 63) * 4 + wave;
+      if (n < NOUT) {
+        const int j = n / K1, r = n % K1;
+        const int row = s_out[n];
+        const bool live = j < np && row >= 0 && (r == 0 || row != s_out[j * K1]);
+        int m_lo = 0, m_hi = -1, mc = -1;
+        if (live) {
+          const int p = p0 + j, w = W - s_b[j];
+          m_lo = (p - w > c_lo ? p - w : c_lo) - c_lo;
+          m_hi = (p + w < c_lo + nc - 1 ? p + w : c_lo + nc - 1) - c_lo;
+          mc = p - c_lo;
+        }
+        const float lab = r == 0 ? 1.0f : 0.0f;
+        for (int m = 0; m < kc; ++m) {
+          float g = 0.0f;
+          if (m >= m_lo && m <= m_hi && m != mc) {
+            float sv = sS[m * SLD + n];
+            for (int m2 = m_lo; m2 < m; ++m2)
+              if (m2 != mc) sv += sS[m2 * SLD + n] * sGx[m * XLD + m2];
+            g = (lab - sigmoid_of(sv, s_expt, t.exact)) * a.alpha;
+            if (r != 0) g *= t.wneg;
           }
+          sS[m * SLD + n] = g;
         }
       }
     }
-    // dY = G^T X  -> output rows
-    for (int mt = 0; mt < 2; ++mt) {
-      for (int nt = 0; nt < DIM / 16; ++nt) {
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < kc; k0 += M::KS)
-          acc = M::mma(M::col_frag(sG, GLD, k0, mt * 16, lane), M::col_frag(sX, LD, k0, nt * 16, lane), acc);
-        const int d = nt * 16 + (lane & 15);
+    __syncthreads();
+    const uint64_t slot0 = ti * (uint64_t)(CTX + NOUT);
+    const uint32_t tb = 1u << t.row_bits;
+    // dX = G Y -> input rows
+    for (int tt = wave; tt < nM * NT_DIM; tt += 4) {
+      const int mt = tt / NT_DIM, nt = tt % NT_DIM;
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int m = mt * 16 + 4 * (lane >> 4) + i;
-          if (m >= np + K) continue;
-          const uint64_t row = (uint64_t)s_out[m];
-          if (t.deterministic) {
-            t.stage[(slot0 + CTX + m) * DIM + d] = acc[i];
-          } else if (sizeof(E) == 4) {
-            atomicAdd(&a.syn1[row * DIM + d], acc[i]);
-          } else {
-            const float old = atomicAdd(&a.syn1[row * DIM + d], acc[i]);
-            t.wbf1[row * DIM + d] = f2bf(old + acc[i]);
-          }
-        }
+      for (int k0 = 0; k0 < NOUT; k0 += M::KU)
+        acc = M::mmau(M::urow_f(sS, SLD, mt * 16, k0, lane), M::ucol(sY, LD, k0, nt * 16, lane), acc);
+      const int dd = nt * 16 + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = mt * 16 + 4 * (lane >> 4) + i;
+        if (m >= nc) continue;
+        if (t.deterministic)
+          t.stage[(slot0 + m) * DIM + dd] = acc[i];
+        else
+          atomicAdd(&a.syn0[(uint64_t)s_ctx[m] * DIM + dd], acc[i]);
+      }
+    }
+    // dY = G^T X -> output rows
+    for (int tt = wave; tt < NT_OUT * NT_DIM; tt += 4) {
+      const int mt = tt / NT_DIM, nt = tt % NT_DIM;
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      for (int k0 = 0; k0 < kc; k0 += M::KU)
+        acc = M::mmau(M::ucol_f(sS, SLD, k0, mt * 16, lane), M::ucol(sX, LD, k0, nt * 16, lane), acc);
+      const int dd = nt * 16 + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = mt * 16 + 4 * (lane >> 4) + i;
+        const int rr = s_out[m];
+        if (rr < 0) continue;
+        if (t.deterministic)
+          t.stage[(slot0 + CTX + m) * DIM + dd] = acc[i];
+        else
+          atomicAdd(&a.syn1[(uint64_t)rr * DIM + dd], acc[i]);
       }
     }
     if (t.deterministic) {
-      const uint64_t k0 = slot0;
-      if (lane < CTX) {
-        t.keys[k0 + lane] = lane < nc ? (uint32_t)s_ctx[lane] : 0xffffffffu;
-        t.vals[k0 + lane] = (uint32_t)(k0 + lane);
+      if (tid < CTX) {
+        t.keys[slot0 + tid] = tid < nc ? (uint32_t)s_ctx[tid] : 0xffffffffu;
+        t.vals[slot0 + tid] = (uint32_t)(slot0 + tid);
       }
-      if (lane < kOut) {
-        t.keys[k0 + CTX + lane] = lane < np + K ? ((uint32_t)s_out[lane] | tb) : 0xffffffffu;
-        t.vals[k0 + CTX + lane] = (uint32_t)(k0 + CTX + lane);
+      if (tid < NOUT) {
+        t.keys[slot0 + CTX + tid] = s_out[tid] >= 0 ? ((uint32_t)s_out[tid] | tb) : 0xffffffffu;
+        t.vals[slot0 + CTX + tid] = (uint32_t)(slot0 + CTX + tid);
       }
     }
     __syncthreads();
   }
 }
 
-// tiles per sentence: ceil(L / kTile) for L >= 2
-__global__ void tile_count_kernel(const uint64_t* sen_off, uint64_t n_sen, uint32_t* cnt) {
+// tiles per sentence: ceil(L / T) for L >= 2
+__global__ void tile_count_kernel(const uint64_t* sen_off, uint64_t n_sen, uint32_t tile,
+                                  uint32_t* cnt) {
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_sen;
        s += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t L = sen_off[s + 1] - sen_off[s];
-    cnt[s] = L >= 2 ? (uint32_t)((L + kTile - 1) / kTile) : 0u;
+    cnt[s] = L >= 2 ? (uint32_t)((L + tile - 1) / tile) : 0u;
   }
 }
 
@@ -546,8 +683,7 @@ template <int D>
 __global__ __launch_bounds__(256) void seg_apply_kernel(const uint32_t* keys, const uint32_t* vals,
                                                         const uint64_t* starts, uint64_t n_seg,
                                                         uint64_t n_valid, const float* stage,
-                                                        uint32_t row_bits, float* syn0, float* syn1,
-                                                        uint16_t* bf0, uint16_t* bf1, int mode,
+                                                        uint32_t row_bits, float* syn0, float* syn1, int mode,
                                                         float lr, float* g0, float* g1) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -567,7 +703,6 @@ __global__ __launch_bounds__(256) void seg_apply_kernel(const uint32_t* keys, co
     const uint32_t tbl = key >> row_bits;
     const uint64_t row = key & ((1u << row_bits) - 1u);
     float* w = (tbl ? syn1 : syn0) + row * (64 * D);
-    uint16_t* wb = tbl ? bf1 : bf0;
     if (mode == 2) {
       float* g = (tbl ? g1 : g0) + row * (64 * D);
 #pragma unroll
@@ -578,11 +713,6 @@ __global__ __launch_bounds__(256) void seg_apply_kernel(const uint32_t* keys, co
     for (int q = 0; q < D; ++q) {
       const float nv = mode == 0 ? w[lane + 64 * q] + acc[q] : w[lane + 64 * q] - lr * acc[q];
       w[lane + 64 * q] = nv;
-      if (wb) {
-        const uint32_t u = __float_as_uint(nv);
-        const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-        wb[row * (64 * D) + lane + 64 * q] = (uint16_t)(r >> 16);
-      }
     }
   }
 }
@@ -599,14 +729,6 @@ __global__ void pair_score_kernel(const uint32_t* rank, const float* syn0, uint3
       for (uint32_t d = lane; d < dim; d += 64) s += syn0[(uint64_t)ru * dim + d] * syn0[(uint64_t)rv * dim + d];
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) out[i] = s;
-  }
-}
-
-__global__ void to_bf16_kernel(const float* src, uint16_t* dst, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t u = __float_as_uint(src[i]);
-    dst[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
   }
 }
 
@@ -632,8 +754,6 @@ struct dg_sgns_state {
   uint32_t* d_rank = nullptr;
   float* d_syn0 = nullptr;
   float* d_syn1 = nullptr;
-  uint16_t* d_bf0 = nullptr;
-  uint16_t* d_bf1 = nullptr;
   float* d_g0 = nullptr;  // gradient tables of the tuple interface
   float* d_g1 = nullptr;
   int32_t* d_table = nullptr;
@@ -649,6 +769,8 @@ struct dg_sgns_state {
   size_t cnt_cap = 0;
   uint2* d_tiles = nullptr;
   size_t tiles_cap = 0;
+  int32_t* d_desc = nullptr;  // tile descriptors of one launch
+  size_t desc_cap = 0;
   uint64_t* d_tile_off = nullptr;
   size_t tile_off_cap = 0;
   // sorted scatter-add
@@ -679,9 +801,9 @@ struct dg_sgns_state {
 void dg_sgns_free(dg_ctx* ctx) {
   dg_sgns_state* s = ctx->sgns;
   if (!s) return;
-  void* bufs[] = {s->d_rank,   s->d_syn0,  s->d_syn1,   s->d_bf0,     s->d_bf1,   s->d_g0,
+  void* bufs[] = {s->d_rank,   s->d_syn0,  s->d_syn1,   s->d_g0,
                   s->d_g1,     s->d_table, s->d_expt,   s->d_keep,    s->d_sen,   s->d_sen_off,
-                  s->d_cnt,    s->d_tiles, s->d_tile_off, s->d_stage, s->d_k[0],  s->d_k[1],
+                  s->d_cnt,    s->d_tiles, s->d_desc,  s->d_tile_off, s->d_stage, s->d_k[0],  s->d_k[1],
                   s->d_v[0],   s->d_v[1],  s->d_hist,   s->d_hoff,    s->d_head,  s->d_hpos,
                   s->d_starts, s->d_tuples, s->d_logits};
   for (void* b : bufs)
@@ -709,6 +831,8 @@ extern "C" void dg_sgns_params_default(dg_sgns_params* p) {
   p->minibatch = 0;
   p->table_size = 100000000u;
   p->init_float_ops = 0;
+  p->tile_positions = 0;
+  p->tile_negatives = 0;
   p->seed = 1;
 }
 
@@ -723,8 +847,23 @@ static int check_params(dg_ctx* ctx, const dg_sgns_params* p) {
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "negative %u: at most %d", p->negative, kMaxNeg);
   if (p->table_size == 0) return dg_fail(ctx, DG_ERR_ARG, "table_size must be positive");
   if (p->form > DG_SGNS_BATCHED_BF16) return dg_fail(ctx, DG_ERR_ARG, "unknown form %u", p->form);
-  if (p->form == DG_SGNS_BATCHED_FP32 && p->dim > 256)
-    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched FP32 form supports dim <= 256");
+  if (p->tile_positions != 0 && p->tile_positions != 8 && p->tile_positions != 16)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "tile_positions %u: 8 or 16", p->tile_positions);
+  if (p->tile_negatives != 0 && p->tile_negatives < p->negative)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "tile_negatives %u: at least negative (%u)",
+                   p->tile_negatives, p->negative);
+  if (p->form != DG_SGNS_PLAIN_FP32) {
+    const uint32_t kd = p->tile_negatives ? p->tile_negatives : p->negative;
+    const uint32_t T = p->tile_positions ? p->tile_positions : 8u;
+    if (T * (1 + kd) > 96u || T + 2 * p->window > 64u)
+      return dg_fail(ctx, DG_ERR_UNSUPPORTED,
+                     "tile of %u positions: needs %u output rows (max 96) and %u context rows (max 64)",
+                     T, T * (1 + kd), T + 2 * p->window);
+  }
+  if (p->form == DG_SGNS_BATCHED_FP32 && p->dim > 128)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched FP32 form supports dim <= 128");
+  if (p->form == DG_SGNS_BATCHED_BF16 && p->dim > 256)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched BF16 form supports dim <= 256");
   return DG_OK;
 }
 
@@ -837,13 +976,6 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
   DG_HIP(ctx, hipMemcpy(s->d_table, table.data(), sizeof(int32_t) * p->table_size, hipMemcpyHostToDevice));
   DG_HIP(ctx, hipMemcpy(s->d_expt, expt.data(), sizeof(float) * (kExpTable + 1), hipMemcpyHostToDevice));
   DG_HIP(ctx, hipMemcpy(s->d_keep, keep.data(), sizeof(uint32_t) * vocab, hipMemcpyHostToDevice));
-  if (p->form == DG_SGNS_BATCHED_BF16) {
-    DG_HIP(ctx, hipMalloc((void**)&s->d_bf0, sizeof(uint16_t) * nrow));
-    DG_HIP(ctx, hipMalloc((void**)&s->d_bf1, sizeof(uint16_t) * nrow));
-    to_bf16_kernel<<<grid_for(ctx, nrow, 256), 256, 0, ctx->stream>>>(s->d_syn0, s->d_bf0, nrow);
-    to_bf16_kernel<<<grid_for(ctx, nrow, 256), 256, 0, ctx->stream>>>(s->d_syn1, s->d_bf1, nrow);
-    DG_HIP(ctx, hipGetLastError());
-  }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DG_OK;
 }
@@ -916,23 +1048,23 @@ static int sorted_scatter_apply(dg_ctx* ctx, dg_sgns_state* s, uint64_t n, int m
   switch (D) {
     case 1:
       seg_apply_kernel<1><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, s->d_bf0,
-                                                      s->d_bf1, mode, lr, s->d_g0, s->d_g1);
+                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
+                                                      lr, s->d_g0, s->d_g1);
       break;
     case 2:
       seg_apply_kernel<2><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, s->d_bf0,
-                                                      s->d_bf1, mode, lr, s->d_g0, s->d_g1);
+                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
+                                                      lr, s->d_g0, s->d_g1);
       break;
     case 4:
       seg_apply_kernel<4><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, s->d_bf0,
-                                                      s->d_bf1, mode, lr, s->d_g0, s->d_g1);
+                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
+                                                      lr, s->d_g0, s->d_g1);
       break;
     case 8:
       seg_apply_kernel<8><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, s->d_bf0,
-                                                      s->d_bf1, mode, lr, s->d_g0, s->d_g1);
+                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
+                                                      lr, s->d_g0, s->d_g1);
       break;
   }
   DG_HIP(ctx, hipGetLastError());
@@ -950,28 +1082,67 @@ static void launch_plain(dg_ctx* ctx, const SgnsDev& a, uint64_t sb, uint64_t se
   }
 }
 
+// Tile shape: T positions per tile (tile_positions, else 8), K negatives
+// drawn per position; staged context rows CTX = T + 2 window rounded up to 32
+// or 64; output columns NOUT = T (1 + K) rounded up to 48 or 96.
+struct TileShape {
+  int T;
+  int ctx;
+  int nout;
+};
+
+static TileShape tile_shape(int form, uint32_t window, uint32_t kd, uint32_t tile) {
+  TileShape ts;
+  ts.T = tile ? (int)tile : 8;
+  ts.ctx = ts.T + 2 * (int)window <= 32 ? 32 : 64;
+  (void)form;
+  ts.nout = ts.T * (1 + (int)kd) <= 48 ? 48 : 96;
+  return ts;
+}
+
+static uint32_t desc_stride(const TileShape& ts) {
+  return (uint32_t)((kDescHead + ts.T + ts.ctx + ts.nout + 3) & ~3);
+}
+
+// Blocks resident per CU for a kernel (queried once), times the CUs.
+template <typename Kern>
+static unsigned resident_grid(dg_ctx* ctx, Kern k, uint64_t n_tiles) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
+  if (g > n_tiles) g = n_tiles;
+  return (unsigned)(g ? g : 1);
+}
+
+template <typename E, int DIM, int T, int CTX, int NOUT>
+static void launch_tile(dg_ctx* ctx, const TileArgs& t) {
+  auto k = sg_tile_kernel<E, DIM, T, CTX, NOUT>;
+  k<<<resident_grid(ctx, k, t.n_tiles), 256, 0, ctx->stream>>>(t);
+}
+
 template <typename E, int DIM>
-static void launch_mfma(dg_ctx* ctx, const TileArgs& t) {
-  uint64_t blocks = t.n_tiles;
-  const uint64_t cap = (uint64_t)ctx->num_cus * 8;
-  if (blocks > cap) blocks = cap;
-  // BF16 always stages 64 context rows: its k step of 32 rounds the context
-  // extent of dY = G^T X up to 64
-  if (sizeof(E) == 4 && kTile + 2 * (int)t.a.window <= 48)
-    sg_mfma_kernel<E, DIM, 48><<<(unsigned)(blocks ? blocks : 1), 64, 0, ctx->stream>>>(t);
-  else
-    sg_mfma_kernel<E, DIM, 64><<<(unsigned)(blocks ? blocks : 1), 64, 0, ctx->stream>>>(t);
+static void launch_tile_shape(dg_ctx* ctx, const TileArgs& t, const TileShape& ts) {
+  constexpr int S = 48;
+  if (ts.T == 16) {
+    launch_tile<E, DIM, 16, 64, 96>(ctx, t);
+  } else if (ts.ctx == 32) {
+    if (ts.nout == S) launch_tile<E, DIM, 8, 32, S>(ctx, t);
+    else launch_tile<E, DIM, 8, 32, 96>(ctx, t);
+  } else {
+    if (ts.nout == S) launch_tile<E, DIM, 8, 64, S>(ctx, t);
+    else launch_tile<E, DIM, 8, 64, 96>(ctx, t);
+  }
 }
 
 template <typename E>
-static void launch_mfma_dim(dg_ctx* ctx, const TileArgs& t, uint32_t dim) {
+static void launch_tile_dim(dg_ctx* ctx, const TileArgs& t, uint32_t dim, const TileShape& ts) {
   switch (dim) {
-    case 64: launch_mfma<E, 64>(ctx, t); break;
-    case 128: launch_mfma<E, 128>(ctx, t); break;
-    case 256: launch_mfma<E, 256>(ctx, t); break;
-    case 512:
-      // FP32 rows of dim 512 do not fit the 160 KiB LDS; rejected by check_params
-      if constexpr (sizeof(E) == 2) launch_mfma<E, 512>(ctx, t);
+    case 64: launch_tile_shape<E, 64>(ctx, t, ts); break;
+    case 128: launch_tile_shape<E, 128>(ctx, t, ts); break;
+    case 256:
+      // FP32 rows of dim 256 do not fit the 160 KiB LDS; rejected by check_params
+      if constexpr (sizeof(E) == 2) launch_tile_shape<E, 256>(ctx, t, ts);
       break;
   }
 }
@@ -983,8 +1154,6 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
   if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
   DG_CHECK(check_params(ctx, pin));
   if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
-  if (pin->form == DG_SGNS_BATCHED_BF16 && !s->d_bf0)
-    return dg_fail(ctx, DG_ERR_STATE, "BF16 form needs dg_sgns_init with the BF16 form");
   DG_HIP(ctx, hipSetDevice(ctx->device));
   const dg_sgns_params p = *pin;
   const uint32_t* d_tok = corpus == DG_CORPUS_TRAIN ? ctx->d_train_tokens : ctx->d_tokens;
@@ -1068,8 +1237,10 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
       }
     } else {
       // tile list
-      tile_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(s->d_sen_off, n_walks,
-                                                                              s->d_cnt);
+      const uint32_t kd = p.tile_negatives ? p.tile_negatives : p.negative;
+      const TileShape ts = tile_shape((int)p.form, p.window, kd, p.tile_positions);
+      tile_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
+          s->d_sen_off, n_walks, (uint32_t)ts.T, s->d_cnt);
       DG_CHECK(dg_reserve(ctx, &s->d_tile_off, &s->tile_off_cap, (size_t)n_walks + 1));
       DG_CHECK(dg_scan_lengths(ctx, s->d_cnt, n_walks, s->d_tile_off));
       std::vector<uint64_t> h_toff(n_walks + 1);
@@ -1081,23 +1252,37 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
                                                                              n_walks, s->d_tiles);
       DG_HIP(ctx, hipGetLastError());
       const uint64_t mb = p.minibatch ? p.minibatch : (p.deterministic ? 8192 : 262144);
+      const uint32_t stride = desc_stride(ts);
+      const uint64_t mb_cap = n_tiles < mb ? n_tiles : mb;
+      DG_CHECK(dg_reserve(ctx, &s->d_desc, &s->desc_cap, (size_t)(mb_cap ? mb_cap : 1) * stride));
+      DescArgs da;
+      da.sen = s->d_sen;
+      da.sen_off = s->d_sen_off;
+      da.table = s->d_table;
+      da.first_sentence = s->sentences_done;
+      da.seed = p.seed;
+      da.epoch = epoch;
+      da.table_size = s->table_size;
+      da.vocab = s->vocab;
+      da.window = (int)p.window;
+      da.kd = (int)kd;
+      da.T = ts.T;
+      da.ctx = ts.ctx;
+      da.nout = ts.nout;
+      da.stride = stride;
+      da.desc = s->d_desc;
       TileArgs t;
       t.a = a;
-      t.first_sentence = s->sentences_done;
-      t.seed = p.seed;
-      t.epoch = epoch;
+      t.desc = s->d_desc;
+      t.desc_stride = stride;
+      t.kd = (int)kd;
+      t.wneg = (float)p.negative / (float)kd;
       t.exact = (int)p.exact_sigmoid;
       t.deterministic = (int)p.deterministic;
       t.row_bits = (uint32_t)row_bits_for(s->vocab);
-      t.bf0 = s->d_bf0;
-      t.bf1 = s->d_bf1;
-      t.wbf0 = s->d_bf0;
-      t.wbf1 = s->d_bf1;
-      const int ctx_rows =
-          (p.form != DG_SGNS_BATCHED_BF16 && kTile + 2 * (int)p.window <= 48) ? 48 : 64;
-      const uint64_t slots_per_tile = (uint64_t)(ctx_rows + kOut);
+      const uint64_t slots_per_tile = (uint64_t)(ts.ctx + ts.nout);
       if (p.deterministic) {
-        const size_t slots = (size_t)mb * slots_per_tile;
+        const size_t slots = (size_t)mb_cap * slots_per_tile;
         DG_CHECK(dg_reserve(ctx, &s->d_stage, &s->stage_cap, slots * s->dim));
         if (s->kv_cap < slots) {
           for (int i = 0; i < 2; ++i) {
@@ -1122,7 +1307,8 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
       }
       for (uint64_t t0 = 0; t0 < n_tiles; t0 += mb) {
         const uint64_t nt = n_tiles - t0 < mb ? n_tiles - t0 : mb;
-        t.tiles = s->d_tiles + t0;
+        da.tiles = s->d_tiles + t0;
+        da.n_tiles = nt;
         t.n_tiles = nt;
         // learning rate from the first sentence of the launch
         uint64_t lo = 0, hi = n_walks;
@@ -1133,10 +1319,16 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
         t.a.alpha = alpha = alpha_at(lo);
         size_t tk;
         DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_tile", &tk));
+        {
+          uint64_t g = (nt + 3) / 4;
+          const uint64_t cap = (uint64_t)ctx->num_cus * 16;
+          if (g > cap) g = cap;
+          tile_desc_kernel<<<(unsigned)(g ? g : 1), 256, 0, ctx->stream>>>(da);
+        }
         if (p.form == DG_SGNS_BATCHED_BF16)
-          launch_mfma_dim<uint16_t>(ctx, t, s->dim);
+          launch_tile_dim<uint16_t>(ctx, t, s->dim, ts);
         else
-          launch_mfma_dim<float>(ctx, t, s->dim);
+          launch_tile_dim<float>(ctx, t, s->dim, ts);
         DG_HIP(ctx, hipGetLastError());
         if (p.deterministic) DG_CHECK(sorted_scatter_apply(ctx, s, nt * slots_per_tile, 0, 0.0f));
         DG_CHECK(dg_time_end(ctx, tk));
@@ -1189,11 +1381,6 @@ extern "C" int dg_sgns_set_tables(dg_ctx* ctx, const float* syn0, const float* s
   const size_t n = (size_t)s->vocab * s->dim;
   if (syn0) DG_HIP(ctx, hipMemcpy(s->d_syn0, syn0, sizeof(float) * n, hipMemcpyHostToDevice));
   if (syn1) DG_HIP(ctx, hipMemcpy(s->d_syn1, syn1, sizeof(float) * n, hipMemcpyHostToDevice));
-  if (s->d_bf0) {
-    to_bf16_kernel<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(s->d_syn0, s->d_bf0, n);
-    to_bf16_kernel<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(s->d_syn1, s->d_bf1, n);
-    DG_HIP(ctx, hipGetLastError());
-  }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DG_OK;
 }

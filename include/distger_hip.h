@@ -205,6 +205,11 @@ typedef struct {
   uint32_t minibatch;      /* batched: position tiles per launch (0 = tuned default) */
   uint32_t table_size;     /* unigram table entries (1e8 in the reference) */
   uint32_t init_float_ops; /* 0: word2vec.cu init rounding (double), 1: dsgl.hpp (float) */
+  uint32_t tile_positions; /* batched: consecutive positions per tile, 4, 8 or 16 with
+                              tile_positions * (1 + tile_negatives) <= 96 (0 = tuned default) */
+  uint32_t tile_negatives; /* batched: negatives drawn per position, >= negative (0 = negative);
+                              each one's gradient is scaled by negative / tile_negatives, so
+                              the expected update is that of `negative` draws per pair */
   uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
 } dg_sgns_params;
 

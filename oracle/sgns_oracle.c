@@ -241,25 +241,33 @@ uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* ba
 }
 
 /* ------------------------------------------------------------------------
- * Batched form: tiles of DGO_TILE consecutive positions of a sentence share
- * K negatives (the sharing of __sgNegReuse, src/word2vec.cu:231-303, applied
- * per tile), and every gradient of a launch of `minibatch` tiles is taken
- * from the tables as they were at its start; the row sums are then added
- * once.  Arithmetic in double, rows stored as float (the FP64 reference the
- * FP32 and BF16 device forms are measured against).
+ * Batched form: tiles of T consecutive positions of a sentence; each
+ * position draws its own K negatives, shared by the contexts of its window
+ * (the per-position minibatch of dsgl.hpp:1100-1159).  Within a position the
+ * contexts are taken in order and each (context, output) pair sees the
+ * output row as updated by the earlier contexts of that position, as
+ * skip_gram_kernel updates it (word2vec.cu:360-396); everything else of a
+ * launch of `minibatch` tiles is taken from the tables as they were at its
+ * start, and the row sums are added once.  Arithmetic in double, rows stored
+ * as float: the FP64 reference the FP32 and BF16 device forms are measured
+ * against.
  *
- * Tile (s, j): positions p0 = 22 j .. p1 = min(p0 + 22, L) - 1 of sentence s
- * (L >= 2).  Philox stream (ctr = (blk, gs, j | epoch << 16, SGNS)), gs =
- * first_sen + s: b_p = next % window for p = p0..p1-1, then K words x_k,
- * negative n_k = table[x_k % table_size] (0 -> x_k % (vocab-1) + 1).
- * Contexts c in [max(0, p0-W), min(L, p1+W)).
- *   gp[c][p] = (1 - sig(x_c . y_{t_p})) alpha          for c in win(p)
- *   gn[c][k] = (0 - sig(x_c . y_{n_k})) alpha m[c][k],  m = #{p : c in win(p), n_k != t_p}
- *   dX[c] = sum_p gp y_{t_p} + sum_k gn y_{n_k};  dYt[p] = sum_c gp x_c;  dYn[k] = sum_c gn x_c
- * win(p) = {c != p : |c - p| <= W - b_p}.
+ * K = `draws` negatives per position (`negative` when 0), each weighted by
+ * wneg = negative / K (a float; 1 when K = negative).  T = `tile` when
+ * non-zero, else 8.
+ * Tile (s, j): positions p0 = T j .. p1 - 1 = min(p0 + T, L) - 1 of sentence
+ * s (L >= 2).  Philox stream (ctr = (blk, gs, j | epoch << 16, SGNS)), gs =
+ * first_sen + s: word q < np gives b_{p0+q} = word % window; word np + q K + k
+ * gives position q's negative k: table[word % table_size] (0 -> word %
+ * (vocab-1) + 1).  Contexts c in [max(0, p0-W), min(L, p1+W)).  Output
+ * column n = q (K+1) + r of position p = p0 + q: r = 0 its target t_p (label
+ * 1), r > 0 negative r-1 (label 0, skipped when equal to t_p).  For each
+ * column, with y~ = y_n and the contexts c of win(p) in ascending order:
+ *   g[c][n] = (label - sig(x_c . y~)) alpha (wneg for negatives);  y~ += g x_c
+ *   dX[c] = sum_n g[c][n] y_n ;  dY[n] = sum_c g[c][n] x_c   (staged rows)
+ * win(p) = {c != p : |c - p| <= W - b_p}.  Row contributions per tile in the
+ * order contexts (ascending c), then columns (ascending n).
  * ---------------------------------------------------------------------- */
-#define DGO_TILE 22
-
 static double sig_d(double f, const float* expt, int exact) {
   if (exact) return 1.0 / (1.0 + exp(-f));
   if (f > MAX_EXP) return 1.0;
@@ -282,17 +290,16 @@ static int cmp_contrib(const void* a, const void* b) {
   return (x->order > y->order) - (x->order < y->order);
 }
 
-static double ddot(const float* x, const float* y, uint32_t dim) {
-  double s = 0.0;
-  for (uint32_t d = 0; d < dim; ++d) s += (double)x[d] * (double)y[d];
-  return s;
+uint32_t dgo_tile_positions(uint32_t draws, uint32_t tile) {
+  (void)draws;
+  return tile ? tile : 8;
 }
 
-uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen) {
-  uint64_t n = 0;
+uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen, uint32_t tile) {
+  uint64_t n = 0, T = tile;
   for (uint64_t s = 0; s < n_sen; ++s) {
     uint64_t L = sen_off[s + 1] - sen_off[s];
-    if (L >= 2) n += (L + DGO_TILE - 1) / DGO_TILE;
+    if (L >= 2) n += (L + T - 1) / T;
   }
   return n;
 }
@@ -301,25 +308,29 @@ void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_
                     uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
                     const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
                     const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
-                    uint32_t dim, float* syn0, float* syn1) {
-  const int K = (int)negative, W = (int)window;
-  /* tile list in order */
-  uint64_t n_tiles = dgo_tile_count(sen_off, n_sen);
+                    uint32_t dim, uint32_t tile, uint32_t draws, float* syn0, float* syn1) {
+  const int K = (int)(draws ? draws : negative), W = (int)window, K1 = K + 1;
+  const float wneg = (float)negative / (float)K;
+  const int T = (int)dgo_tile_positions((uint32_t)K, tile);
+  uint64_t n_tiles = dgo_tile_count(sen_off, n_sen, (uint32_t)T);
   uint64_t* ts = (uint64_t*)malloc(sizeof(uint64_t) * (n_tiles ? n_tiles : 1));
   uint32_t* tj = (uint32_t*)malloc(sizeof(uint32_t) * (n_tiles ? n_tiles : 1));
   uint64_t nt = 0;
   for (uint64_t s = 0; s < n_sen; ++s) {
     uint64_t L = sen_off[s + 1] - sen_off[s];
     if (L < 2) continue;
-    for (uint32_t j = 0; j * DGO_TILE < L; ++j) {
+    for (uint32_t j = 0; (uint64_t)j * T < L; ++j) {
       ts[nt] = s;
       tj[nt] = j;
       ++nt;
     }
   }
+  int32_t* out = (int32_t*)malloc(sizeof(int32_t) * (size_t)T * K1);
+  int* b = (int*)malloc(sizeof(int) * (size_t)T);
+  double* ytil = (double*)malloc(sizeof(double) * dim);
   for (uint64_t t0 = 0; t0 < n_tiles; t0 += minibatch) {
     uint64_t t1 = t0 + minibatch < n_tiles ? t0 + minibatch : n_tiles;
-    uint64_t cap = (t1 - t0) * (2 * DGO_TILE + 2 * (uint64_t)W + (uint64_t)K);
+    uint64_t cap = (t1 - t0) * ((uint64_t)T * K1 + (uint64_t)T + 2 * (uint64_t)W);
     contrib_t* cs = (contrib_t*)malloc(sizeof(contrib_t) * (cap ? cap : 1));
     double* vecs = (double*)calloc(cap ? cap * dim : 1, sizeof(double));
     uint64_t ns = 0;
@@ -327,72 +338,67 @@ void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_
       uint64_t s = ts[ti];
       const int32_t* t = sen + sen_off[s];
       int L = (int)(sen_off[s + 1] - sen_off[s]);
-      int p0 = (int)tj[ti] * DGO_TILE, p1 = p0 + DGO_TILE < L ? p0 + DGO_TILE : L;
-      uint64_t gs = first_sen + s;
+      int p0 = (int)tj[ti] * T, p1 = p0 + T < L ? p0 + T : L;
+      int np = p1 - p0;
       dg_stream st;
-      dg_stream_init(&st, seed, (uint32_t)gs, tj[ti] | (epoch << 16), DG_TAG_SGNS);
-      int b[DGO_TILE];
-      int32_t neg[64];
-      for (int p = p0; p < p1; ++p) b[p - p0] = (int)(dg_stream_next(&st) % window);
-      for (int k = 0; k < K; ++k) {
-        uint32_t x = dg_stream_next(&st);
-        int32_t n = table[x % table_size];
-        if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
-        neg[k] = n;
-      }
+      dg_stream_init(&st, seed, (uint32_t)(first_sen + s), tj[ti] | (epoch << 16), DG_TAG_SGNS);
+      for (int q = 0; q < np; ++q) b[q] = (int)(dg_stream_next(&st) % window);
+      for (int q = 0; q < np; ++q) out[q * K1] = t[p0 + q];
+      for (int q = 0; q < np; ++q)
+        for (int k = 0; k < K; ++k) {
+          uint32_t x = dg_stream_next(&st);
+          int32_t n = table[x % table_size];
+          if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
+          out[q * K1 + 1 + k] = n;
+        }
       int c_lo = p0 - W > 0 ? p0 - W : 0, c_hi = p1 + W < L ? p1 + W : L;
-      int nc = c_hi - c_lo, np = p1 - p0;
-      double* gp = (double*)calloc((size_t)nc * np + 1, sizeof(double));
-      double* gn = (double*)calloc((size_t)nc * (K + 1), sizeof(double));
-      for (int c = c_lo; c < c_hi; ++c) {
-        const float* x = syn0 + (uint64_t)t[c] * dim;
-        int m[64] = {0};
-        for (int p = p0; p < p1; ++p) {
-          int w = W - b[p - p0];
-          if (c == p || c < p - w || c > p + w) continue;
-          for (int k = 0; k < K; ++k) m[k] += (neg[k] != t[p]);
-          double f = ddot(x, syn1 + (uint64_t)t[p] * dim, dim);
-          gp[(size_t)(c - c_lo) * np + (p - p0)] = (1.0 - sig_d(f, expt, exact_sigmoid)) * alpha;
-        }
-        for (int k = 0; k < K; ++k) {
-          if (!m[k]) continue;
-          double f = ddot(x, syn1 + (uint64_t)neg[k] * dim, dim);
-          gn[(size_t)(c - c_lo) * K + k] = (0.0 - sig_d(f, expt, exact_sigmoid)) * alpha * m[k];
+      int nc = c_hi - c_lo;
+      int no = np * K1;
+      double* g = (double*)calloc((size_t)nc * no + 1, sizeof(double)); /* [c][n] */
+      for (int n = 0; n < no; ++n) {
+        int q = n / K1, r = n % K1, p = p0 + q, w = W - b[q];
+        int32_t row = out[n];
+        if (r > 0 && row == t[p]) continue;
+        double lab = r == 0 ? 1.0 : 0.0;
+        for (uint32_t d = 0; d < dim; ++d) ytil[d] = (double)syn1[(uint64_t)row * dim + d];
+        for (int c = p - w > c_lo ? p - w : c_lo; c <= p + w && c < c_hi; ++c) {
+          if (c == p) continue;
+          const float* x = syn0 + (uint64_t)t[c] * dim;
+          double f = 0.0;
+          for (uint32_t d = 0; d < dim; ++d) f += (double)x[d] * ytil[d];
+          double gg = (lab - sig_d(f, expt, exact_sigmoid)) * alpha;
+          if (r > 0) gg *= wneg;
+          g[(size_t)(c - c_lo) * no + n] = gg;
+          for (uint32_t d = 0; d < dim; ++d) ytil[d] += gg * (double)x[d];
         }
       }
-      for (int c = c_lo; c < c_hi; ++c) {
+      for (int c = 0; c < nc; ++c) {
         double* v = vecs + ns * dim;
-        for (int p = 0; p < np; ++p) {
-          double g = gp[(size_t)(c - c_lo) * np + p];
-          if (g != 0.0)
-            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn1[(uint64_t)t[p0 + p] * dim + d];
+        for (int n = 0; n < no; ++n) {
+          double gg = g[(size_t)c * no + n];
+          if (gg == 0.0) continue;
+          for (uint32_t d = 0; d < dim; ++d) v[d] += gg * syn1[(uint64_t)out[n] * dim + d];
         }
-        for (int k = 0; k < K; ++k) {
-          double g = gn[(size_t)(c - c_lo) * K + k];
-          if (g != 0.0)
-            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn1[(uint64_t)neg[k] * dim + d];
-        }
-        cs[ns].row = (uint32_t)t[c];
+        cs[ns].row = (uint32_t)t[c_lo + c];
         cs[ns].table = 0;
         cs[ns].order = ns;
         cs[ns].vec = v;
         ++ns;
       }
-      for (int p = 0; p < np + K; ++p) {
+      for (int n = 0; n < no; ++n) {
         double* v = vecs + ns * dim;
         for (int c = 0; c < nc; ++c) {
-          double g = p < np ? gp[(size_t)c * np + p] : gn[(size_t)c * K + (p - np)];
-          if (g != 0.0)
-            for (uint32_t d = 0; d < dim; ++d) v[d] += g * syn0[(uint64_t)t[c_lo + c] * dim + d];
+          double gg = g[(size_t)c * no + n];
+          if (gg != 0.0)
+            for (uint32_t d = 0; d < dim; ++d) v[d] += gg * syn0[(uint64_t)t[c_lo + c] * dim + d];
         }
-        cs[ns].row = p < np ? (uint32_t)t[p0 + p] : (uint32_t)neg[p - np];
+        cs[ns].row = (uint32_t)out[n];
         cs[ns].table = 1;
         cs[ns].order = ns;
         cs[ns].vec = v;
         ++ns;
       }
-      free(gp);
-      free(gn);
+      free(g);
     }
     qsort(cs, ns, sizeof(contrib_t), cmp_contrib);
     double* acc = (double*)malloc(sizeof(double) * dim);
@@ -412,6 +418,9 @@ void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_
     free(cs);
     free(vecs);
   }
+  free(ytil);
+  free(b);
+  free(out);
   free(ts);
   free(tj);
 }

@@ -37,10 +37,10 @@ def lib():
     L.dgo_plain_batches.restype = c.c_uint64
     L.dgo_plain_batches.argtypes = [vp, c.c_uint64, vp, c.c_uint64]
     L.dgo_tile_count.restype = c.c_uint64
-    L.dgo_tile_count.argtypes = [vp, c.c_uint64]
+    L.dgo_tile_count.argtypes = [vp, c.c_uint64, c.c_uint32]
     L.dgo_sg_batched.argtypes = [vp, vp, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint32,
                                  c.c_uint32, vp, c.c_uint32, c.c_uint32, c.c_float, vp, c.c_int,
-                                 c.c_uint64, c.c_uint32, c.c_uint32, vp, vp]
+                                 c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp]
     L.dgo_tuple_loss.restype = c.c_double
     L.dgo_tuple_loss.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32]
     _L = L
@@ -105,12 +105,13 @@ class Trainer:
         return nb
 
     def batched(self, sen, off, window, negative, alpha, minibatch, exact, seed, epoch,
-                first_sen=0):
+                first_sen=0, tile=0, draws=0):
         L = lib()
         nsen = len(off) - 1
         L.dgo_sg_batched(_vp(sen), _vp(off), first_sen, nsen, minibatch, window, negative,
                          _vp(self.table), self.table_size, self.vocab, alpha, _vp(self.expt),
-                         int(exact), seed, epoch, self.dim, _vp(self.syn0), _vp(self.syn1))
+                         int(exact), seed, epoch, self.dim, tile, draws, _vp(self.syn0),
+                         _vp(self.syn1))
 
 
 def tuple_loss(tuples, K, syn0, syn1):
