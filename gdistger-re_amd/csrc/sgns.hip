@@ -187,32 +187,47 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
   }
 }
 
-// ---- batched form: position tiles on MFMA ----------------------------------
-// A tile is T consecutive positions p0 .. p1-1 of a sentence.  Its context
-// rows X (positions c_lo .. c_hi-1, c_lo = max(0, p0-W), c_hi = min(L, p1+W))
-// and output rows Y (per position j, the columns j(1+K) .. j(1+K)+K hold its
-// target and its K negatives) are staged in LDS, then
-//   S  = X Y^T,  Gx = X X^T                                (MFMA)
-//   per output column n (position p, label l, weight w), over the contexts
-//   c_1 < c_2 < ... of the shrunk window of p:
-//     s_k = S[c_k][n] + sum_{j<k} g_j Gx[c_k][c_j],   g_k = alpha w (l - sig(s_k))
-//   dX = G Y,  dY = G^T X                                  (MFMA)
-// s_k equals x_{c_k} . (y_n + sum_{j<k} g_j x_{c_j}): the output row as it
-// stands after the updates of the earlier contexts of the same position,
-// which is the order of skip_gram_kernel (word2vec.cu:360-396), where every
-// (context, target) pair updates the target row before the next context is
-// read.  The Gram matrix turns that dependency chain into a scalar
-// recursion, so the row products stay on MFMA.  dX is taken with the rows of
-// Y as staged (the context side of the reference update differs from this
-// by terms of order alpha^2).
-// Rows live in HBM as FP32 only.  The FP32 form stages them as they are and
-// runs v_mfma_f32_16x16x4f32; the BF16 form rounds them to BF16 while staging
-// and runs v_mfma_f32_16x16x32_bf16 (logits, Gram) and 16x16x16_bf16 (the
-// update products, G rounded to BF16), both with FP32 accumulation, adding
-// its FP32 results to the FP32 rows.  Every staged row is also updated by the tile that staged it, so a
-// BF16 copy in HBM would need a refresh (4 B read + 2 B written per element)
-// for each 2 B it saves on the gather, and would be stale in between.
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+// ---- batched form: sentence-resident pools on MFMA ---------------------------
+// One wavefront trains one sentence, position after position, in the order
+// of skip_gram_kernel (word2vec.cu:305-399), with the K negatives of a
+// position drawn once and shared by all its contexts (the per-position
+// negatives of dsgl.hpp:1100-1159).
+//
+// The distinct vertices of the sentence (walks revisit vertices: a
+// 22-vertex walk holds about 12) form a pool whose input rows live in
+// registers for the whole sentence: the row as loaded (E) plus an FP32
+// delta that the update product accumulates into directly, so every
+// position reads the input rows as the earlier positions left them.  Per
+// position, with X the pool rows, Y the target and negative rows (staged in
+// LDS from HBM) and the contexts u_1 < u_2 < ... of the shrunk window:
+//   S = X Y^T,  Gx = X X^T,  Gy = Y Y^T                        (MFMA)
+//   per output column n (lanes 0..K), over the contexts in order:
+//     s_i = S[u_i][n] + sum_{j<i} g_j[n] Gx[u_i][u_j]
+//                     + sum_{j<i, u_j = u_i} sum_n' g_j[n'] Gy[n'][n]
+//     g_i[n] = alpha w_n (label_n - sig(s_i))
+//   Gp[u][n] = sum_{i: u_i = u} g_i[n]
+//   H[u][v]  = sum_{j<i, u_i = u, u_j = v} g_i . g_j
+//   M[n][n'] = sum_{j<i, u_j = u_i} g_i[n] g_j[n']
+//   dX = Gp Y + H X (into the register deltas),
+//   dY = Gp^T X + M Y (atomics to HBM)                          (MFMA)
+// This is the sequential order to first order in the updates: the second
+// term of s_i is the output row as updated by the earlier contexts of the
+// position, the third the context row as updated by its own earlier
+// occurrences in the window; H X is the part of a context row's update that
+// comes from output rows already moved by earlier contexts (all of it while
+// the output rows are still zero), M Y the part of an output row's update
+// that comes from context rows already moved.  Changes of changes (and the
+// coupling of two equal negatives of one position) are dropped.  Output rows are read
+// from HBM at each position, so a row updated by one position is seen by
+// the next ones.  The deltas are added to the FP32 input rows when the
+// sentence (or a segment of it, if its windows hold more distinct vertices
+// than the pool) is done.
+//
+// Rows live in HBM as FP32 only; the BF16 form rounds them to BF16 as they
+// are staged and runs v_mfma_f32_16x16x16_bf16, the FP32 form runs
+// v_mfma_f32_16x16x4f32, both accumulating in FP32.  Every staged row is
+// also updated, so a BF16 copy in HBM would need a refresh (4 B read + 2 B
+// written per element) for each 2 B it saves on the gather.
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
@@ -220,121 +235,71 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   const uint32_t u = __float_as_uint(f);
   return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+constexpr int kSentWaves = 2;   // sentences (wavefronts) per block
+constexpr int kOutCols = 16;    // output columns: the target and up to 15 negatives
+constexpr int kSpan = 64;       // positions of a segment's windows held at once
+constexpr int kMaxSentWindow = 15;
+constexpr int kMaxSlots = 2 * kMaxSentWindow;  // contexts of one position
+
+// A 16 x 16 x 16 product over four-element k chunks.  For one MFMA call,
+// lane l supplies chunk element `e` of k index 4 (l/16) + e of row or column
+// l % 16; the caller passes the four k values of a chunk in order.  The FP32
+// form issues four 16x16x4 steps (k = 4 (l/16) + e, e = 0..3), the BF16 form
+// one 16x16x16 step; both sum the same products.
 template <typename E>
-struct Mfma;
+struct Chunk;
 
 template <>
-struct Mfma<float> {
-  // logits / Gram products (k = dim) and update products (k = rows) both on
-  // v_mfma_f32_16x16x4f32
-  static constexpr int KS = 4;
-  static constexpr int KU = 4;
-  typedef float frag_t;
-  typedef float ufrag_t;
-  // M[r0 + lane%16][k0 + lane/16]
-  __device__ static frag_t row_frag(const float* M, int ld, int r0, int k0, int lane) {
-    return M[(r0 + (lane & 15)) * ld + k0 + (lane >> 4)];
+struct Chunk<float> {
+  typedef float4 v_t;  // four k values
+  __device__ static f32x4_t mma(v_t a, v_t b, f32x4_t c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
   }
-  __device__ static f32x4_t mma(frag_t a, frag_t b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  __device__ static v_t from(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+  __device__ static float elem(const float* p) { return *p; }
+  __device__ static void put4(float* dst, float a, float b, float c, float d) {
+    *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
   }
-  // update products: A from the FP32 gradient matrix, B from staged rows
-  __device__ static ufrag_t urow_f(const float* M, int ld, int r0, int k0, int lane) {
-    return M[(r0 + (lane & 15)) * ld + k0 + (lane >> 4)];
-  }
-  __device__ static ufrag_t ucol_f(const float* M, int ld, int k0, int c0, int lane) {
-    return M[(k0 + (lane >> 4)) * ld + c0 + (lane & 15)];
-  }
-  __device__ static ufrag_t ucol(const float* M, int ld, int k0, int c0, int lane) {
-    return M[(k0 + (lane >> 4)) * ld + c0 + (lane & 15)];
-  }
-  __device__ static f32x4_t mmau(ufrag_t a, ufrag_t b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
+  // four consecutive stored elements
+  __device__ static v_t load4(const float* src) { return *reinterpret_cast<const float4*>(src); }
 };
 
 template <>
-struct Mfma<uint16_t> {
-  // logits / Gram products on v_mfma_f32_16x16x32_bf16 (k = dim); update
-  // products on v_mfma_f32_16x16x16_bf16, so row counts pad to 16 only
-  static constexpr int KS = 32;
-  static constexpr int KU = 16;
-  typedef bf16x8_t frag_t;
-  typedef bf16x4_t ufrag_t;
+struct Chunk<uint16_t> {
   union U {
-    uint4 q;
-    uint16_t h[8];
-    bf16x8_t v;
-  };
-  union U4 {
     uint2 q;
     uint16_t h[4];
     bf16x4_t v;
   };
-  // M[r0 + lane%16][k0 + 8 (lane/16) .. +7]
-  __device__ static frag_t row_frag(const uint16_t* M, int ld, int r0, int k0, int lane) {
-    U u;
-    u.q = *reinterpret_cast<const uint4*>(M + (r0 + (lane & 15)) * ld + k0 + 8 * (lane >> 4));
-    return u.v;
-  }
-  __device__ static f32x4_t mma(frag_t a, frag_t b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-  // FP32 M[r0 + lane%16][k0 + 4 (lane/16) .. +3] rounded to BF16 (16-byte aligned rows)
-  __device__ static ufrag_t urow_f(const float* M, int ld, int r0, int k0, int lane) {
-    const float4 a = *reinterpret_cast<const float4*>(M + (r0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4));
-    U4 u;
-    u.h[0] = f2bf(a.x);
-    u.h[1] = f2bf(a.y);
-    u.h[2] = f2bf(a.z);
-    u.h[3] = f2bf(a.w);
-    return u.v;
-  }
-  // FP32 M[k0 + 4 (lane/16) + i][c0 + lane%16], i = 0..3, rounded to BF16
-  __device__ static ufrag_t ucol_f(const float* M, int ld, int k0, int c0, int lane) {
-    U4 u;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) u.h[i] = f2bf(M[(k0 + 4 * (lane >> 4) + i) * ld + c0 + (lane & 15)]);
-    return u.v;
-  }
-  // BF16 M[k0 + 4 (lane/16) + i][c0 + lane%16], i = 0..3
-  __device__ static ufrag_t ucol(const uint16_t* M, int ld, int k0, int c0, int lane) {
-    U4 u;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) u.h[i] = M[(k0 + 4 * (lane >> 4) + i) * ld + c0 + (lane & 15)];
-    return u.v;
-  }
-  __device__ static f32x4_t mmau(ufrag_t a, ufrag_t b, f32x4_t c) {
+  typedef bf16x4_t v_t;
+  __device__ static f32x4_t mma(v_t a, v_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
   }
-};
-
-// Tile descriptor, built by tile_desc_kernel for every tile of a launch:
-//   [0] p0, [1] c_lo, [2] nc (context rows), [3] np (positions),
-//   [4 .. 4+T)            window shrink b of each position,
-//   [4+T .. 4+T+CTX)      context rows (vocabulary ranks, -1 past nc),
-//   [4+T+CTX .. +NOUT)    output rows per column (-1 past np (1+K)).
-constexpr int kDescHead = 4;
-
-struct DescArgs {
-  const int32_t* sen;
-  const uint64_t* sen_off;
-  const int32_t* table;
-  const uint2* tiles;  // {sentence index, tile index within the sentence}
-  uint64_t n_tiles;
-  uint64_t first_sentence;
-  uint64_t seed;
-  uint32_t epoch;
-  uint32_t table_size;
-  uint32_t vocab;
-  int window;
-  int kd;  // negatives drawn per position
-  int T;
-  int ctx;
-  int nout;
-  uint32_t stride;  // ints per descriptor
-  int32_t* desc;
+  __device__ static v_t from(float a, float b, float c, float d) {
+    U u;
+    u.h[0] = f2bf(a);
+    u.h[1] = f2bf(b);
+    u.h[2] = f2bf(c);
+    u.h[3] = f2bf(d);
+    return u.v;
+  }
+  __device__ static float elem(const uint16_t* p) { return bf2f(*p); }
+  __device__ static void put4(uint16_t* dst, float a, float b, float c, float d) {
+    uint2 h;
+    h.x = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+    h.y = (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16);
+    *reinterpret_cast<uint2*>(dst) = h;
+  }
+  __device__ static v_t load4(const uint16_t* src) {
+    U u;
+    u.q = *reinterpret_cast<const uint2*>(src);
+    return u.v;
+  }
 };
 
 // word j of a Philox stream (block j/4, word j%4)
@@ -345,260 +310,406 @@ __device__ __forceinline__ uint32_t stream_word(uint64_t seed, uint32_t c1, uint
   return r == 0 ? o.x0 : (r == 1 ? o.x1 : (r == 2 ? o.x2 : o.x3));
 }
 
-// One wavefront per tile.  Draws: word q < np of the tile's stream gives
-// b = word % window of position p0+q; word np + q K + k gives negative k of
-// position q: table[word % table_size], 0 -> word % (vocab-1) + 1.
-__global__ __launch_bounds__(256) void tile_desc_kernel(DescArgs a) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const int K = a.kd, W = a.window;
-  for (uint64_t ti = wave; ti < a.n_tiles; ti += nwaves) {
-    const uint2 tl = a.tiles[ti];
-    const uint64_t sb = a.sen_off[tl.x];
-    const int L = (int)(a.sen_off[tl.x + 1] - sb);
-    const int p0 = (int)tl.y * a.T;
-    const int p1 = p0 + a.T < L ? p0 + a.T : L;
-    const int np = p1 - p0;
-    const int c_lo = p0 - W > 0 ? p0 - W : 0;
-    const int c_hi = p1 + W < L ? p1 + W : L;
-    const int nc = c_hi - c_lo;
-    const uint32_t c1 = (uint32_t)(a.first_sentence + tl.x), c2 = tl.y | (a.epoch << 16);
-    int32_t* d = a.desc + ti * a.stride;
-    const int n_ent = kDescHead + a.T + a.ctx + a.nout;
-    for (int e = lane; e < n_ent; e += 64) {
-      int v;
-      if (e < kDescHead) {
-        v = e == 0 ? p0 : (e == 1 ? c_lo : (e == 2 ? nc : np));
-      } else if (e < kDescHead + a.T) {
-        const int q = e - kDescHead;
-        v = q < np ? (int)(stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)q) % (uint32_t)W) : 0;
-      } else if (e < kDescHead + a.T + a.ctx) {
-        const int m = e - kDescHead - a.T;
-        v = m < nc ? a.sen[sb + c_lo + m] : -1;
-      } else {
-        const int o = e - kDescHead - a.T - a.ctx;
-        const int j = o / (1 + K), r = o % (1 + K);
-        if (j >= np) {
-          v = -1;
-        } else if (r == 0) {
-          v = a.sen[sb + p0 + j];
-        } else {
-          const uint32_t w = stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)(np + j * K + r - 1));
-          v = a.table[w % a.table_size];
-          if (v == 0) v = (int)(w % (a.vocab - 1) + 1);
-        }
-      }
-      d[e] = v;
-    }
-  }
-}
-
-struct TileArgs {
-  SgnsDev a;
-  const int32_t* desc;
-  uint32_t desc_stride;
-  uint64_t n_tiles;
-  int kd;      // negatives drawn per position
-  float wneg;  // negative / kd: weight of one drawn negative
-  int exact;
-  int deterministic;
-  float* stage;     // deterministic: per tile (CTX + NOUT) slots of dim floats
-  uint32_t* keys;   // key = row | table << row_bits; 0xffffffff = unused slot
-  uint32_t* vals;
-  uint32_t row_bits;
+struct DrawArgs {
+  const int32_t* sen;
+  const uint64_t* sen_off;  // of the epoch's sentences
+  uint64_t s_begin, s_end;  // sentences of the launch
+  uint64_t first_sentence;  // Philox counter of sentence 0 of the epoch
+  uint64_t seed;
+  uint32_t epoch;
+  const int32_t* table;
+  uint32_t table_size;
+  uint32_t vocab;
+  uint32_t window;
+  int kd;
+  uint8_t* shrink;  // per kept position of the launch
+  int32_t* negs;    // kd per kept position of the launch
+  uint8_t* ucount;  // per sentence of the launch: distinct vertices, 255 if above 64
 };
 
-template <typename E, int DIM, int T, int CTX, int NOUT>
-__global__ __launch_bounds__(256) void sg_tile_kernel(TileArgs t) {
-  typedef Mfma<E> M;
-  constexpr int PAD = 16 / (int)sizeof(E);
-  constexpr int LD = DIM + PAD;   // staged rows (elements)
-  constexpr int SLD = NOUT + 4;   // S / G rows (floats, 16-byte aligned)
-  constexpr int XLD = CTX + 1;    // Gx rows (floats)
-  constexpr int C4 = DIM / 4;     // four-element chunks per row
-  constexpr int NCHUNK = (CTX + NOUT) * C4;
-  constexpr int NLD = (NCHUNK + 255) / 256;  // chunks per thread
-  constexpr int BATCH = NLD < 12 ? NLD : 12; // loads in flight per thread
-  constexpr int NDESC = kDescHead + T + CTX + NOUT;
-  static_assert(CTX % 16 == 0 && NOUT % 16 == 0 && NDESC <= 256, "tile shape");
-  __shared__ __attribute__((aligned(16))) E sX[CTX * LD];
-  __shared__ __attribute__((aligned(16))) E sY[NOUT * LD];
-  __shared__ __attribute__((aligned(16))) float sS[CTX * SLD];
-  __shared__ float sGx[CTX * XLD];
-  __shared__ float s_expt[kExpTable + 1];
-  __shared__ int s_d[NDESC];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const SgnsDev& a = t.a;
-  const int W = (int)a.window, K1 = t.kd + 1;
-  const int* s_b = s_d + kDescHead;
-  const int* s_ctx = s_d + kDescHead + T;
-  const int* s_out = s_d + kDescHead + T + CTX;
-  constexpr int NT_OUT = NOUT / 16;
-  constexpr int NT_DIM = DIM / 16;
-  for (int i = tid; i <= kExpTable; i += 256) s_expt[i] = a.expt[i];
-  // descriptor of the first tile; later ones are loaded one tile ahead
-  int dnext = 0;
-  if (blockIdx.x < t.n_tiles && tid < NDESC) dnext = t.desc[(uint64_t)blockIdx.x * t.desc_stride + tid];
-  for (uint64_t ti = blockIdx.x; ti < t.n_tiles; ti += gridDim.x) {
-    if (tid < NDESC) s_d[tid] = dnext;
-    __syncthreads();
-    const int p0 = s_d[0], c_lo = s_d[1], nc = s_d[2], np = s_d[3];
-    {
-      const uint64_t tn = ti + gridDim.x;
-      if (tn < t.n_tiles && tid < NDESC) dnext = t.desc[tn * t.desc_stride + tid];
+// Draws of every position: the stream of position q of sentence s (counter
+// (blk, first_sentence + s, q | epoch << 20, SGNS)): word 0 gives b = word %
+// window, word k negative k: table[word % table_size], 0 -> word % (vocab-1) + 1.
+__global__ void sent_draw_kernel(DrawArgs a) {
+  const uint64_t base = a.sen_off[a.s_begin];
+  const uint64_t n = a.sen_off[a.s_end] - base;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    // sentence of kept position base + i
+    uint64_t lo = a.s_begin, hi = a.s_end;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (a.sen_off[mid] <= base + i) lo = mid; else hi = mid;
     }
-    // stage the rows: four FP32 elements per load, BATCH loads in flight, each
-    // rounded to E as stored
-#pragma unroll
-    for (int b0 = 0; b0 < NLD; b0 += BATCH) {
-      float4 v[BATCH];
-#pragma unroll
-      for (int q = 0; q < BATCH; ++q) {
-        const int i = (b0 + q) * 256 + tid;
-        v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (b0 + q < NLD && i < NCHUNK) {
-          const int r = i / C4, c = i % C4;
-          const int row = r < CTX ? s_ctx[r] : s_out[r - CTX];
-          if (row >= 0)
-            v[q] = reinterpret_cast<const float4*>((r < CTX ? a.syn0 : a.syn1) + (uint64_t)row * DIM)[c];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < BATCH; ++q) {
-        const int i = (b0 + q) * 256 + tid;
-        if (b0 + q < NLD && i < NCHUNK) {
-          const int r = i / C4, c = i % C4;
-          E* dst = (r < CTX ? sX + r * LD : sY + (r - CTX) * LD) + 4 * c;
-          if constexpr (sizeof(E) == 4) {
-            *reinterpret_cast<float4*>(dst) = v[q];
-          } else {
-            uint2 h;
-            h.x = (uint32_t)f2bf(v[q].x) | ((uint32_t)f2bf(v[q].y) << 16);
-            h.y = (uint32_t)f2bf(v[q].z) | ((uint32_t)f2bf(v[q].w) << 16);
-            *reinterpret_cast<uint2*>(dst) = h;
-          }
-        }
-      }
+    const uint32_t q = (uint32_t)(base + i - a.sen_off[lo]);
+    const uint32_t c1 = (uint32_t)(a.first_sentence + lo), c2 = q | (a.epoch << 20);
+    a.shrink[i] = (uint8_t)(stream_word(a.seed, c1, c2, DG_TAG_SGNS, 0) % a.window);
+    for (int k = 0; k < a.kd; ++k) {
+      const uint32_t w = stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)(k + 1));
+      int v = a.table[w % a.table_size];
+      if (v == 0) v = (int)(w % (a.vocab - 1) + 1);
+      a.negs[i * a.kd + k] = v;
     }
-    __syncthreads();
-    const int nM = (nc + 15) / 16;
-    const int kc = nM * 16;  // <= CTX, a multiple of the update k step
-    // S = X Y^T (nM x NT_OUT tiles) and Gx = X X^T (nM x nM tiles)
-    for (int tt = wave; tt < nM * NT_OUT + nM * nM; tt += 4) {
-      const bool is_s = tt < nM * NT_OUT;
-      const int u = is_s ? tt : tt - nM * NT_OUT;
-      const int mt = is_s ? u / NT_OUT : u / nM;
-      const int nt = is_s ? u % NT_OUT : u % nM;
-      const E* B = is_s ? sY : sX;
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-      for (int k0 = 0; k0 < DIM; k0 += M::KS)
-        acc = M::mma(M::row_frag(sX, LD, mt * 16, k0, lane), M::row_frag(B, LD, nt * 16, k0, lane), acc);
-      float* dst = is_s ? sS : sGx;
-      const int ld = is_s ? SLD : XLD;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        dst[(mt * 16 + 4 * (lane >> 4) + i) * ld + nt * 16 + (lane & 15)] = acc[i];
-    }
-    __syncthreads();
-    // the per-column recursion; columns spread over the four waves
-    {
-      const int n = (tid & 63) * 4 + wave;
-      if (n < NOUT) {
-        const int j = n / K1, r = n % K1;
-        const int row = s_out[n];
-        const bool live = j < np && row >= 0 && (r == 0 || row != s_out[j * K1]);
-        int m_lo = 0, m_hi = -1, mc = -1;
-        if (live) {
-          const int p = p0 + j, w = W - s_b[j];
-          m_lo = (p - w > c_lo ? p - w : c_lo) - c_lo;
-          m_hi = (p + w < c_lo + nc - 1 ? p + w : c_lo + nc - 1) - c_lo;
-          mc = p - c_lo;
-        }
-        const float lab = r == 0 ? 1.0f : 0.0f;
-        for (int m = 0; m < kc; ++m) {
-          float g = 0.0f;
-          if (m >= m_lo && m <= m_hi && m != mc) {
-            float sv = sS[m * SLD + n];
-            for (int m2 = m_lo; m2 < m; ++m2)
-              if (m2 != mc) sv += sS[m2 * SLD + n] * sGx[m * XLD + m2];
-            g = (lab - sigmoid_of(sv, s_expt, t.exact)) * a.alpha;
-            if (r != 0) g *= t.wneg;
-          }
-          sS[m * SLD + n] = g;
-        }
-      }
-    }
-    __syncthreads();
-    const uint64_t slot0 = ti * (uint64_t)(CTX + NOUT);
-    const uint32_t tb = 1u << t.row_bits;
-    // dX = G Y -> input rows
-    for (int tt = wave; tt < nM * NT_DIM; tt += 4) {
-      const int mt = tt / NT_DIM, nt = tt % NT_DIM;
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int k0 = 0; k0 < NOUT; k0 += M::KU)
-        acc = M::mmau(M::urow_f(sS, SLD, mt * 16, k0, lane), M::ucol(sY, LD, k0, nt * 16, lane), acc);
-      const int dd = nt * 16 + (lane & 15);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = mt * 16 + 4 * (lane >> 4) + i;
-        if (m >= nc) continue;
-        if (t.deterministic)
-          t.stage[(slot0 + m) * DIM + dd] = acc[i];
-        else
-          atomicAdd(&a.syn0[(uint64_t)s_ctx[m] * DIM + dd], acc[i]);
-      }
-    }
-    // dY = G^T X -> output rows
-    for (int tt = wave; tt < NT_OUT * NT_DIM; tt += 4) {
-      const int mt = tt / NT_DIM, nt = tt % NT_DIM;
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k0 = 0; k0 < kc; k0 += M::KU)
-        acc = M::mmau(M::ucol_f(sS, SLD, k0, mt * 16, lane), M::ucol(sX, LD, k0, nt * 16, lane), acc);
-      const int dd = nt * 16 + (lane & 15);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = mt * 16 + 4 * (lane >> 4) + i;
-        const int rr = s_out[m];
-        if (rr < 0) continue;
-        if (t.deterministic)
-          t.stage[(slot0 + CTX + m) * DIM + dd] = acc[i];
-        else
-          atomicAdd(&a.syn1[(uint64_t)rr * DIM + dd], acc[i]);
-      }
-    }
-    if (t.deterministic) {
-      if (tid < CTX) {
-        t.keys[slot0 + tid] = tid < nc ? (uint32_t)s_ctx[tid] : 0xffffffffu;
-        t.vals[slot0 + tid] = (uint32_t)(slot0 + tid);
-      }
-      if (tid < NOUT) {
-        t.keys[slot0 + CTX + tid] = s_out[tid] >= 0 ? ((uint32_t)s_out[tid] | tb) : 0xffffffffu;
-        t.vals[slot0 + CTX + tid] = (uint32_t)(slot0 + CTX + tid);
-      }
-    }
-    __syncthreads();
   }
 }
 
-// tiles per sentence: ceil(L / T) for L >= 2
-__global__ void tile_count_kernel(const uint64_t* sen_off, uint64_t n_sen, uint32_t tile,
-                                  uint32_t* cnt) {
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_sen;
+// Distinct vertices per sentence (255 when the sentence is longer than 64).
+__global__ void sent_count_kernel(DrawArgs a) {
+  for (uint64_t s = a.s_begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.s_end;
        s += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t L = sen_off[s + 1] - sen_off[s];
-    cnt[s] = L >= 2 ? (uint32_t)((L + tile - 1) / tile) : 0u;
+    const uint64_t sb = a.sen_off[s];
+    const int L = (int)(a.sen_off[s + 1] - sb);
+    int u = 255;
+    if (L <= 64) {
+      u = 0;
+      for (int i = 0; i < L; ++i) {
+        const int v = a.sen[sb + i];
+        bool seen = false;
+        for (int j = 0; j < i; ++j) seen |= a.sen[sb + j] == v;
+        u += !seen;
+      }
+    }
+    a.ucount[s - a.s_begin] = (uint8_t)u;
   }
 }
 
-__global__ void tile_list_kernel(const uint32_t* cnt, const uint64_t* off, uint64_t n_sen,
-                                 uint2* tiles) {
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_sen;
-       s += (uint64_t)gridDim.x * blockDim.x)
-    for (uint32_t k = 0; k < cnt[s]; ++k) tiles[off[s] + k] = make_uint2((uint32_t)s, k);
+struct SentArgs {
+  const int32_t* sen;
+  const uint64_t* sen_off;
+  uint64_t s_begin, s_end;
+  const uint8_t* shrink;
+  const int32_t* negs;
+  const uint8_t* ucount;
+  const float* expt;
+  float* syn0;
+  float* syn1;
+  float alpha;
+  float wneg;
+  int window;
+  int kd;
+  int exact;
+  int deterministic;
+  int all;  // take every sentence (else only those of this pool size)
+};
+
+template <typename E, int DIM, int P>
+struct alignas(16) SentLds {
+  static constexpr int LD = DIM + 16 / (int)sizeof(E);
+  E xc[P * LD];             // pool rows as used at this position
+  E yb[kOutCols * LD];      // output rows (target, negatives), unused rows zero
+  float S[P * 17];          // S[u][n]
+  float Gx[P * (P + 1)];    // Gx[u][v]
+  float Gy[kOutCols * 17];  // Gy[n][n']
+  float Gp[P * 20];         // Gp[u][n], rows 16-byte aligned
+  float H[P * (P + 4)];     // H[u][v] = sum over context pairs j < i (v_i = u, v_j = v) of g_i . g_j
+  float Mm[kOutCols * 20];  // M[n][n'] = sum over pairs j < i of one vertex of g_i[n] g_j[n']
+  float Gs[kMaxSlots * kOutCols];  // g_i[n] by context slot
+  int su[kMaxSlots];        // pool index of context slot i
+  uint8_t p2p[kSpan];       // position - lo -> pool index
+};
+
+// P = 16: sentences of at most 16 distinct vertices (one segment each);
+// P = 32: every other sentence, in segments whose windows hold <= 32.
+template <typename E, int DIM, int P>
+__global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
+  typedef Chunk<E> C;
+  typedef SentLds<E, DIM, P> Lds;
+  constexpr int LD = Lds::LD;
+  constexpr int NT = DIM / 16;   // dim tiles
+  constexpr int NPT = P / 16;    // pool tiles
+  constexpr int YCH = kOutCols * DIM / 4 / 64;  // float4 chunks per lane for 16 output rows
+  __shared__ Lds lds_all[kSentWaves];
+  __shared__ float s_expt[kExpTable + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l16 = lane & 15, g4 = lane >> 4;
+  Lds& L = lds_all[wv];
+  for (int i = threadIdx.x; i <= kExpTable; i += blockDim.x) s_expt[i] = t.expt[i];
+  for (int i = lane; i < kOutCols * LD; i += 64) L.yb[i] = (E)0;
+  __syncthreads();
+  const int W = t.window, K = t.kd;
+  const uint64_t pos_base = t.sen_off[t.s_begin];
+  uint64_t wave_id = (uint64_t)blockIdx.x * kSentWaves + wv;
+  uint64_t nwaves = (uint64_t)gridDim.x * kSentWaves;
+  if (t.deterministic) {
+    nwaves = 1;
+    if (wave_id != 0) wave_id = t.s_end;  // one wavefront, sentences in order
+  }
+  for (uint64_t s = t.s_begin + wave_id; s < t.s_end; s += nwaves) {
+    const int uc = t.ucount[s - t.s_begin];
+    if (!t.all && (P == 16) != (uc <= 16)) continue;
+    const uint64_t sb = t.sen_off[s];
+    const int Ls = (int)(t.sen_off[s + 1] - sb);
+    if (Ls < 2) continue;
+    const int32_t* tok = t.sen + sb;
+    const uint8_t* shr = t.shrink + (sb - pos_base);
+    const int32_t* ngs = t.negs + (sb - pos_base) * (uint64_t)K;
+    int q0 = 0;
+    while (q0 < Ls) {
+      // ---- pool of the segment [q0, q1): the vertices of positions
+      // lo .. q1-1+W; lane u < np holds pool vertex u
+      if (t.deterministic) __threadfence();
+      const int lo = q0 - W > 0 ? q0 - W : 0;
+      const int tk = lo + lane < Ls ? tok[lo + lane] : -1;  // tokens lo .. lo+63
+      // positions lo, lo+1, ... join the pool in order until it is full; the
+      // segment is every position whose whole window has joined
+      int my_id = -1, np = 0, c_last = lo - 1;
+      for (int c = lo; c < Ls && c - lo < kSpan; ++c) {
+        const int v = __shfl(tk, c - lo, 64);
+        const uint64_t m = __ballot(lane < np && my_id == v);
+        int idx;
+        if (m) {
+          idx = __ffsll((unsigned long long)m) - 1;
+        } else {
+          if (np == P) break;
+          idx = np;
+          if (lane == np) my_id = v;
+          ++np;
+        }
+        if (lane == 0) L.p2p[c - lo] = (uint8_t)idx;
+        c_last = c;
+      }
+      // 2 W + 1 <= P and 2 W < kSpan: the window of q0 always joins
+      const int q1 = c_last == Ls - 1 ? Ls : c_last - W + 1;
+      // ---- pool rows: lane (u = p*16 + l16, g4) holds dims 16 t + 4 g4 + e
+      typename C::v_t ox[NPT][NT];   // as loaded, rounded to E
+      f32x4_t dx[NPT][NT];           // FP32 deltas
+#pragma unroll
+      for (int p = 0; p < NPT; ++p) {
+        const int u = p * 16 + l16;
+        const int row = __shfl(my_id, u, 64);
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (u < np) v = *reinterpret_cast<const float4*>(t.syn0 + (uint64_t)row * DIM + 16 * tt + 4 * g4);
+          ox[p][tt] = C::from(v.x, v.y, v.z, v.w);
+          dx[p][tt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+      // ---- positions of the segment
+      for (int q = q0; q < q1; ++q) {
+        if (t.deterministic) __threadfence();
+        // output rows: lane n <= K holds row n (target, then negatives)
+        const int target = __shfl(tk, q - lo, 64);
+        int orow = -1;
+        if (lane == 0) orow = target;
+        else if (lane <= K) orow = ngs[(uint64_t)q * K + lane - 1];
+        const int b = shr[q];
+        // stage the output rows (rows > K stay zero): all loads issued first
+        {
+          float4 yv[YCH];
+#pragma unroll
+          for (int r = 0; r < YCH; ++r) {
+            const int ch = r * 64 + lane;
+            const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
+            const int row = __shfl(orow, n, 64);
+            yv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (n <= K) yv[r] = *reinterpret_cast<const float4*>(t.syn1 + (uint64_t)row * DIM + 4 * c4);
+          }
+#pragma unroll
+          for (int r = 0; r < YCH; ++r) {
+            const int ch = r * 64 + lane;
+            const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
+            if (n <= K) C::put4(L.yb + n * LD + 4 * c4, yv[r].x, yv[r].y, yv[r].z, yv[r].w);
+          }
+        }
+        // pool rows as they stand, for the dY product
+#pragma unroll
+        for (int p = 0; p < NPT; ++p)
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) {
+            float o[4];
+            if constexpr (sizeof(E) == 4) {
+              o[0] = ox[p][tt].x; o[1] = ox[p][tt].y; o[2] = ox[p][tt].z; o[3] = ox[p][tt].w;
+            } else {
+              typename Chunk<uint16_t>::U u;
+              u.v = ox[p][tt];
+              o[0] = bf2f(u.h[0]); o[1] = bf2f(u.h[1]); o[2] = bf2f(u.h[2]); o[3] = bf2f(u.h[3]);
+            }
+            C::put4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4, o[0] + dx[p][tt][0],
+                    o[1] + dx[p][tt][1], o[2] + dx[p][tt][2], o[3] + dx[p][tt][3]);
+          }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // S^T = Y X^T, Gx = X X^T, Gy = Y Y^T; chunk operands read back from LDS
+        {
+          f32x4_t gy = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) {
+            const typename C::v_t yv = C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4);
+            gy = C::mma(yv, yv, gy);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) L.Gy[(4 * g4 + e) * 17 + l16] = gy[e];
+        }
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+          f32x4_t st = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt)
+            st = C::mma(C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4),
+                        C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4), st);
+          // st[e] = S^T[n = 4 g4 + e][u = p*16 + l16]
+#pragma unroll
+          for (int e = 0; e < 4; ++e) L.S[(p * 16 + l16) * 17 + 4 * g4 + e] = st[e];
+#pragma unroll
+          for (int p2 = 0; p2 < NPT; ++p2) {
+            f32x4_t gx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt)
+              gx = C::mma(C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4),
+                          C::load4(L.xc + (p2 * 16 + l16) * LD + 16 * tt + 4 * g4), gx);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) L.Gx[(p * 16 + 4 * g4 + e) * (P + 1) + p2 * 16 + l16] = gx[e];
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // ---- the recursion, one lane per output column
+        {
+          const int n = lane;
+          const int row0 = __shfl(orow, 0, 64);
+          const bool live = n <= K && (n == 0 || orow != row0);
+          const float lab = n == 0 ? 1.0f : 0.0f;
+          const float wt = n == 0 ? t.alpha : t.alpha * t.wneg;
+          float gyr[kOutCols];
+#pragma unroll
+          for (int k = 0; k < kOutCols; ++k) gyr[k] = n < kOutCols ? L.Gy[k * 17 + (n & 15)] : 0.0f;
+          float corr[P], gp[P], mrow[kOutCols];
+#pragma unroll
+          for (int v = 0; v < P; ++v) {
+            corr[v] = 0.0f;
+            gp[v] = 0.0f;
+          }
+#pragma unroll
+          for (int k = 0; k < kOutCols; ++k) mrow[k] = 0.0f;
+          int slot = 0;
+          const int w = W - b;
+          const int c_lo = q - w > 0 ? q - w : 0;
+          const int c_hi = q + w < Ls - 1 ? q + w : Ls - 1;
+          for (int c = c_lo; c <= c_hi; ++c) {
+            if (c == q) continue;
+            const int u = L.p2p[c - lo];
+            float cu = 0.0f, gpu = 0.0f;
+#pragma unroll
+            for (int v = 0; v < P; ++v) {
+              cu = v == u ? corr[v] : cu;
+              gpu = v == u ? gp[v] : gpu;
+            }
+            float sv = (n < kOutCols ? L.S[u * 17 + (n & 15)] : 0.0f) + cu;
+            float gpk[kOutCols];  // g of the earlier occurrences of this vertex, per column
+#pragma unroll
+            for (int k = 0; k < kOutCols; ++k) {
+              gpk[k] = k <= K ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gpu), k)) : 0.0f;
+              sv += gpk[k] * gyr[k];
+            }
+            float g = 0.0f;
+            if (live) g = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
+#pragma unroll
+            for (int k = 0; k < kOutCols; ++k) mrow[k] += g * gpk[k];
+            if (n < kOutCols) L.Gs[slot * kOutCols + n] = g;
+            if (lane == 0) L.su[slot] = u;
+            ++slot;
+#pragma unroll
+            for (int v = 0; v < P; ++v) {
+              gp[v] += v == u ? g : 0.0f;
+              corr[v] += g * L.Gx[v * (P + 1) + u];
+            }
+          }
+          if (n < kOutCols) {
+#pragma unroll
+            for (int v = 0; v < P; ++v) L.Gp[v * 20 + n] = gp[v];
+#pragma unroll
+            for (int k = 0; k < kOutCols; ++k) L.Mm[n * 20 + k] = mrow[k];
+          }
+          for (int i = lane; i < P * (P + 4); i += 64) L.H[i] = 0.0f;
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          // H[v_i][v_j] += g_i . g_j for every pair of context slots j < i
+          for (int i = 1; i < slot; ++i) {
+            if (lane < i) {
+              float dot = 0.0f;
+#pragma unroll
+              for (int k = 0; k < kOutCols; ++k)
+                if (k <= K) dot += L.Gs[i * kOutCols + k] * L.Gs[lane * kOutCols + k];
+              atomicAdd(&L.H[L.su[i] * (P + 4) + L.su[lane]], dot);
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // ---- dX = Gp Y into the deltas: dX^T[d][u] = sum_n Y[n][d] Gp[u][n]
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+          const float4 gq = *reinterpret_cast<const float4*>(L.Gp + (p * 16 + l16) * 20 + 4 * g4);
+          const typename C::v_t bgp = C::from(gq.x, gq.y, gq.z, gq.w);
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) {
+            const E* yc = L.yb + (4 * g4) * LD + 16 * tt + l16;
+            const typename C::v_t ay =
+                C::from(C::elem(yc), C::elem(yc + LD), C::elem(yc + 2 * LD), C::elem(yc + 3 * LD));
+            dx[p][tt] = C::mma(ay, bgp, dx[p][tt]);
+          }
+          // + H X: dX^T[d][u] += sum_v X[v][d] H[u][v]
+#pragma unroll
+          for (int p2 = 0; p2 < NPT; ++p2) {
+            const float4 hq = *reinterpret_cast<const float4*>(L.H + (p * 16 + l16) * (P + 4) + p2 * 16 + 4 * g4);
+            const typename C::v_t bh = C::from(hq.x, hq.y, hq.z, hq.w);
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) {
+              const E* xq = L.xc + (p2 * 16 + 4 * g4) * LD + 16 * tt + l16;
+              const typename C::v_t ax =
+                  C::from(C::elem(xq), C::elem(xq + LD), C::elem(xq + 2 * LD), C::elem(xq + 3 * LD));
+              dx[p][tt] = C::mma(ax, bh, dx[p][tt]);
+            }
+          }
+        }
+        // ---- dY = Gp^T X: dY[n][d] = sum_u Gp[u][n] X[u][d] -> output rows
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+          f32x4_t dy = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int p = 0; p < NPT; ++p) {
+            const float* gc = L.Gp + (p * 16 + 4 * g4) * 20 + l16;
+            const typename C::v_t ag = C::from(gc[0], gc[20], gc[40], gc[60]);
+            const E* xcol = L.xc + (p * 16 + 4 * g4) * LD + 16 * tt + l16;
+            const typename C::v_t bx =
+                C::from(C::elem(xcol), C::elem(xcol + LD), C::elem(xcol + 2 * LD), C::elem(xcol + 3 * LD));
+            dy = C::mma(ag, bx, dy);
+          }
+          {
+            // + M Y: dY[n][d] += sum_n' M[n][n'] Y[n'][d]
+            const float4 mq = *reinterpret_cast<const float4*>(L.Mm + l16 * 20 + 4 * g4);
+            const E* yq = L.yb + (4 * g4) * LD + 16 * tt + l16;
+            dy = C::mma(C::from(mq.x, mq.y, mq.z, mq.w),
+                        C::from(C::elem(yq), C::elem(yq + LD), C::elem(yq + 2 * LD), C::elem(yq + 3 * LD)), dy);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int n = 4 * g4 + e;
+            const int row = __shfl(orow, n, 64);
+            const int row0 = __shfl(orow, 0, 64);
+            if (n <= K && (n == 0 || row != row0))
+              atomicAdd(&t.syn1[(uint64_t)row * DIM + 16 * tt + l16], dy[e]);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      // ---- the segment's deltas into the input rows
+#pragma unroll
+      for (int p = 0; p < NPT; ++p) {
+        const int u = p * 16 + l16;
+        const int row = __shfl(my_id, u, 64);
+        if (u < np) {
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              atomicAdd(&t.syn0[(uint64_t)row * DIM + 16 * tt + 4 * g4 + e], dx[p][tt][e]);
+        }
+      }
+      q0 = q1;
+    }
+  }
 }
 
 // ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass -----------
@@ -767,12 +878,12 @@ struct dg_sgns_state {
   size_t sen_off_cap = 0;
   uint32_t* d_cnt = nullptr;
   size_t cnt_cap = 0;
-  uint2* d_tiles = nullptr;
-  size_t tiles_cap = 0;
-  int32_t* d_desc = nullptr;  // tile descriptors of one launch
-  size_t desc_cap = 0;
-  uint64_t* d_tile_off = nullptr;
-  size_t tile_off_cap = 0;
+  uint8_t* d_shrink = nullptr;  // batched: window shrink per kept position of a launch
+  size_t shrink_cap = 0;
+  int32_t* d_negs = nullptr;    // batched: negatives per kept position of a launch
+  size_t negs_cap = 0;
+  uint8_t* d_ucount = nullptr;  // batched: distinct vertices per sentence of a launch
+  size_t ucount_cap = 0;
   // sorted scatter-add
   float* d_stage = nullptr;
   size_t stage_cap = 0;
@@ -803,7 +914,7 @@ void dg_sgns_free(dg_ctx* ctx) {
   if (!s) return;
   void* bufs[] = {s->d_rank,   s->d_syn0,  s->d_syn1,   s->d_g0,
                   s->d_g1,     s->d_table, s->d_expt,   s->d_keep,    s->d_sen,   s->d_sen_off,
-                  s->d_cnt,    s->d_tiles, s->d_desc,  s->d_tile_off, s->d_stage, s->d_k[0],  s->d_k[1],
+                  s->d_cnt,    s->d_shrink, s->d_negs, s->d_ucount, s->d_stage, s->d_k[0],  s->d_k[1],
                   s->d_v[0],   s->d_v[1],  s->d_hist,   s->d_hoff,    s->d_head,  s->d_hpos,
                   s->d_starts, s->d_tuples, s->d_logits};
   for (void* b : bufs)
@@ -831,8 +942,7 @@ extern "C" void dg_sgns_params_default(dg_sgns_params* p) {
   p->minibatch = 0;
   p->table_size = 100000000u;
   p->init_float_ops = 0;
-  p->tile_positions = 0;
-  p->tile_negatives = 0;
+  p->negative_draws = 0;
   p->seed = 1;
 }
 
@@ -847,18 +957,16 @@ static int check_params(dg_ctx* ctx, const dg_sgns_params* p) {
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "negative %u: at most %d", p->negative, kMaxNeg);
   if (p->table_size == 0) return dg_fail(ctx, DG_ERR_ARG, "table_size must be positive");
   if (p->form > DG_SGNS_BATCHED_BF16) return dg_fail(ctx, DG_ERR_ARG, "unknown form %u", p->form);
-  if (p->tile_positions != 0 && p->tile_positions != 8 && p->tile_positions != 16)
-    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "tile_positions %u: 8 or 16", p->tile_positions);
-  if (p->tile_negatives != 0 && p->tile_negatives < p->negative)
-    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "tile_negatives %u: at least negative (%u)",
-                   p->tile_negatives, p->negative);
+  if (p->negative_draws != 0 && p->negative_draws < p->negative)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "negative_draws %u: at least negative (%u)",
+                   p->negative_draws, p->negative);
   if (p->form != DG_SGNS_PLAIN_FP32) {
-    const uint32_t kd = p->tile_negatives ? p->tile_negatives : p->negative;
-    const uint32_t T = p->tile_positions ? p->tile_positions : 8u;
-    if (T * (1 + kd) > 96u || T + 2 * p->window > 64u)
-      return dg_fail(ctx, DG_ERR_UNSUPPORTED,
-                     "tile of %u positions: needs %u output rows (max 96) and %u context rows (max 64)",
-                     T, T * (1 + kd), T + 2 * p->window);
+    const uint32_t kd = p->negative_draws ? p->negative_draws : p->negative;
+    if (kd + 1 > (uint32_t)kOutCols)
+      return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched forms: at most %d negatives per position",
+                     kOutCols - 1);
+    if (p->window > (uint32_t)kMaxSentWindow)
+      return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched forms: window at most %d", kMaxSentWindow);
   }
   if (p->form == DG_SGNS_BATCHED_FP32 && p->dim > 128)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched FP32 form supports dim <= 128");
@@ -1082,67 +1190,47 @@ static void launch_plain(dg_ctx* ctx, const SgnsDev& a, uint64_t sb, uint64_t se
   }
 }
 
-// Tile shape: T positions per tile (tile_positions, else 8), K negatives
-// drawn per position; staged context rows CTX = T + 2 window rounded up to 32
-// or 64; output columns NOUT = T (1 + K) rounded up to 48 or 96.
-struct TileShape {
-  int T;
-  int ctx;
-  int nout;
-};
-
-static TileShape tile_shape(int form, uint32_t window, uint32_t kd, uint32_t tile) {
-  TileShape ts;
-  ts.T = tile ? (int)tile : 8;
-  ts.ctx = ts.T + 2 * (int)window <= 32 ? 32 : 64;
-  (void)form;
-  ts.nout = ts.T * (1 + (int)kd) <= 48 ? 48 : 96;
-  return ts;
-}
-
-static uint32_t desc_stride(const TileShape& ts) {
-  return (uint32_t)((kDescHead + ts.T + ts.ctx + ts.nout + 3) & ~3);
-}
-
-// Blocks resident per CU for a kernel (queried once), times the CUs.
+// Blocks resident per CU for a kernel, times the CUs.
 template <typename Kern>
-static unsigned resident_grid(dg_ctx* ctx, Kern k, uint64_t n_tiles) {
+static unsigned resident_grid(dg_ctx* ctx, Kern k, int block) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
-  uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
-  if (g > n_tiles) g = n_tiles;
-  return (unsigned)(g ? g : 1);
+  return (unsigned)per_cu * (unsigned)ctx->num_cus;
 }
 
-template <typename E, int DIM, int T, int CTX, int NOUT>
-static void launch_tile(dg_ctx* ctx, const TileArgs& t) {
-  auto k = sg_tile_kernel<E, DIM, T, CTX, NOUT>;
-  k<<<resident_grid(ctx, k, t.n_tiles), 256, 0, ctx->stream>>>(t);
+template <typename E, int DIM, int P>
+static void launch_sent(dg_ctx* ctx, const SentArgs& t) {
+  auto k = sg_sent_kernel<E, DIM, P>;
+  uint64_t g = t.deterministic ? 1 : resident_grid(ctx, k, 64 * kSentWaves);
+  const uint64_t need = (t.s_end - t.s_begin + kSentWaves - 1) / kSentWaves;
+  if (g > need) g = need;
+  k<<<(unsigned)(g ? g : 1), 64 * kSentWaves, 0, ctx->stream>>>(t);
 }
 
+// Hogwild: sentences of <= 16 distinct vertices on the 16-row pool, the
+// others on the 32-row pool.  Deterministic: one wavefront, every sentence
+// in order, on the 32-row pool.
 template <typename E, int DIM>
-static void launch_tile_shape(dg_ctx* ctx, const TileArgs& t, const TileShape& ts) {
-  constexpr int S = 48;
-  if (ts.T == 16) {
-    launch_tile<E, DIM, 16, 64, 96>(ctx, t);
-  } else if (ts.ctx == 32) {
-    if (ts.nout == S) launch_tile<E, DIM, 8, 32, S>(ctx, t);
-    else launch_tile<E, DIM, 8, 32, 96>(ctx, t);
+static void launch_sent_dim(dg_ctx* ctx, SentArgs t) {
+  if (t.deterministic) {
+    t.all = 1;
+    launch_sent<E, DIM, 32>(ctx, t);
   } else {
-    if (ts.nout == S) launch_tile<E, DIM, 8, 64, S>(ctx, t);
-    else launch_tile<E, DIM, 8, 64, 96>(ctx, t);
+    t.all = 0;
+    launch_sent<E, DIM, 16>(ctx, t);
+    launch_sent<E, DIM, 32>(ctx, t);
   }
 }
 
 template <typename E>
-static void launch_tile_dim(dg_ctx* ctx, const TileArgs& t, uint32_t dim, const TileShape& ts) {
+static void launch_sent_form(dg_ctx* ctx, const SentArgs& t, uint32_t dim) {
   switch (dim) {
-    case 64: launch_tile_shape<E, 64>(ctx, t, ts); break;
-    case 128: launch_tile_shape<E, 128>(ctx, t, ts); break;
+    case 64: launch_sent_dim<E, 64>(ctx, t); break;
+    case 128: launch_sent_dim<E, 128>(ctx, t); break;
     case 256:
-      // FP32 rows of dim 256 do not fit the 160 KiB LDS; rejected by check_params
-      if constexpr (sizeof(E) == 2) launch_tile_shape<E, 256>(ctx, t, ts);
+      // FP32 pools of dim 256 exceed the register file; rejected by check_params
+      if constexpr (sizeof(E) == 2) launch_sent_dim<E, 256>(ctx, t);
       break;
   }
 }
@@ -1236,101 +1324,58 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
         sb = se;
       }
     } else {
-      // tile list
-      const uint32_t kd = p.tile_negatives ? p.tile_negatives : p.negative;
-      const TileShape ts = tile_shape((int)p.form, p.window, kd, p.tile_positions);
-      tile_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
-          s->d_sen_off, n_walks, (uint32_t)ts.T, s->d_cnt);
-      DG_CHECK(dg_reserve(ctx, &s->d_tile_off, &s->tile_off_cap, (size_t)n_walks + 1));
-      DG_CHECK(dg_scan_lengths(ctx, s->d_cnt, n_walks, s->d_tile_off));
-      std::vector<uint64_t> h_toff(n_walks + 1);
-      DG_HIP(ctx, hipMemcpy(h_toff.data(), s->d_tile_off, sizeof(uint64_t) * (n_walks + 1),
-                            hipMemcpyDeviceToHost));
-      const uint64_t n_tiles = h_toff[n_walks];
-      DG_CHECK(dg_reserve(ctx, &s->d_tiles, &s->tiles_cap, (size_t)n_tiles + 1));
-      tile_list_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(s->d_cnt, s->d_tile_off,
-                                                                             n_walks, s->d_tiles);
-      DG_HIP(ctx, hipGetLastError());
-      const uint64_t mb = p.minibatch ? p.minibatch : (p.deterministic ? 8192 : 262144);
-      const uint32_t stride = desc_stride(ts);
-      const uint64_t mb_cap = n_tiles < mb ? n_tiles : mb;
-      DG_CHECK(dg_reserve(ctx, &s->d_desc, &s->desc_cap, (size_t)(mb_cap ? mb_cap : 1) * stride));
-      DescArgs da;
-      da.sen = s->d_sen;
-      da.sen_off = s->d_sen_off;
-      da.table = s->d_table;
-      da.first_sentence = s->sentences_done;
-      da.seed = p.seed;
-      da.epoch = epoch;
-      da.table_size = s->table_size;
-      da.vocab = s->vocab;
-      da.window = (int)p.window;
-      da.kd = (int)kd;
-      da.T = ts.T;
-      da.ctx = ts.ctx;
-      da.nout = ts.nout;
-      da.stride = stride;
-      da.desc = s->d_desc;
-      TileArgs t;
-      t.a = a;
-      t.desc = s->d_desc;
-      t.desc_stride = stride;
-      t.kd = (int)kd;
-      t.wneg = (float)p.negative / (float)kd;
-      t.exact = (int)p.exact_sigmoid;
-      t.deterministic = (int)p.deterministic;
-      t.row_bits = (uint32_t)row_bits_for(s->vocab);
-      const uint64_t slots_per_tile = (uint64_t)(ts.ctx + ts.nout);
-      if (p.deterministic) {
-        const size_t slots = (size_t)mb_cap * slots_per_tile;
-        DG_CHECK(dg_reserve(ctx, &s->d_stage, &s->stage_cap, slots * s->dim));
-        if (s->kv_cap < slots) {
-          for (int i = 0; i < 2; ++i) {
-            size_t c = 0;
-            if (s->d_k[i]) hipFree(s->d_k[i]);
-            if (s->d_v[i]) hipFree(s->d_v[i]);
-            s->d_k[i] = nullptr;
-            s->d_v[i] = nullptr;
-            DG_CHECK(dg_reserve(ctx, &s->d_k[i], &c, slots));
-            c = 0;
-            DG_CHECK(dg_reserve(ctx, &s->d_v[i], &c, slots));
-          }
-          s->kv_cap = slots;
-        }
-        t.stage = s->d_stage;
-        t.keys = s->d_k[0];
-        t.vals = s->d_v[0];
-      } else {
-        t.stage = nullptr;
-        t.keys = nullptr;
-        t.vals = nullptr;
-      }
-      for (uint64_t t0 = 0; t0 < n_tiles; t0 += mb) {
-        const uint64_t nt = n_tiles - t0 < mb ? n_tiles - t0 : mb;
-        da.tiles = s->d_tiles + t0;
-        da.n_tiles = nt;
-        t.n_tiles = nt;
-        // learning rate from the first sentence of the launch
-        uint64_t lo = 0, hi = n_walks;
-        while (lo < hi) {  // first sentence whose tiles start at or after t0
-          const uint64_t mid = (lo + hi) / 2;
-          if (h_toff[mid + 1] <= t0) lo = mid + 1; else hi = mid;
-        }
-        t.a.alpha = alpha = alpha_at(lo);
+      const uint32_t kd = p.negative_draws ? p.negative_draws : p.negative;
+      const uint64_t mb = p.minibatch ? p.minibatch : 131072;
+      for (uint64_t s0 = 0; s0 < n_walks; s0 += mb) {
+        const uint64_t s1 = n_walks - s0 < mb ? n_walks : s0 + mb;
+        const uint64_t n_pos = h_soff[s1] - h_soff[s0];
+        DG_CHECK(dg_reserve(ctx, &s->d_shrink, &s->shrink_cap, (size_t)n_pos + 1));
+        DG_CHECK(dg_reserve(ctx, &s->d_negs, &s->negs_cap, (size_t)n_pos * kd + 1));
+        DG_CHECK(dg_reserve(ctx, &s->d_ucount, &s->ucount_cap, (size_t)(s1 - s0) + 1));
+        DrawArgs da;
+        da.sen = s->d_sen;
+        da.sen_off = s->d_sen_off;
+        da.s_begin = s0;
+        da.s_end = s1;
+        da.first_sentence = s->sentences_done;
+        da.seed = p.seed;
+        da.epoch = epoch;
+        da.table = s->d_table;
+        da.table_size = s->table_size;
+        da.vocab = s->vocab;
+        da.window = p.window;
+        da.kd = (int)kd;
+        da.shrink = s->d_shrink;
+        da.negs = s->d_negs;
+        da.ucount = s->d_ucount;
+        SentArgs t;
+        t.sen = s->d_sen;
+        t.sen_off = s->d_sen_off;
+        t.s_begin = s0;
+        t.s_end = s1;
+        t.shrink = s->d_shrink;
+        t.negs = s->d_negs;
+        t.ucount = s->d_ucount;
+        t.expt = s->d_expt;
+        t.syn0 = s->d_syn0;
+        t.syn1 = s->d_syn1;
+        t.alpha = alpha = alpha_at(s0);
+        t.wneg = (float)p.negative / (float)kd;
+        t.window = (int)p.window;
+        t.kd = (int)kd;
+        t.exact = (int)p.exact_sigmoid;
+        t.deterministic = (int)p.deterministic;
+        t.all = 0;
         size_t tk;
-        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_tile", &tk));
-        {
-          uint64_t g = (nt + 3) / 4;
-          const uint64_t cap = (uint64_t)ctx->num_cus * 16;
-          if (g > cap) g = cap;
-          tile_desc_kernel<<<(unsigned)(g ? g : 1), 256, 0, ctx->stream>>>(da);
-        }
+        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_fp32b", &tk));
+        if (n_pos)
+          sent_draw_kernel<<<grid_for(ctx, n_pos, 256), 256, 0, ctx->stream>>>(da);
+        sent_count_kernel<<<grid_for(ctx, s1 - s0, 256), 256, 0, ctx->stream>>>(da);
         if (p.form == DG_SGNS_BATCHED_BF16)
-          launch_tile_dim<uint16_t>(ctx, t, s->dim, ts);
+          launch_sent_form<uint16_t>(ctx, t, s->dim);
         else
-          launch_tile_dim<float>(ctx, t, s->dim, ts);
+          launch_sent_form<float>(ctx, t, s->dim);
         DG_HIP(ctx, hipGetLastError());
-        if (p.deterministic) DG_CHECK(sorted_scatter_apply(ctx, s, nt * slots_per_tile, 0, 0.0f));
         DG_CHECK(dg_time_end(ctx, tk));
         st.launches++;
       }

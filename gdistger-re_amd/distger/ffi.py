@@ -96,8 +96,7 @@ class SgnsParams(ctypes.Structure):
         ("minibatch", ctypes.c_uint32),
         ("table_size", ctypes.c_uint32),
         ("init_float_ops", ctypes.c_uint32),
-        ("tile_positions", ctypes.c_uint32),
-        ("tile_negatives", ctypes.c_uint32),
+        ("negative_draws", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
     ]
 

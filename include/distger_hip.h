@@ -177,8 +177,8 @@ int dg_train_corpus_clear(dg_ctx* ctx);
 /* ---- skip-gram negative-sampling trainer (S1-S9) ------------------------ */
 
 #define DG_SGNS_PLAIN_FP32 0   /* skip_gram_kernel semantics (word2vec.cu:305-399), FP32 */
-#define DG_SGNS_BATCHED_FP32 1 /* shared-negative position tiles, FP32 rows */
-#define DG_SGNS_BATCHED_BF16 2 /* same on BF16 rows and BF16 MFMA, FP32 master rows */
+#define DG_SGNS_BATCHED_FP32 1 /* per-position shared negatives, sentence pools on FP32 MFMA */
+#define DG_SGNS_BATCHED_BF16 2 /* same with rows rounded to BF16 for BF16 MFMA, FP32 rows in HBM */
 
 #define DG_VOCAB_DEGREE 0 /* word counts = in + out degree (word2vec.cu:791-809, huge_walk.cpp:43-46) */
 #define DG_VOCAB_VISITS 1 /* word counts = walk visit counts (dsgl.hpp:314-382) */
@@ -198,18 +198,16 @@ typedef struct {
   uint32_t form;           /* DG_SGNS_* */
   uint32_t vocab_source;   /* DG_VOCAB_* */
   uint32_t alpha_schedule; /* DG_ALPHA_* */
-  uint32_t deterministic;  /* plain: sentences one after another in one wavefront;
-                              batched: sorted-key scatter-add per minibatch instead of
-                              concurrent atomic updates */
+  uint32_t deterministic;  /* 1: sentences one after another in one wavefront (a fixed
+                              order of every update); 0: concurrent sentences with atomic
+                              row updates (Hogwild, as the reference kernels) */
   uint32_t exact_sigmoid;  /* 0: the reference's 1000-entry sigmoid table; 1: 1/(1+e^-f) */
-  uint32_t minibatch;      /* batched: position tiles per launch (0 = tuned default) */
+  uint32_t minibatch;      /* batched: sentences per launch (0 = tuned default) */
   uint32_t table_size;     /* unigram table entries (1e8 in the reference) */
   uint32_t init_float_ops; /* 0: word2vec.cu init rounding (double), 1: dsgl.hpp (float) */
-  uint32_t tile_positions; /* batched: consecutive positions per tile, 4, 8 or 16 with
-                              tile_positions * (1 + tile_negatives) <= 96 (0 = tuned default) */
-  uint32_t tile_negatives; /* batched: negatives drawn per position, >= negative (0 = negative);
-                              each one's gradient is scaled by negative / tile_negatives, so
-                              the expected update is that of `negative` draws per pair */
+  uint32_t negative_draws; /* batched: negatives drawn per position, >= negative (0 =
+                              negative); each one's gradient is scaled by negative /
+                              negative_draws, so the expected update is that of `negative` */
   uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
 } dg_sgns_params;
 

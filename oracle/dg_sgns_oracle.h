@@ -26,13 +26,22 @@ void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_
                         uint32_t dim, float* syn0, float* syn1);
 uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,
                            uint64_t cap);
-uint32_t dgo_tile_positions(uint32_t draws, uint32_t tile);
-uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen, uint32_t tile);
-void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
-                    uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
-                    const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
-                    const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
-                    uint32_t dim, uint32_t tile, uint32_t draws, float* syn0, float* syn1);
+/* Philox draws of one position: window shrink b and `draws` negatives. */
+void dgo_position_draws(uint64_t seed, uint64_t gs, uint32_t q, uint32_t epoch, uint32_t window,
+                        uint32_t draws, const int32_t* table, uint32_t table_size, uint32_t vocab,
+                        uint32_t* b, int32_t* negs);
+/* Batched form: sequential SGNS with per-position shared negatives, FP64. */
+void dgo_sg_shared(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen, uint64_t n_sen,
+                   uint32_t window, uint32_t negative, uint32_t draws, const int32_t* table,
+                   uint32_t table_size, uint32_t vocab, float alpha, const float* expt,
+                   int exact_sigmoid, uint64_t seed, uint32_t epoch, uint32_t dim, float* syn0,
+                   float* syn1);
+/* The batched forms' evaluation of that order (first order in the updates). */
+void dgo_sg_shared_first_order(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
+                               uint64_t n_sen, uint32_t window, uint32_t negative, uint32_t draws,
+                               const int32_t* table, uint32_t table_size, uint32_t vocab,
+                               float alpha, const float* expt, int exact_sigmoid, uint64_t seed,
+                               uint32_t epoch, uint32_t dim, float* syn0, float* syn1);
 double dgo_tuple_loss(const uint32_t* tuples, uint64_t n, uint32_t K, const double* syn0,
                       const double* syn1, uint32_t dim);
 

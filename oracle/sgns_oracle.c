@@ -241,32 +241,19 @@ uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* ba
 }
 
 /* ------------------------------------------------------------------------
- * Batched form: tiles of T consecutive positions of a sentence; each
- * position draws its own K negatives, shared by the contexts of its window
- * (the per-position minibatch of dsgl.hpp:1100-1159).  Within a position the
- * contexts are taken in order and each (context, output) pair sees the
- * output row as updated by the earlier contexts of that position, as
- * skip_gram_kernel updates it (word2vec.cu:360-396); everything else of a
- * launch of `minibatch` tiles is taken from the tables as they were at its
- * start, and the row sums are added once.  Arithmetic in double, rows stored
- * as float: the FP64 reference the FP32 and BF16 device forms are measured
- * against.
+ * Batched form (shared negatives): skip_gram_kernel's order (word2vec.cu:
+ * 305-399) - sentences in order, positions in order, contexts of the shrunk
+ * window in order, the output row updated after every (context, output)
+ * pair and the context row after all outputs of its pair - except that each
+ * position draws its K negatives once, shared by all its contexts (the
+ * per-position negatives of dsgl.hpp:1100-1159).  K = `draws` (`negative`
+ * when 0), each negative's gradient weighted by wneg = negative / K (a float,
+ * 1 when K = negative).  Tables are held in double for the whole call and
+ * rounded to float at the end: the FP64 reference the device forms follow.
  *
- * K = `draws` negatives per position (`negative` when 0), each weighted by
- * wneg = negative / K (a float; 1 when K = negative).  T = `tile` when
- * non-zero, else 8.
- * Tile (s, j): positions p0 = T j .. p1 - 1 = min(p0 + T, L) - 1 of sentence
- * s (L >= 2).  Philox stream (ctr = (blk, gs, j | epoch << 16, SGNS)), gs =
- * first_sen + s: word q < np gives b_{p0+q} = word % window; word np + q K + k
- * gives position q's negative k: table[word % table_size] (0 -> word %
- * (vocab-1) + 1).  Contexts c in [max(0, p0-W), min(L, p1+W)).  Output
- * column n = q (K+1) + r of position p = p0 + q: r = 0 its target t_p (label
- * 1), r > 0 negative r-1 (label 0, skipped when equal to t_p).  For each
- * column, with y~ = y_n and the contexts c of win(p) in ascending order:
- *   g[c][n] = (label - sig(x_c . y~)) alpha (wneg for negatives);  y~ += g x_c
- *   dX[c] = sum_n g[c][n] y_n ;  dY[n] = sum_c g[c][n] x_c   (staged rows)
- * win(p) = {c != p : |c - p| <= W - b_p}.  Row contributions per tile in the
- * order contexts (ascending c), then columns (ascending n).
+ * Draws: Philox stream (ctr = (blk, first_sen + s, q | epoch << 20, SGNS))
+ * of position q of sentence s: word 0 gives b = word % window, word k
+ * (1..K) negative k: table[word % table_size], 0 -> word % (vocab-1) + 1.
  * ---------------------------------------------------------------------- */
 static double sig_d(double f, const float* expt, int exact) {
   if (exact) return 1.0 / (1.0 + exp(-f));
@@ -275,154 +262,205 @@ static double sig_d(double f, const float* expt, int exact) {
   return (double)expt[(int)((float)(f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))];
 }
 
-typedef struct {
-  uint32_t row;
-  uint32_t table; /* 0 = input (syn0), 1 = output (syn1) */
-  uint64_t order;
-  double* vec;
-} contrib_t;
-
-static int cmp_contrib(const void* a, const void* b) {
-  const contrib_t* x = (const contrib_t*)a;
-  const contrib_t* y = (const contrib_t*)b;
-  if (x->table != y->table) return x->table < y->table ? -1 : 1;
-  if (x->row != y->row) return x->row < y->row ? -1 : 1;
-  return (x->order > y->order) - (x->order < y->order);
-}
-
-uint32_t dgo_tile_positions(uint32_t draws, uint32_t tile) {
-  (void)draws;
-  return tile ? tile : 8;
-}
-
-uint64_t dgo_tile_count(const uint64_t* sen_off, uint64_t n_sen, uint32_t tile) {
-  uint64_t n = 0, T = tile;
-  for (uint64_t s = 0; s < n_sen; ++s) {
-    uint64_t L = sen_off[s + 1] - sen_off[s];
-    if (L >= 2) n += (L + T - 1) / T;
+void dgo_position_draws(uint64_t seed, uint64_t gs, uint32_t q, uint32_t epoch, uint32_t window,
+                        uint32_t draws, const int32_t* table, uint32_t table_size, uint32_t vocab,
+                        uint32_t* b, int32_t* negs) {
+  dg_stream st;
+  dg_stream_init(&st, seed, (uint32_t)gs, q | (epoch << 20), DG_TAG_SGNS);
+  *b = dg_stream_next(&st) % window;
+  for (uint32_t k = 0; k < draws; ++k) {
+    uint32_t x = dg_stream_next(&st);
+    int32_t n = table[x % table_size];
+    if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
+    negs[k] = n;
   }
-  return n;
 }
 
-void dgo_sg_batched(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
-                    uint64_t n_sen, uint64_t minibatch, uint32_t window, uint32_t negative,
-                    const int32_t* table, uint32_t table_size, uint32_t vocab, float alpha,
-                    const float* expt, int exact_sigmoid, uint64_t seed, uint32_t epoch,
-                    uint32_t dim, uint32_t tile, uint32_t draws, float* syn0, float* syn1) {
-  const int K = (int)(draws ? draws : negative), W = (int)window, K1 = K + 1;
+void dgo_sg_shared(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen, uint64_t n_sen,
+                   uint32_t window, uint32_t negative, uint32_t draws, const int32_t* table,
+                   uint32_t table_size, uint32_t vocab, float alpha, const float* expt,
+                   int exact_sigmoid, uint64_t seed, uint32_t epoch, uint32_t dim, float* syn0,
+                   float* syn1) {
+  const uint32_t K = draws ? draws : negative;
   const float wneg = (float)negative / (float)K;
-  const int T = (int)dgo_tile_positions((uint32_t)K, tile);
-  uint64_t n_tiles = dgo_tile_count(sen_off, n_sen, (uint32_t)T);
-  uint64_t* ts = (uint64_t*)malloc(sizeof(uint64_t) * (n_tiles ? n_tiles : 1));
-  uint32_t* tj = (uint32_t*)malloc(sizeof(uint32_t) * (n_tiles ? n_tiles : 1));
-  uint64_t nt = 0;
+  uint64_t n_rows = 0;
+  for (uint64_t i = 0; i < sen_off[n_sen]; ++i)
+    if ((uint64_t)sen[i] + 1 > n_rows) n_rows = (uint64_t)sen[i] + 1;
+  for (uint32_t i = 0; i < table_size; ++i)
+    if ((uint64_t)table[i] + 1 > n_rows) n_rows = (uint64_t)table[i] + 1;
+  if (vocab > n_rows) n_rows = vocab;
+  double* x0 = (double*)malloc(sizeof(double) * n_rows * dim);
+  double* y1 = (double*)malloc(sizeof(double) * n_rows * dim);
+  for (uint64_t i = 0; i < n_rows * dim; ++i) {
+    x0[i] = (double)syn0[i];
+    y1[i] = (double)syn1[i];
+  }
+  int32_t* negs = (int32_t*)malloc(sizeof(int32_t) * (K ? K : 1));
+  double* neu1e = (double*)malloc(sizeof(double) * dim);
   for (uint64_t s = 0; s < n_sen; ++s) {
-    uint64_t L = sen_off[s + 1] - sen_off[s];
-    if (L < 2) continue;
-    for (uint32_t j = 0; (uint64_t)j * T < L; ++j) {
-      ts[nt] = s;
-      tj[nt] = j;
-      ++nt;
-    }
-  }
-  int32_t* out = (int32_t*)malloc(sizeof(int32_t) * (size_t)T * K1);
-  int* b = (int*)malloc(sizeof(int) * (size_t)T);
-  double* ytil = (double*)malloc(sizeof(double) * dim);
-  for (uint64_t t0 = 0; t0 < n_tiles; t0 += minibatch) {
-    uint64_t t1 = t0 + minibatch < n_tiles ? t0 + minibatch : n_tiles;
-    uint64_t cap = (t1 - t0) * ((uint64_t)T * K1 + (uint64_t)T + 2 * (uint64_t)W);
-    contrib_t* cs = (contrib_t*)malloc(sizeof(contrib_t) * (cap ? cap : 1));
-    double* vecs = (double*)calloc(cap ? cap * dim : 1, sizeof(double));
-    uint64_t ns = 0;
-    for (uint64_t ti = t0; ti < t1; ++ti) {
-      uint64_t s = ts[ti];
-      const int32_t* t = sen + sen_off[s];
-      int L = (int)(sen_off[s + 1] - sen_off[s]);
-      int p0 = (int)tj[ti] * T, p1 = p0 + T < L ? p0 + T : L;
-      int np = p1 - p0;
-      dg_stream st;
-      dg_stream_init(&st, seed, (uint32_t)(first_sen + s), tj[ti] | (epoch << 16), DG_TAG_SGNS);
-      for (int q = 0; q < np; ++q) b[q] = (int)(dg_stream_next(&st) % window);
-      for (int q = 0; q < np; ++q) out[q * K1] = t[p0 + q];
-      for (int q = 0; q < np; ++q)
-        for (int k = 0; k < K; ++k) {
-          uint32_t x = dg_stream_next(&st);
-          int32_t n = table[x % table_size];
-          if (n == 0) n = (int32_t)(x % (vocab - 1) + 1);
-          out[q * K1 + 1 + k] = n;
-        }
-      int c_lo = p0 - W > 0 ? p0 - W : 0, c_hi = p1 + W < L ? p1 + W : L;
-      int nc = c_hi - c_lo;
-      int no = np * K1;
-      double* g = (double*)calloc((size_t)nc * no + 1, sizeof(double)); /* [c][n] */
-      for (int n = 0; n < no; ++n) {
-        int q = n / K1, r = n % K1, p = p0 + q, w = W - b[q];
-        int32_t row = out[n];
-        if (r > 0 && row == t[p]) continue;
-        double lab = r == 0 ? 1.0 : 0.0;
-        for (uint32_t d = 0; d < dim; ++d) ytil[d] = (double)syn1[(uint64_t)row * dim + d];
-        for (int c = p - w > c_lo ? p - w : c_lo; c <= p + w && c < c_hi; ++c) {
-          if (c == p) continue;
-          const float* x = syn0 + (uint64_t)t[c] * dim;
+    const int32_t* t = sen + sen_off[s];
+    const int L = (int)(sen_off[s + 1] - sen_off[s]);
+    for (int q = 0; q < L; ++q) {
+      uint32_t b;
+      dgo_position_draws(seed, first_sen + s, (uint32_t)q, epoch, window, K, table, table_size,
+                         vocab, &b, negs);
+      const int w = (int)window - (int)b;
+      for (int c = q - w; c <= q + w; ++c) {
+        if (c < 0 || c >= L || c == q) continue;
+        double* x = x0 + (uint64_t)t[c] * dim;
+        for (uint32_t d = 0; d < dim; ++d) neu1e[d] = 0.0;
+        for (uint32_t n = 0; n <= K; ++n) {
+          const int32_t row = n == 0 ? t[q] : negs[n - 1];
+          if (n > 0 && row == t[q]) continue;
+          double* y = y1 + (uint64_t)row * dim;
           double f = 0.0;
-          for (uint32_t d = 0; d < dim; ++d) f += (double)x[d] * ytil[d];
-          double gg = (lab - sig_d(f, expt, exact_sigmoid)) * alpha;
-          if (r > 0) gg *= wneg;
-          g[(size_t)(c - c_lo) * no + n] = gg;
-          for (uint32_t d = 0; d < dim; ++d) ytil[d] += gg * (double)x[d];
+          for (uint32_t d = 0; d < dim; ++d) f += x[d] * y[d];
+          double g = ((n == 0 ? 1.0 : 0.0) - sig_d(f, expt, exact_sigmoid)) * alpha;
+          if (n > 0) g *= wneg;
+          for (uint32_t d = 0; d < dim; ++d) {
+            neu1e[d] += g * y[d];
+            y[d] += g * x[d];
+          }
         }
+        for (uint32_t d = 0; d < dim; ++d) x[d] += neu1e[d];
       }
-      for (int c = 0; c < nc; ++c) {
-        double* v = vecs + ns * dim;
-        for (int n = 0; n < no; ++n) {
-          double gg = g[(size_t)c * no + n];
-          if (gg == 0.0) continue;
-          for (uint32_t d = 0; d < dim; ++d) v[d] += gg * syn1[(uint64_t)out[n] * dim + d];
-        }
-        cs[ns].row = (uint32_t)t[c_lo + c];
-        cs[ns].table = 0;
-        cs[ns].order = ns;
-        cs[ns].vec = v;
-        ++ns;
-      }
-      for (int n = 0; n < no; ++n) {
-        double* v = vecs + ns * dim;
-        for (int c = 0; c < nc; ++c) {
-          double gg = g[(size_t)c * no + n];
-          if (gg != 0.0)
-            for (uint32_t d = 0; d < dim; ++d) v[d] += gg * syn0[(uint64_t)t[c_lo + c] * dim + d];
-        }
-        cs[ns].row = (uint32_t)out[n];
-        cs[ns].table = 1;
-        cs[ns].order = ns;
-        cs[ns].vec = v;
-        ++ns;
-      }
-      free(g);
     }
-    qsort(cs, ns, sizeof(contrib_t), cmp_contrib);
-    double* acc = (double*)malloc(sizeof(double) * dim);
-    uint64_t i = 0;
-    while (i < ns) {
-      uint64_t j = i;
-      for (uint32_t d = 0; d < dim; ++d) acc[d] = 0.0;
-      while (j < ns && cs[j].row == cs[i].row && cs[j].table == cs[i].table) {
-        for (uint32_t d = 0; d < dim; ++d) acc[d] += cs[j].vec[d];
-        ++j;
-      }
-      float* w = (cs[i].table == 0 ? syn0 : syn1) + (uint64_t)cs[i].row * dim;
-      for (uint32_t d = 0; d < dim; ++d) w[d] = (float)((double)w[d] + acc[d]);
-      i = j;
-    }
-    free(acc);
-    free(cs);
-    free(vecs);
   }
-  free(ytil);
-  free(b);
-  free(out);
-  free(ts);
-  free(tj);
+  for (uint64_t i = 0; i < n_rows * dim; ++i) {
+    syn0[i] = (float)x0[i];
+    syn1[i] = (float)y1[i];
+  }
+  free(neu1e);
+  free(negs);
+  free(x0);
+  free(y1);
+}
+
+/* ------------------------------------------------------------------------
+ * The same order as dgo_sg_shared, evaluated the way the batched device
+ * forms evaluate it: per position, from the rows as the earlier positions
+ * left them (x = syn0 rows of the contexts, y = syn1 rows of the target and
+ * negatives), the gradients of the contexts c_1 < c_2 < ... in order,
+ *   s_i[n] = x_i.y_n + sum_{j<i} g_j[n] x_i.x_j
+ *                    + sum_{j<i, v_j = v_i} sum_n' g_j[n'] y_n'.y_n
+ *   g_i[n] = (label_n - sig(s_i[n])) alpha w_n
+ * (v_i the vertex of context i), then the updates
+ *   syn0[v] += sum_{i: v_i = v} [ sum_n g_i[n] y_n + sum_{j<i} (g_i . g_j) x_j ]
+ *   syn1[y_n] += sum_i g_i[n] x_i + sum_{i, j<i, v_j = v_i} sum_n' g_i[n] g_j[n'] y_n'.
+ * These are the sequential updates of dgo_sg_shared up to terms of second
+ * order in the updates (an output row or a context row changed by an earlier
+ * pair of the same position shifts a later logit or update in proportion to
+ * that change; the change of a change is dropped), and two negatives of one
+ * position on the same row do not see each other's updates.  FP64.
+ * ---------------------------------------------------------------------- */
+void dgo_sg_shared_first_order(const int32_t* sen, const uint64_t* sen_off, uint64_t first_sen,
+                               uint64_t n_sen, uint32_t window, uint32_t negative, uint32_t draws,
+                               const int32_t* table, uint32_t table_size, uint32_t vocab,
+                               float alpha, const float* expt, int exact_sigmoid, uint64_t seed,
+                               uint32_t epoch, uint32_t dim, float* syn0, float* syn1) {
+  const uint32_t K = draws ? draws : negative, K1 = K + 1;
+  const float wneg = (float)negative / (float)K;
+  const uint64_t n_rows = vocab;
+  double* x0 = (double*)malloc(sizeof(double) * n_rows * dim);
+  double* y1 = (double*)malloc(sizeof(double) * n_rows * dim);
+  for (uint64_t i = 0; i < n_rows * dim; ++i) {
+    x0[i] = (double)syn0[i];
+    y1[i] = (double)syn1[i];
+  }
+  const int MS = 2 * (int)window;
+  int32_t* negs = (int32_t*)malloc(sizeof(int32_t) * (K ? K : 1));
+  int32_t* rows = (int32_t*)malloc(sizeof(int32_t) * K1);
+  int* live = (int*)malloc(sizeof(int) * K1);
+  int* cv = (int*)malloc(sizeof(int) * (MS + 1));
+  double* X = (double*)malloc(sizeof(double) * (MS + 1) * dim);   /* context rows by slot */
+  double* Y = (double*)malloc(sizeof(double) * K1 * dim);
+  double* g = (double*)malloc(sizeof(double) * (MS + 1) * K1);
+  double* dX = (double*)malloc(sizeof(double) * (MS + 1) * dim);
+  double* dY = (double*)malloc(sizeof(double) * K1 * dim);
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    const int32_t* t = sen + sen_off[s];
+    const int L = (int)(sen_off[s + 1] - sen_off[s]);
+    if (L < 2) continue;
+    for (int q = 0; q < L; ++q) {
+      uint32_t b;
+      dgo_position_draws(seed, first_sen + s, (uint32_t)q, epoch, window, K, table, table_size,
+                         vocab, &b, negs);
+      const int w = (int)window - (int)b;
+      int m = 0;
+      for (int c = q - w; c <= q + w; ++c)
+        if (c >= 0 && c < L && c != q) cv[m++] = t[c];
+      for (uint32_t n = 0; n < K1; ++n) {
+        rows[n] = n == 0 ? t[q] : negs[n - 1];
+        live[n] = n == 0 || rows[n] != t[q];
+        memcpy(Y + (size_t)n * dim, y1 + (uint64_t)rows[n] * dim, sizeof(double) * dim);
+      }
+      for (int i = 0; i < m; ++i) memcpy(X + (size_t)i * dim, x0 + (uint64_t)cv[i] * dim, sizeof(double) * dim);
+      for (int i = 0; i < m; ++i)
+        for (uint32_t n = 0; n < K1; ++n) {
+          double gg = 0.0;
+          if (live[n]) {
+            double sv = 0.0;
+            for (uint32_t d = 0; d < dim; ++d) sv += X[(size_t)i * dim + d] * Y[(size_t)n * dim + d];
+            for (int j = 0; j < i; ++j) {
+              double xx = 0.0;
+              for (uint32_t d = 0; d < dim; ++d) xx += X[(size_t)i * dim + d] * X[(size_t)j * dim + d];
+              sv += g[j * K1 + n] * xx;
+              if (cv[j] == cv[i])
+                for (uint32_t n2 = 0; n2 < K1; ++n2) {
+                  double yy = 0.0;
+                  for (uint32_t d = 0; d < dim; ++d) yy += Y[(size_t)n2 * dim + d] * Y[(size_t)n * dim + d];
+                  sv += g[j * K1 + n2] * yy;
+                }
+            }
+            gg = ((n == 0 ? 1.0 : 0.0) - sig_d(sv, expt, exact_sigmoid)) * alpha;
+            if (n > 0) gg *= wneg;
+          }
+          g[i * K1 + n] = gg;
+        }
+      for (int i = 0; i < m; ++i)
+        for (uint32_t d = 0; d < dim; ++d) {
+          double v = 0.0;
+          for (uint32_t n = 0; n < K1; ++n) v += g[i * K1 + n] * Y[(size_t)n * dim + d];
+          for (int j = 0; j < i; ++j) {
+            double gij = 0.0;
+            for (uint32_t n = 0; n < K1; ++n) gij += g[i * K1 + n] * g[j * K1 + n];
+            v += gij * X[(size_t)j * dim + d];
+          }
+          dX[(size_t)i * dim + d] = v;
+        }
+      for (uint32_t n = 0; n < K1; ++n)
+        for (uint32_t d = 0; d < dim; ++d) {
+          double v = 0.0;
+          for (int i = 0; i < m; ++i) {
+            v += g[i * K1 + n] * X[(size_t)i * dim + d];
+            for (int j = 0; j < i; ++j)
+              if (cv[j] == cv[i])
+                for (uint32_t n2 = 0; n2 < K1; ++n2) v += g[i * K1 + n] * g[j * K1 + n2] * Y[(size_t)n2 * dim + d];
+          }
+          dY[(size_t)n * dim + d] = v;
+        }
+      for (int i = 0; i < m; ++i)
+        for (uint32_t d = 0; d < dim; ++d) x0[(uint64_t)cv[i] * dim + d] += dX[(size_t)i * dim + d];
+      for (uint32_t n = 0; n < K1; ++n)
+        if (live[n])
+          for (uint32_t d = 0; d < dim; ++d) y1[(uint64_t)rows[n] * dim + d] += dY[(size_t)n * dim + d];
+    }
+  }
+  for (uint64_t i = 0; i < n_rows * dim; ++i) {
+    syn0[i] = (float)x0[i];
+    syn1[i] = (float)y1[i];
+  }
+  free(dY);
+  free(dX);
+  free(g);
+  free(Y);
+  free(X);
+  free(cv);
+  free(live);
+  free(rows);
+  free(negs);
+  free(x0);
+  free(y1);
 }
 
 /* ------------------------------------------------------------------------

@@ -36,11 +36,13 @@ def lib():
                                      c.c_uint32, c.c_float, vp, c.c_uint32, vp, vp]
     L.dgo_plain_batches.restype = c.c_uint64
     L.dgo_plain_batches.argtypes = [vp, c.c_uint64, vp, c.c_uint64]
-    L.dgo_tile_count.restype = c.c_uint64
-    L.dgo_tile_count.argtypes = [vp, c.c_uint64, c.c_uint32]
-    L.dgo_sg_batched.argtypes = [vp, vp, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint32,
-                                 c.c_uint32, vp, c.c_uint32, c.c_uint32, c.c_float, vp, c.c_int,
-                                 c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp]
+    L.dgo_position_draws.argtypes = [c.c_uint64, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32,
+                                     c.c_uint32, vp, c.c_uint32, c.c_uint32,
+                                     c.POINTER(c.c_uint32), vp]
+    L.dgo_sg_shared.argtypes = [vp, vp, c.c_uint64, c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32,
+                                vp, c.c_uint32, c.c_uint32, c.c_float, vp, c.c_int, c.c_uint64,
+                                c.c_uint32, c.c_uint32, vp, vp]
+    L.dgo_sg_shared_first_order.argtypes = L.dgo_sg_shared.argtypes
     L.dgo_tuple_loss.restype = c.c_double
     L.dgo_tuple_loss.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32]
     _L = L
@@ -104,14 +106,31 @@ class Trainer:
                                  _vp(self.syn0), _vp(self.syn1))
         return nb
 
-    def batched(self, sen, off, window, negative, alpha, minibatch, exact, seed, epoch,
-                first_sen=0, tile=0, draws=0):
+    def shared(self, sen, off, window, negative, alpha, exact, seed, epoch, first_sen=0, draws=0):
+        """Sequential SGNS with per-position shared negatives (the batched forms' order)."""
         L = lib()
         nsen = len(off) - 1
-        L.dgo_sg_batched(_vp(sen), _vp(off), first_sen, nsen, minibatch, window, negative,
-                         _vp(self.table), self.table_size, self.vocab, alpha, _vp(self.expt),
-                         int(exact), seed, epoch, self.dim, tile, draws, _vp(self.syn0),
-                         _vp(self.syn1))
+        L.dgo_sg_shared(_vp(sen), _vp(off), first_sen, nsen, window, negative, draws,
+                        _vp(self.table), self.table_size, self.vocab, alpha, _vp(self.expt),
+                        int(exact), seed, epoch, self.dim, _vp(self.syn0), _vp(self.syn1))
+
+    def shared_first_order(self, sen, off, window, negative, alpha, exact, seed, epoch,
+                           first_sen=0, draws=0):
+        """The batched forms' first-order evaluation of the shared() order."""
+        L = lib()
+        nsen = len(off) - 1
+        L.dgo_sg_shared_first_order(_vp(sen), _vp(off), first_sen, nsen, window, negative, draws,
+                                    _vp(self.table), self.table_size, self.vocab, alpha,
+                                    _vp(self.expt), int(exact), seed, epoch, self.dim,
+                                    _vp(self.syn0), _vp(self.syn1))
+
+    def position_draws(self, seed, gs, q, epoch, window, draws):
+        L = lib()
+        b = ctypes.c_uint32()
+        negs = np.zeros(max(draws, 1), np.int32)
+        L.dgo_position_draws(seed, gs, q, epoch, window, draws, _vp(self.table), self.table_size,
+                             self.vocab, ctypes.byref(b), _vp(negs))
+        return b.value, negs[:draws].copy()
 
 
 def tuple_loss(tuples, K, syn0, syn1):
