@@ -392,39 +392,48 @@ struct SentArgs {
   int all;  // take every sentence (else only those of this pool size)
 };
 
-template <typename E, int DIM, int P>
+template <typename E, int DIM, int P, int KC>
 struct alignas(16) SentLds {
   static constexpr int LD = DIM + 16 / (int)sizeof(E);
-  E xc[P * LD];             // pool rows as used at this position
-  E yb[kOutCols * LD];      // output rows (target, negatives), unused rows zero
-  float S[P * 17];          // S[u][n]
-  float Gx[P * (P + 1)];    // Gx[u][v]
-  float Gy[kOutCols * 17];  // Gy[n][n']
-  float Gp[P * 20];         // Gp[u][n], rows 16-byte aligned
-  float H[P * (P + 4)];     // H[u][v] = sum over context pairs j < i (v_i = u, v_j = v) of g_i . g_j
-  float Mm[kOutCols * 20];  // M[n][n'] = sum over pairs j < i of one vertex of g_i[n] g_j[n']
-  float Gs[kMaxSlots * kOutCols];  // g_i[n] by context slot
-  int su[kMaxSlots];        // pool index of context slot i
-  uint8_t p2p[kSpan];       // position - lo -> pool index
+  static constexpr int GXL = P + 4;   // Gx, H rows (16-byte aligned)
+  static constexpr int GPL = KC + 4;  // Gp, M rows (16-byte aligned)
+  E xc[P * LD];              // pool rows as used at this position
+  E yb[KC * LD];             // output rows (target, negatives), rows past K zero
+  float S[P * (KC + 1)];     // S[u][n]
+  float Gx[P * GXL];         // Gx[u][v]
+  float Gy[KC * (KC + 1)];   // Gy[n][n']
+  float Gp[P * GPL];         // Gp[u][n]
+  float H[P * GXL];          // H[u][v] = sum over context pairs j < i (v_i = u, v_j = v) of g_i . g_j
+  float Mm[KC * GPL];        // M[n][n'] = sum over pairs j < i of one vertex of g_i[n] g_j[n']
+  float Gs[kMaxSlots * KC];  // g_i[n] by context slot
+  int su[kMaxSlots];         // pool index of context slot i
+  uint8_t p2p[kSpan];        // position - lo -> pool index
 };
 
 // P = 16: sentences of at most 16 distinct vertices (one segment each);
 // P = 32: every other sentence, in segments whose windows hold <= 32.
-template <typename E, int DIM, int P>
-__global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
+// KC output columns: 8 (up to 7 negatives) or 16.
+template <typename E, int DIM, int P, int KC>
+__global__ __launch_bounds__(64 * kSentWaves) __attribute__((amdgpu_waves_per_eu(P == 16 ? 2 : 1)))
+void sg_sent_kernel(SentArgs t) {
   typedef Chunk<E> C;
-  typedef SentLds<E, DIM, P> Lds;
-  constexpr int LD = Lds::LD;
+  typedef typename C::v_t V;
+  typedef SentLds<E, DIM, P, KC> Lds;
+  constexpr int LD = Lds::LD, GXL = Lds::GXL, GPL = Lds::GPL;
   constexpr int NT = DIM / 16;   // dim tiles
   constexpr int NPT = P / 16;    // pool tiles
-  constexpr int YCH = kOutCols * DIM / 4 / 64;  // float4 chunks per lane for 16 output rows
+  constexpr int YCH = KC * DIM / 4 / 64;  // float4 chunks per lane for KC output rows
+  static_assert(KC % 4 == 0 && KC <= 16 && YCH >= 1, "shape");
   __shared__ Lds lds_all[kSentWaves];
   __shared__ float s_expt[kExpTable + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, g4 = lane >> 4;
+  const bool krow = l16 < KC;      // lane supplies an output row (m index) of an MFMA
+  const bool kchk = 4 * g4 < KC;   // lane supplies output columns (k index) of an MFMA
+  const V vzero = C::from(0.f, 0.f, 0.f, 0.f);
   Lds& L = lds_all[wv];
   for (int i = threadIdx.x; i <= kExpTable; i += blockDim.x) s_expt[i] = t.expt[i];
-  for (int i = lane; i < kOutCols * LD; i += 64) L.yb[i] = (E)0;
+  for (int i = lane; i < KC * LD; i += 64) L.yb[i] = (E)0;
   __syncthreads();
   const int W = t.window, K = t.kd;
   const uint64_t pos_base = t.sen_off[t.s_begin];
@@ -471,8 +480,8 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
       // 2 W + 1 <= P and 2 W < kSpan: the window of q0 always joins
       const int q1 = c_last == Ls - 1 ? Ls : c_last - W + 1;
       // ---- pool rows: lane (u = p*16 + l16, g4) holds dims 16 t + 4 g4 + e
-      typename C::v_t ox[NPT][NT];   // as loaded, rounded to E
-      f32x4_t dx[NPT][NT];           // FP32 deltas
+      V ox[NPT][NT];           // as loaded, rounded to E
+      f32x4_t dx[NPT][NT];     // FP32 deltas
 #pragma unroll
       for (int p = 0; p < NPT; ++p) {
         const int u = p * 16 + l16;
@@ -493,8 +502,9 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
         int orow = -1;
         if (lane == 0) orow = target;
         else if (lane <= K) orow = ngs[(uint64_t)q * K + lane - 1];
+        const int row0 = __shfl(orow, 0, 64);
         const int b = shr[q];
-        // stage the output rows (rows > K stay zero): all loads issued first
+        // stage the output rows (rows past K stay zero): all loads issued first
         {
           float4 yv[YCH];
 #pragma unroll
@@ -512,7 +522,7 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
             if (n <= K) C::put4(L.yb + n * LD + 4 * c4, yv[r].x, yv[r].y, yv[r].z, yv[r].w);
           }
         }
-        // pool rows as they stand, for the dY product
+        // pool rows as they stand
 #pragma unroll
         for (int p = 0; p < NPT; ++p)
 #pragma unroll
@@ -530,27 +540,31 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
           }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        // S^T = Y X^T, Gx = X X^T, Gy = Y Y^T; chunk operands read back from LDS
+        // S^T = Y X^T, Gx = X X^T, Gy = Y Y^T (chunk operands from LDS)
         {
           f32x4_t gy = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) {
-            const typename C::v_t yv = C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4);
+            const V yv = krow ? C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4) : vzero;
             gy = C::mma(yv, yv, gy);
           }
+          if (kchk && krow) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) L.Gy[(4 * g4 + e) * 17 + l16] = gy[e];
+            for (int e = 0; e < 4; ++e) L.Gy[(4 * g4 + e) * (KC + 1) + l16] = gy[e];
+          }
         }
 #pragma unroll
         for (int p = 0; p < NPT; ++p) {
           f32x4_t st = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt)
-            st = C::mma(C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4),
+            st = C::mma(krow ? C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4) : vzero,
                         C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4), st);
           // st[e] = S^T[n = 4 g4 + e][u = p*16 + l16]
+          if (kchk) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) L.S[(p * 16 + l16) * 17 + 4 * g4 + e] = st[e];
+            for (int e = 0; e < 4; ++e) L.S[(p * 16 + l16) * (KC + 1) + 4 * g4 + e] = st[e];
+          }
 #pragma unroll
           for (int p2 = 0; p2 < NPT; ++p2) {
             f32x4_t gx = {0.f, 0.f, 0.f, 0.f};
@@ -559,30 +573,30 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
               gx = C::mma(C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4),
                           C::load4(L.xc + (p2 * 16 + l16) * LD + 16 * tt + 4 * g4), gx);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) L.Gx[(p * 16 + 4 * g4 + e) * (P + 1) + p2 * 16 + l16] = gx[e];
+            for (int e = 0; e < 4; ++e) L.Gx[(p * 16 + 4 * g4 + e) * GXL + p2 * 16 + l16] = gx[e];
           }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         // ---- the recursion, one lane per output column
+        int slot = 0;
         {
           const int n = lane;
-          const int row0 = __shfl(orow, 0, 64);
           const bool live = n <= K && (n == 0 || orow != row0);
           const float lab = n == 0 ? 1.0f : 0.0f;
           const float wt = n == 0 ? t.alpha : t.alpha * t.wneg;
-          float gyr[kOutCols];
+          const int nc = n < KC ? n : 0;
+          float gyr[KC];
 #pragma unroll
-          for (int k = 0; k < kOutCols; ++k) gyr[k] = n < kOutCols ? L.Gy[k * 17 + (n & 15)] : 0.0f;
-          float corr[P], gp[P], mrow[kOutCols];
+          for (int k = 0; k < KC; ++k) gyr[k] = L.Gy[k * (KC + 1) + nc];
+          float corr[P], gp[P], mrow[KC];
 #pragma unroll
           for (int v = 0; v < P; ++v) {
             corr[v] = 0.0f;
             gp[v] = 0.0f;
           }
 #pragma unroll
-          for (int k = 0; k < kOutCols; ++k) mrow[k] = 0.0f;
-          int slot = 0;
+          for (int k = 0; k < KC; ++k) mrow[k] = 0.0f;
           const int w = W - b;
           const int c_lo = q - w > 0 ? q - w : 0;
           const int c_hi = q + w < Ls - 1 ? q + w : Ls - 1;
@@ -595,99 +609,114 @@ __global__ __launch_bounds__(64 * kSentWaves) void sg_sent_kernel(SentArgs t) {
               cu = v == u ? corr[v] : cu;
               gpu = v == u ? gp[v] : gpu;
             }
-            float sv = (n < kOutCols ? L.S[u * 17 + (n & 15)] : 0.0f) + cu;
-            float gpk[kOutCols];  // g of the earlier occurrences of this vertex, per column
+            float sv = L.S[u * (KC + 1) + nc] + cu;
+            float gpk[KC];  // g of the earlier occurrences of this vertex, per column
 #pragma unroll
-            for (int k = 0; k < kOutCols; ++k) {
+            for (int k = 0; k < KC; ++k) {
               gpk[k] = k <= K ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gpu), k)) : 0.0f;
               sv += gpk[k] * gyr[k];
             }
             float g = 0.0f;
             if (live) g = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
 #pragma unroll
-            for (int k = 0; k < kOutCols; ++k) mrow[k] += g * gpk[k];
-            if (n < kOutCols) L.Gs[slot * kOutCols + n] = g;
+            for (int k = 0; k < KC; ++k) mrow[k] += g * gpk[k];
+            if (n < KC) L.Gs[slot * KC + n] = g;
             if (lane == 0) L.su[slot] = u;
             ++slot;
+            const float4* gxr = reinterpret_cast<const float4*>(L.Gx + u * GXL);  // Gx symmetric
 #pragma unroll
-            for (int v = 0; v < P; ++v) {
-              gp[v] += v == u ? g : 0.0f;
-              corr[v] += g * L.Gx[v * (P + 1) + u];
+            for (int v4 = 0; v4 < P / 4; ++v4) {
+              const float4 gq = gxr[v4];
+              corr[4 * v4 + 0] += g * gq.x;
+              corr[4 * v4 + 1] += g * gq.y;
+              corr[4 * v4 + 2] += g * gq.z;
+              corr[4 * v4 + 3] += g * gq.w;
             }
+#pragma unroll
+            for (int v = 0; v < P; ++v) gp[v] += v == u ? g : 0.0f;
           }
-          if (n < kOutCols) {
+          if (n < KC) {
 #pragma unroll
-            for (int v = 0; v < P; ++v) L.Gp[v * 20 + n] = gp[v];
+            for (int v = 0; v < P; ++v) L.Gp[v * GPL + n] = gp[v];
 #pragma unroll
-            for (int k = 0; k < kOutCols; ++k) L.Mm[n * 20 + k] = mrow[k];
+            for (int k = 0; k < KC; ++k) L.Mm[n * GPL + k] = mrow[k];
           }
-          for (int i = lane; i < P * (P + 4); i += 64) L.H[i] = 0.0f;
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          // H[v_i][v_j] += g_i . g_j for every pair of context slots j < i
-          for (int i = 1; i < slot; ++i) {
-            if (lane < i) {
-              float dot = 0.0f;
+        }
+        for (int i = lane; i < P * GXL; i += 64) L.H[i] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // H[v_i][v_j] += g_i . g_j for every pair of context slots j < i
+        for (int i = 1; i < slot; ++i) {
+          if (lane < i) {
+            float dot = 0.0f;
 #pragma unroll
-              for (int k = 0; k < kOutCols; ++k)
-                if (k <= K) dot += L.Gs[i * kOutCols + k] * L.Gs[lane * kOutCols + k];
-              atomicAdd(&L.H[L.su[i] * (P + 4) + L.su[lane]], dot);
-            }
+            for (int k = 0; k < KC; ++k) dot += L.Gs[i * KC + k] * L.Gs[lane * KC + k];
+            atomicAdd(&L.H[L.su[i] * GXL + L.su[lane]], dot);
           }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        // ---- dX = Gp Y into the deltas: dX^T[d][u] = sum_n Y[n][d] Gp[u][n]
+        // ---- dX = Gp Y + H X into the deltas (dX^T[d][u] layout)
 #pragma unroll
         for (int p = 0; p < NPT; ++p) {
-          const float4 gq = *reinterpret_cast<const float4*>(L.Gp + (p * 16 + l16) * 20 + 4 * g4);
-          const typename C::v_t bgp = C::from(gq.x, gq.y, gq.z, gq.w);
+          V bgp = vzero;
+          if (kchk) {
+            const float4 gq = *reinterpret_cast<const float4*>(L.Gp + (p * 16 + l16) * GPL + 4 * g4);
+            bgp = C::from(gq.x, gq.y, gq.z, gq.w);
+          }
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) {
-            const E* yc = L.yb + (4 * g4) * LD + 16 * tt + l16;
-            const typename C::v_t ay =
-                C::from(C::elem(yc), C::elem(yc + LD), C::elem(yc + 2 * LD), C::elem(yc + 3 * LD));
+            V ay = vzero;
+            if (kchk) {
+              const E* yc = L.yb + (4 * g4) * LD + 16 * tt + l16;
+              ay = C::from(C::elem(yc), C::elem(yc + LD), C::elem(yc + 2 * LD), C::elem(yc + 3 * LD));
+            }
             dx[p][tt] = C::mma(ay, bgp, dx[p][tt]);
           }
-          // + H X: dX^T[d][u] += sum_v X[v][d] H[u][v]
 #pragma unroll
           for (int p2 = 0; p2 < NPT; ++p2) {
-            const float4 hq = *reinterpret_cast<const float4*>(L.H + (p * 16 + l16) * (P + 4) + p2 * 16 + 4 * g4);
-            const typename C::v_t bh = C::from(hq.x, hq.y, hq.z, hq.w);
+            const float4 hq = *reinterpret_cast<const float4*>(L.H + (p * 16 + l16) * GXL + p2 * 16 + 4 * g4);
+            const V bh = C::from(hq.x, hq.y, hq.z, hq.w);
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt) {
               const E* xq = L.xc + (p2 * 16 + 4 * g4) * LD + 16 * tt + l16;
-              const typename C::v_t ax =
-                  C::from(C::elem(xq), C::elem(xq + LD), C::elem(xq + 2 * LD), C::elem(xq + 3 * LD));
+              const V ax = C::from(C::elem(xq), C::elem(xq + LD), C::elem(xq + 2 * LD), C::elem(xq + 3 * LD));
               dx[p][tt] = C::mma(ax, bh, dx[p][tt]);
             }
           }
         }
-        // ---- dY = Gp^T X: dY[n][d] = sum_u Gp[u][n] X[u][d] -> output rows
+        // ---- dY = Gp^T X + M Y -> output rows
+        V am = vzero;
+        if (krow && kchk) {
+          const float4 mq = *reinterpret_cast<const float4*>(L.Mm + l16 * GPL + 4 * g4);
+          am = C::from(mq.x, mq.y, mq.z, mq.w);
+        }
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
           f32x4_t dy = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int p = 0; p < NPT; ++p) {
-            const float* gc = L.Gp + (p * 16 + 4 * g4) * 20 + l16;
-            const typename C::v_t ag = C::from(gc[0], gc[20], gc[40], gc[60]);
+            V ag = vzero;
+            if (krow) {
+              const float* gc = L.Gp + (p * 16 + 4 * g4) * GPL + l16;
+              ag = C::from(gc[0], gc[GPL], gc[2 * GPL], gc[3 * GPL]);
+            }
             const E* xcol = L.xc + (p * 16 + 4 * g4) * LD + 16 * tt + l16;
-            const typename C::v_t bx =
-                C::from(C::elem(xcol), C::elem(xcol + LD), C::elem(xcol + 2 * LD), C::elem(xcol + 3 * LD));
+            const V bx = C::from(C::elem(xcol), C::elem(xcol + LD), C::elem(xcol + 2 * LD), C::elem(xcol + 3 * LD));
             dy = C::mma(ag, bx, dy);
           }
           {
-            // + M Y: dY[n][d] += sum_n' M[n][n'] Y[n'][d]
-            const float4 mq = *reinterpret_cast<const float4*>(L.Mm + l16 * 20 + 4 * g4);
-            const E* yq = L.yb + (4 * g4) * LD + 16 * tt + l16;
-            dy = C::mma(C::from(mq.x, mq.y, mq.z, mq.w),
-                        C::from(C::elem(yq), C::elem(yq + LD), C::elem(yq + 2 * LD), C::elem(yq + 3 * LD)), dy);
+            V by = vzero;
+            if (kchk) {
+              const E* yq = L.yb + (4 * g4) * LD + 16 * tt + l16;
+              by = C::from(C::elem(yq), C::elem(yq + LD), C::elem(yq + 2 * LD), C::elem(yq + 3 * LD));
+            }
+            dy = C::mma(am, by, dy);
           }
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int n = 4 * g4 + e;
             const int row = __shfl(orow, n, 64);
-            const int row0 = __shfl(orow, 0, 64);
             if (n <= K && (n == 0 || row != row0))
               atomicAdd(&t.syn1[(uint64_t)row * DIM + 16 * tt + l16], dy[e]);
           }
@@ -1199,9 +1228,9 @@ static unsigned resident_grid(dg_ctx* ctx, Kern k, int block) {
   return (unsigned)per_cu * (unsigned)ctx->num_cus;
 }
 
-template <typename E, int DIM, int P>
+template <typename E, int DIM, int P, int KC>
 static void launch_sent(dg_ctx* ctx, const SentArgs& t) {
-  auto k = sg_sent_kernel<E, DIM, P>;
+  auto k = sg_sent_kernel<E, DIM, P, KC>;
   uint64_t g = t.deterministic ? 1 : resident_grid(ctx, k, 64 * kSentWaves);
   const uint64_t need = (t.s_end - t.s_begin + kSentWaves - 1) / kSentWaves;
   if (g > need) g = need;
@@ -1211,16 +1240,24 @@ static void launch_sent(dg_ctx* ctx, const SentArgs& t) {
 // Hogwild: sentences of <= 16 distinct vertices on the 16-row pool, the
 // others on the 32-row pool.  Deterministic: one wavefront, every sentence
 // in order, on the 32-row pool.
-template <typename E, int DIM>
-static void launch_sent_dim(dg_ctx* ctx, SentArgs t) {
+template <typename E, int DIM, int KC>
+static void launch_sent_kc(dg_ctx* ctx, SentArgs t) {
   if (t.deterministic) {
     t.all = 1;
-    launch_sent<E, DIM, 32>(ctx, t);
+    launch_sent<E, DIM, 32, KC>(ctx, t);
   } else {
     t.all = 0;
-    launch_sent<E, DIM, 16>(ctx, t);
-    launch_sent<E, DIM, 32>(ctx, t);
+    launch_sent<E, DIM, 16, KC>(ctx, t);
+    launch_sent<E, DIM, 32, KC>(ctx, t);
   }
+}
+
+template <typename E, int DIM>
+static void launch_sent_dim(dg_ctx* ctx, const SentArgs& t) {
+  if (t.kd + 1 <= 8)
+    launch_sent_kc<E, DIM, 8>(ctx, t);
+  else
+    launch_sent_kc<E, DIM, 16>(ctx, t);
 }
 
 template <typename E>
