@@ -57,13 +57,29 @@ def walker_bytes(stats_list):
     return 16 * moves + 8 * draws + 12 * tokens + 12 * walks
 
 
-def sgns_tile_bytes(kept, tiles, dim, negative):
-    """Algorithmic HBM bytes of the BF16 tile kernel: every staged row (context
-    rows ~ kept positions, output rows = kept + K per tile) is read in BF16 (2 B
-    per element), its FP32 master element read and written (8 B) and its BF16
-    copy rewritten (2 B); plus 4 B per kept token of the sentence buffer."""
-    rows = 2 * kept + negative * tiles
-    return rows * dim * 12 + 4 * kept
+def sgns_sent_bytes(kept, pool_rows, dim, negative):
+    """Algorithmic HBM bytes of the sentence kernel: per position its K + 1
+    output rows (target, negatives) are read and updated in place (FP32: 4 B
+    read + 4 B read and 4 B written by the atomic add, per element); per
+    sentence each distinct input row is read once and updated once the same
+    way; per position 4 B token, 1 B window shrink and 4 B per negative id."""
+    rows = (negative + 1) * kept + pool_rows
+    return rows * dim * 12 + kept * (5 + 4 * negative)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the BF16 sentence kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy
+    kernel), or None if the summary is absent."""
+    path = os.path.join(REPO, "profiles", "sg_sent_kernel_bf16_c2_hbm_counters.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def auc_of(ctx, pos, neg):
@@ -174,6 +190,7 @@ def main():
         _, pack_ms = ctx.timing("pack")
         _, re_ms = ctx.timing("re")
         n_bf, bf_ms = ctx.timing("sgns_bf16")
+        _, prep_ms = ctx.timing("sgns_prep")
         words = sum(s.words for s in sstats)
         steps = sum(s.steps for s in wstats)
         value = words / wall
@@ -184,18 +201,20 @@ def main():
         # walker roofline (per launch)
         walk_bytes = walker_bytes(wstats) / n_walk
         walk_gbs = walk_bytes / (walk_ms / n_walk * 1e-3) / 1e9
-        # SGNS BF16 roofline (per launch)
+        # SGNS BF16 roofline (per launch of the sentence kernels)
         kept = sum(s.kept for s in sstats)
-        tiles = sum(s.sentences for s in sstats)  # one tile per walk of <= 22 kept words
-        bf_bytes = sgns_tile_bytes(kept, tiles, args.dim, 5) / n_bf
+        pool_rows = sum(s.pool_rows for s in sstats)
+        bf_bytes = sgns_sent_bytes(kept, pool_rows, args.dim, 5) / n_bf
         bf_gbs = bf_bytes / (bf_ms / n_bf * 1e-3) / 1e9
         out["walk"] = {"walk_steps_per_s": steps / (walk_ms * 1e-3),
                        "round_steps_per_s": steps / ((walk_ms + pack_ms + re_ms) * 1e-3),
                        "kernel_ms": walk_ms / n_walk, "pack_ms": pack_ms / n_walk,
                        "re_ms": re_ms / n_walk, "bytes_per_launch": walk_bytes,
                        "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS}
-        out["sgns_bf16_pipeline"] = {"words_per_s": words / (bf_ms * 1e-3),
+        out["sgns_bf16_pipeline"] = {"words_per_s": words / ((bf_ms + prep_ms) * 1e-3),
                                      "kernel_ms_per_launch": bf_ms / n_bf, "launches": n_bf,
+                                     "prep_ms_per_launch": prep_ms / n_bf,
+                                     "kept": kept, "pool_rows": pool_rows,
                                      "bytes_per_launch": bf_bytes, "achieved_gbs": bf_gbs}
 
         if not args.no_compare:
@@ -244,9 +263,9 @@ def main():
                                 f"per step one walk round of V walks (min_L 20, mu 0.999) + SGNS "
                                 f"BF16-MFMA dim {args.dim} w10 K5 sample 1e-3 on that round"),
                    "global_batch": V, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "sg_mfma_kernel<bf16>",
+        "roofline": {"bound": "hbm", "kernel": "sg_sent_kernel<bf16, 128, 16, 8>",
                      "achieved": bf_gbs, "peak": HBM_GBS, "unit": "GB/s",
-                     "frac": bf_gbs / HBM_GBS, "traffic": None},
+                     "frac": bf_gbs / HBM_GBS, "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     line.update(out)

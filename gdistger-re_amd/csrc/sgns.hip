@@ -325,6 +325,7 @@ struct DrawArgs {
   uint8_t* shrink;  // per kept position of the launch
   int32_t* negs;    // kd per kept position of the launch
   uint8_t* ucount;  // per sentence of the launch: distinct vertices, 255 if above 64
+  unsigned long long* pool_rows;  // sum of the distinct vertices (sentences of L >= 2)
 };
 
 // Draws of every position: the stream of position q of sentence s (counter
@@ -370,7 +371,25 @@ __global__ void sent_count_kernel(DrawArgs a) {
       }
     }
     a.ucount[s - a.s_begin] = (uint8_t)u;
+    if (L >= 2) {
+      // longer sentences: count them here as well
+      int d = u;
+      if (u == 255) {
+        d = 0;
+        for (int i = 0; i < L; ++i) {
+          const int v = a.sen[sb + i];
+          bool seen = false;
+          for (int j = 0; j < i && !seen; ++j) seen = a.sen[sb + j] == v;
+          d += !seen;
+        }
+      }
+      atomicAdd(a.pool_rows, (unsigned long long)d);
+    }
   }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0ull;
 }
 
 struct SentArgs {
@@ -913,6 +932,7 @@ struct dg_sgns_state {
   size_t negs_cap = 0;
   uint8_t* d_ucount = nullptr;  // batched: distinct vertices per sentence of a launch
   size_t ucount_cap = 0;
+  unsigned long long* d_pool_rows = nullptr;  // batched: distinct vertices per sentence, summed
   // sorted scatter-add
   float* d_stage = nullptr;
   size_t stage_cap = 0;
@@ -943,7 +963,7 @@ void dg_sgns_free(dg_ctx* ctx) {
   if (!s) return;
   void* bufs[] = {s->d_rank,   s->d_syn0,  s->d_syn1,   s->d_g0,
                   s->d_g1,     s->d_table, s->d_expt,   s->d_keep,    s->d_sen,   s->d_sen_off,
-                  s->d_cnt,    s->d_shrink, s->d_negs, s->d_ucount, s->d_stage, s->d_k[0],  s->d_k[1],
+                  s->d_cnt,    s->d_shrink, s->d_negs, s->d_ucount, s->d_pool_rows, s->d_stage, s->d_k[0],  s->d_k[1],
                   s->d_v[0],   s->d_v[1],  s->d_hist,   s->d_hoff,    s->d_head,  s->d_hpos,
                   s->d_starts, s->d_tuples, s->d_logits};
   for (void* b : bufs)
@@ -1298,6 +1318,8 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
   float alpha = alpha0;
   DG_CHECK(dg_reserve(ctx, &s->d_cnt, &s->cnt_cap, (size_t)n_walks + 1));
   DG_CHECK(dg_reserve(ctx, &s->d_sen_off, &s->sen_off_cap, (size_t)n_walks + 1));
+  if (!s->d_pool_rows) DG_HIP(ctx, hipMalloc((void**)&s->d_pool_rows, sizeof(unsigned long long)));
+  zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(s->d_pool_rows);
   for (uint32_t ep = 0; ep < p.iter; ++ep) {
     const uint32_t epoch = s->epoch++;
     // subsample (two passes) into ranked sentences
@@ -1385,6 +1407,7 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
         da.shrink = s->d_shrink;
         da.negs = s->d_negs;
         da.ucount = s->d_ucount;
+        da.pool_rows = s->d_pool_rows;
         SentArgs t;
         t.sen = s->d_sen;
         t.sen_off = s->d_sen_off;
@@ -1404,10 +1427,12 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
         t.deterministic = (int)p.deterministic;
         t.all = 0;
         size_t tk;
-        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_fp32b", &tk));
+        DG_CHECK(dg_time_begin(ctx, "sgns_prep", &tk));
         if (n_pos)
           sent_draw_kernel<<<grid_for(ctx, n_pos, 256), 256, 0, ctx->stream>>>(da);
         sent_count_kernel<<<grid_for(ctx, s1 - s0, 256), 256, 0, ctx->stream>>>(da);
+        DG_CHECK(dg_time_end(ctx, tk));
+        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_fp32b", &tk));
         if (p.form == DG_SGNS_BATCHED_BF16)
           launch_sent_form<uint16_t>(ctx, t, s->dim);
         else
@@ -1426,6 +1451,11 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
     if (pe.name.rfind("sgns", 0) == 0) ms += m;
   }
   DG_CHECK(dg_time_resolve(ctx));
+  {
+    unsigned long long pr = 0;
+    DG_HIP(ctx, hipMemcpy(&pr, s->d_pool_rows, sizeof(pr), hipMemcpyDeviceToHost));
+    st.pool_rows = p.form == DG_SGNS_PLAIN_FP32 ? 0 : (uint64_t)pr;
+  }
   st.words = (uint64_t)p.iter * n_tok;
   st.sentences = (uint64_t)p.iter * n_walks;
   st.ms = ms;

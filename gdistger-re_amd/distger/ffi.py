@@ -110,6 +110,7 @@ class SgnsStats(ctypes.Structure):
         ("ms", ctypes.c_double),
         ("alpha_end", ctypes.c_float),
         ("reserved", ctypes.c_float),
+        ("pool_rows", ctypes.c_uint64),
     ]
 
 

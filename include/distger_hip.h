@@ -227,6 +227,8 @@ typedef struct {
   double ms;            /* device time of the training kernels */
   float alpha_end;      /* learning rate of the last launch */
   float reserved;
+  uint64_t pool_rows;   /* batched: input rows held per sentence, summed (distinct
+                           vertices of each sentence; rows loaded and written back once) */
 } dg_sgns_stats;
 
 #define DG_CORPUS_ROUND 0 /* the current round's corpus (one walk file of the reference) */
