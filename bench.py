@@ -57,6 +57,16 @@ def walker_bytes(stats_list):
     return 16 * moves + 8 * draws + 12 * tokens + 12 * walks
 
 
+def walker_transaction_bytes(stats_list):
+    """The walker per SURVEY's secondary-data-path figure: every random access
+    (an adjacency entry drawn, a row-offset pair read) moves one 64-byte
+    transaction, plus 4 B per token written."""
+    tokens = sum(s.tokens for s in stats_list)
+    draws = sum(s.draws for s in stats_list)
+    moves = sum(s.moves for s in stats_list)
+    return 64 * (draws + moves) + 4 * tokens
+
+
 def sgns_sent_bytes(kept, pool_rows, dim, negative):
     """Algorithmic HBM bytes of the sentence kernel: per position its K + 1
     output rows (target, negatives) are read and updated in place (FP32: 4 B
@@ -201,6 +211,8 @@ def main():
         # walker roofline (per launch)
         walk_bytes = walker_bytes(wstats) / n_walk
         walk_gbs = walk_bytes / (walk_ms / n_walk * 1e-3) / 1e9
+        walk_tbytes = walker_transaction_bytes(wstats) / n_walk
+        walk_tgbs = walk_tbytes / (walk_ms / n_walk * 1e-3) / 1e9
         # SGNS BF16 roofline (per launch of the sentence kernels)
         kept = sum(s.kept for s in sstats)
         pool_rows = sum(s.pool_rows for s in sstats)
@@ -210,7 +222,9 @@ def main():
                        "round_steps_per_s": steps / ((walk_ms + pack_ms + re_ms) * 1e-3),
                        "kernel_ms": walk_ms / n_walk, "pack_ms": pack_ms / n_walk,
                        "re_ms": re_ms / n_walk, "bytes_per_launch": walk_bytes,
-                       "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS}
+                       "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS,
+                       "transaction_bytes_per_launch": walk_tbytes,
+                       "transaction_gbs": walk_tgbs, "transaction_frac_hbm": walk_tgbs / HBM_GBS}
         out["sgns_bf16_pipeline"] = {"words_per_s": words / ((bf_ms + prep_ms) * 1e-3),
                                      "kernel_ms_per_launch": bf_ms / n_bf, "launches": n_bf,
                                      "prep_ms_per_launch": prep_ms / n_bf,

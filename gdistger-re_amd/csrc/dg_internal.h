@@ -16,7 +16,7 @@
 // Device-side walk queue and round statistics (one per context).
 struct dg_walk_queue {
   unsigned long long next;    // next walk index to hand out (relative to walker_begin)
-  unsigned long long cursor;  // tokens reserved in the raw token buffer
+  unsigned long long cursor;  // tokens of walks longer than a slot, in the overflow region
   unsigned int flags;         // DG_QFLAG_*
   unsigned int max_len;       // longest walk
   unsigned int zero_terms;    // RE terms with p_i == 0 or q_i == 0
@@ -24,6 +24,7 @@ struct dg_walk_queue {
   double re;                  // relative entropy of the round
   unsigned long long draws;   // candidate draws (adjacency units read)
   unsigned long long moves;   // row-offset pairs read (walk starts + moves)
+  unsigned long long tokens;  // tokens written
 };
 
 #define DG_QFLAG_TOKEN_OVERFLOW 1u

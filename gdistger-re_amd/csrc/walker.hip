@@ -49,8 +49,9 @@ struct WalkArgs {
   uint32_t length_limit;
   uint32_t spill_per_slot;
   double mu;
-  uint32_t* tokens;
+  uint32_t* tokens;   // raw buffer: walker_num slots of kPathLds, then the overflow region
   uint64_t tok_cap;
+  uint64_t slots_end;  // walker_num * kPathLds
   uint64_t* walk_off;
   uint32_t* walk_len;
   uint32_t* spill;
@@ -138,6 +139,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
   uint32_t wid = 0, cur = 0, cur_deg = 0, len = 0;
   uint64_t cur_rb = 0;
   uint32_t ndraw = 0, nmove = 0;
+  // this lane's totals over its walks, added to the queue once per wave at the end
+  uint64_t tot_draw = 0, tot_move = 0, tot_tok = 0;
+  uint32_t max_l = 0;
   bool pending_back = false;
   WalkState ws = {0, 0, 0, 0, 0, 0, 0};
   dg_stream rs;
@@ -258,59 +262,74 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
     }
 
     // ---- write finished walks out, cooperatively per wave ------------------
+    // A walk of up to kPathLds vertices goes to its own slot of the raw buffer
+    // (walk index * kPathLds); longer walks take space after the slots from an
+    // overflow cursor.  No wave waits on a shared counter for a short walk.
     const uint64_t fin = __ballot(finish);
     if (fin != 0) {
-      const uint32_t my_len = finish ? len : 0u;
-      uint32_t incl = my_len;
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(incl, d, 64);
-        if ((int)lane >= d) incl += t;
-      }
-      const uint32_t total = __shfl(incl, 63, 64);
-      const uint64_t draws = wave_sum_u64(finish ? ndraw : 0u);
-      const uint64_t moves = wave_sum_u64(finish ? nmove : 0u);
-      unsigned long long base = 0;
-      if (lane == 0) {
-        base = atomicAdd(&a.q->cursor, (unsigned long long)total);
-        atomicAdd(&a.q->draws, (unsigned long long)draws);
-        atomicAdd(&a.q->moves, (unsigned long long)moves);
-      }
-      base = __shfl(base, 0, 64);
-      const bool fits = base + total <= a.tok_cap;
-      const uint64_t my_off = base + incl - my_len;
+      uint64_t my_off = 0;
+      bool fits = true;
       if (finish) {
+        tot_draw += ndraw;
+        tot_move += nmove;
+        tot_tok += len;
+        max_l = len > max_l ? len : max_l;
+        if (len <= (uint32_t)kPathLds) {
+          my_off = (uint64_t)wid * kPathLds;
+        } else {
+          my_off = a.slots_end + atomicAdd(&a.q->cursor, (unsigned long long)len);
+          fits = my_off + len <= a.tok_cap;
+          if (!fits) atomicOr(&a.q->flags, DG_QFLAG_TOKEN_OVERFLOW);
+        }
         a.walk_off[wid] = my_off;
         a.walk_len[wid] = len;
-        atomicMax(&a.q->max_len, len);
       }
-      if (!fits) {
-        if (lane == 0) atomicOr(&a.q->flags, DG_QFLAG_TOKEN_OVERFLOW);
-      } else {
-        __builtin_amdgcn_wave_barrier();
-        uint64_t mm = fin;
-        const uint32_t* wave_path = s_path + (threadIdx.x - lane) * kPathLds;
-        while (mm != 0) {
-          const int L = __builtin_ctzll(mm);
-          mm &= mm - 1;
-          const uint32_t n = __shfl(len, L, 64);
-          const unsigned long long off = __shfl((unsigned long long)my_off, L, 64);
-          const uint32_t nl = n < (uint32_t)kPathLds ? n : (uint32_t)kPathLds;
-          const uint32_t* src = wave_path + L * kPathLds;
-          for (uint32_t j = lane; j < nl; j += 64) {
-            const uint32_t v = src[j];
-            a.tokens[off + j] = v;
-            atomicAdd(&a.freq[v], 1u);  // path.hpp:91,95 (context_map_freq, vertex_cn)
-          }
+      __builtin_amdgcn_wave_barrier();
+      uint64_t mm = fin & __ballot(fits);
+      const uint32_t* wave_path = s_path + (threadIdx.x - lane) * kPathLds;
+      while (mm != 0) {
+        const int L = __builtin_ctzll(mm);
+        mm &= mm - 1;
+        const uint32_t n = __shfl(len, L, 64);
+        const unsigned long long off = __shfl((unsigned long long)my_off, L, 64);
+        const uint32_t nl = n < (uint32_t)kPathLds ? n : (uint32_t)kPathLds;
+        const uint32_t* src = wave_path + L * kPathLds;
+        // lane j < nl holds token j (nl <= kPathLds < 64)
+        const uint32_t v = (uint32_t)lane < nl ? src[lane] : 0xffffffffu;
+        if ((uint32_t)lane < nl) a.tokens[off + lane] = v;
+        // visit counts (path.hpp:91,95, context_map_freq / vertex_cn): one
+        // atomic per distinct vertex of the walk, adding its multiplicity
+        // (walks revisit their hub at every other step)
+        uint64_t rem = __ballot((uint32_t)lane < nl);
+        while (rem != 0) {
+          const int ld = __builtin_ctzll(rem);
+          const uint32_t vl = __shfl(v, ld, 64);
+          const uint64_t same = __ballot(v == vl) & rem;
+          if (lane == ld) atomicAdd(&a.freq[vl], (uint32_t)__popcll(same));
+          rem &= ~same;
         }
-        if (finish)  // the spilled tail, by its own lane
-          for (uint32_t j = kPathLds; j < len; ++j) {
-            const uint32_t v = spill[j - kPathLds];
-            a.tokens[my_off + j] = v;
-            atomicAdd(&a.freq[v], 1u);
-          }
       }
+      if (finish && fits)  // the spilled tail, by its own lane
+        for (uint32_t j = kPathLds; j < len; ++j) {
+          const uint32_t v = spill[j - kPathLds];
+          a.tokens[my_off + j] = v;
+          atomicAdd(&a.freq[v], 1u);
+        }
       if (finish) active = false;
     }
+  }
+  // ---- the lane totals, one atomic each per wave
+  const uint64_t wd = wave_sum_u64(tot_draw), wm = wave_sum_u64(tot_move), wt = wave_sum_u64(tot_tok);
+  uint32_t wmax = max_l;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(wmax, d, 64);
+    wmax = o > wmax ? o : wmax;
+  }
+  if (lane == 0) {
+    if (wd) atomicAdd(&a.q->draws, (unsigned long long)wd);
+    if (wm) atomicAdd(&a.q->moves, (unsigned long long)wm);
+    if (wt) atomicAdd(&a.q->tokens, (unsigned long long)wt);
+    if (wmax) atomicMax(&a.q->max_len, wmax);
   }
 }
 
@@ -324,6 +343,7 @@ __global__ void queue_reset_kernel(dg_walk_queue* q) {
     q->re = 0.0;
     q->draws = 0;
     q->moves = 0;
+    q->tokens = 0;
   }
 }
 
@@ -616,8 +636,9 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   DG_CHECK(dg_reserve(ctx, &ctx->d_offsets, &ctx->offsets_cap, (size_t)walker_num + 1));
   DG_CHECK(dg_reserve(ctx, &ctx->d_spill, &ctx->spill_cap,
                       (size_t)blocks * kWalkBlock * spill_per_slot));
-  const uint32_t est_len = p->max_length ? (p->max_length < 48 ? p->max_length : 48) : 32;
-  size_t tok_need = (size_t)walker_num * est_len;
+  // raw buffer: one slot of kPathLds tokens per walk, then room for longer walks
+  const uint64_t slots_end = (uint64_t)walker_num * kPathLds;
+  size_t tok_need = (size_t)slots_end + (size_t)walker_num / 8 + 4096;
   if (ctx->tok_raw_cap < tok_need) DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, tok_need));
 
   WalkArgs a;
@@ -644,6 +665,7 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   a.spill = ctx->d_spill;
   a.q = ctx->d_queue;
 
+  a.slots_end = slots_end;
   for (int attempt = 0; attempt < 3; ++attempt) {
     a.tokens = ctx->d_tok_raw;
     a.tok_cap = ctx->tok_raw_cap;
@@ -663,11 +685,12 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
                      "round %u: a walk reached the length limit %u (raise length_limit)", round,
                      length_limit);
     if (!(ctx->h_queue->flags & DG_QFLAG_TOKEN_OVERFLOW)) break;
-    const size_t need = (size_t)(ctx->h_queue->cursor + ctx->h_queue->cursor / 8 + 1024);
+    const size_t need =
+        (size_t)(slots_end + ctx->h_queue->cursor + ctx->h_queue->cursor / 8 + 1024);
     DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, need));
     if (attempt == 2) return dg_fail(ctx, DG_ERR_STATE, "token buffer sizing did not converge");
   }
-  const uint64_t n_tok = ctx->h_queue->cursor;
+  const uint64_t n_tok = ctx->h_queue->tokens;
   const uint32_t max_len = ctx->h_queue->max_len;
   const uint64_t draws = ctx->h_queue->draws;
   const uint64_t moves = ctx->h_queue->moves;
