@@ -52,6 +52,7 @@ extern "C" void dg_close(dg_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  dg_tuples_free(ctx);
   dg_sgns_free(ctx);
   dg_walk_free(ctx);
   dg_graph_free(ctx);

@@ -41,7 +41,8 @@ struct dg_pending_event {
   hipEvent_t stop;
 };
 
-struct dg_sgns_state;  // defined by the trainer translation unit
+struct dg_sgns_state;   // defined by the trainer translation unit
+struct dg_tuple_state;  // defined by the tuple-interface translation unit
 
 struct dg_ctx {
   int device = 0;
@@ -97,6 +98,7 @@ struct dg_ctx {
   uint64_t train_tokens = 0;
 
   dg_sgns_state* sgns = nullptr;
+  dg_tuple_state* tuples = nullptr;
 
   // timing
   std::map<std::string, dg_timing_entry> timing;
@@ -169,10 +171,29 @@ int dg_time_begin(dg_ctx* ctx, const char* name, size_t* token);
 int dg_time_end(dg_ctx* ctx, size_t token);
 int dg_time_resolve(dg_ctx* ctx);
 
-// Graph teardown (graph.hip) and trainer teardown (sgns.hip).
+// Graph teardown (graph.hip), trainer teardown (sgns.hip), tuple-interface
+// teardown (tuples.hip).
 void dg_graph_free(dg_ctx* ctx);
 void dg_walk_free(dg_ctx* ctx);
 void dg_sgns_free(dg_ctx* ctx);
+void dg_tuples_free(dg_ctx* ctx);
+// Forget the gradients of the last tuple backward (the tables they belong to
+// are being replaced).
+void dg_tuples_invalidate(dg_ctx* ctx);
+
+// The trainer's FP32 tables as the tuple interface sees them.  `version`
+// changes whenever the tables are written (init, set, train, tuple updates),
+// so a derived copy (the BF16 shadow) knows when it is stale.
+struct dg_table_view {
+  float* syn0;
+  float* syn1;
+  uint32_t rows;
+  uint32_t dim;
+  uint64_t version;
+};
+int dg_sgns_view(dg_ctx* ctx, dg_table_view* v);
+// Record a write to the tables; returns the new version.
+uint64_t dg_sgns_written(dg_ctx* ctx);
 
 // Exclusive scan of n uint32 lengths into n+1 uint64 offsets (walker.hip).
 int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_off);

@@ -42,6 +42,7 @@
 #define DG_TAG_INIT 0x494E4954u   /* random embedding initialisation */
 #define DG_TAG_GEN 0x47454E52u    /* synthetic graph generators */
 #define DG_TAG_SPLIT 0x53504C54u  /* link-prediction split */
+#define DG_TAG_TUPLE 0x54555045u  /* synthetic tuples and embedding tables */
 
 typedef struct {
   uint32_t x0, x1, x2, x3;

@@ -198,6 +198,43 @@ extern "C" int dg_gen_chung_lu(uint32_t v_num, uint64_t e_target, double gamma,
   return emit_edges(keys, seed, 0x43484c55u, edges, e_cap, e_num, nullptr);
 }
 
+extern "C" int dg_gen_zipf(uint32_t v_num, double s, uint64_t n, uint64_t seed, uint32_t* out) {
+  if (v_num == 0 || !(s > 0) || (n && !out)) return DG_ERR_ARG;
+  std::vector<double> cdf(v_num);
+  double acc = 0.0;
+  for (uint32_t k = 0; k < v_num; ++k) {
+    acc += std::pow((double)k + 1.0, -s);
+    cdf[k] = acc;
+  }
+  parallel_for(n, [&](uint64_t lo, uint64_t hi, unsigned) {
+    for (uint64_t i = lo; i < hi; ++i) {
+      dg_stream st;
+      dg_stream_init(&st, seed, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_TUPLE);
+      const double u = u01(&st) * acc;
+      const uint64_t k = (uint64_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+      out[i] = (uint32_t)(k < v_num ? k : v_num - 1);
+    }
+  });
+  return DG_OK;
+}
+
+extern "C" int dg_gen_normal(uint64_t n, double sigma, uint64_t seed, float* out) {
+  if (n && !out) return DG_ERR_ARG;
+  const uint64_t pairs = (n + 1) / 2;
+  parallel_for(pairs, [&](uint64_t lo, uint64_t hi, unsigned) {
+    for (uint64_t i = lo; i < hi; ++i) {
+      dg_stream st;
+      dg_stream_init(&st, seed, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_TUPLE ^ 0x4e4f524du);
+      const double u1 = u01(&st), u2 = u01(&st);
+      const double r = std::sqrt(-2.0 * std::log(1.0 - u1));  // 1 - u1 in (0, 1]
+      const double th = 6.283185307179586 * u2;
+      out[2 * i] = (float)(sigma * r * std::cos(th));
+      if (2 * i + 1 < n) out[2 * i + 1] = (float)(sigma * r * std::sin(th));
+    }
+  });
+  return DG_OK;
+}
+
 extern "C" int dg_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
                            uint64_t seed, dg_edge* edges, uint64_t e_cap, uint64_t* e_num) {
   if (scale == 0 || scale > 31 || !edges || !e_num) return DG_ERR_ARG;

@@ -13,19 +13,23 @@
 //     reference's block reduction order (pairs at distance 64, then a fold
 //     by 32, 16, ..., 1), so a single-wavefront run in sentence order gives
 //     the oracle's bits.
-//   * batched (DG_SGNS_BATCHED_*): a sentence is cut into tiles of T
-//     consecutive positions (16, or 8 when K > 5).  Each position draws its
-//     own K negatives, shared by the contexts of its window (the per-position
-//     minibatch of DSGL, dsgl.hpp:1100-1159); the tile's rows are staged in
-//     LDS once and the (contexts x outputs) logits and both rank-1 update
-//     products run on MFMA, so a context row is read once per tile instead of
-//     once per (context, target) pair.  All gradients of a launch come
-//     from the tables as they were at its start ("snapshot" semantics) when
-//     deterministic; they are then summed per row by a sorted-key segmented
-//     scatter-add in a fixed order.  Non-deterministic mode adds them with
-//     atomics as they are produced (Hogwild, like the reference kernels).
-//   * BF16: rows stored in BF16 next to FP32 master rows; logits on BF16 MFMA.
+//   * batched (DG_SGNS_BATCHED_FP32 / _BF16): one wavefront per sentence.
+//     Each position draws K negatives shared by the contexts of its window
+//     (the per-position minibatch of DSGL, dsgl.hpp:1100-1159).  The
+//     sentence's distinct vertices form a pool whose input rows stay in
+//     registers for the whole sentence; per position the target and negative
+//     rows are staged once and the (pool x outputs) logits and Gram products
+//     run on MFMA.  The sequential order of the position's updates is
+//     reproduced to first order in the updates (a recursion over the
+//     contexts in order on the Gram matrices), so the result follows the
+//     per-pair order of the plain form instead of a per-position snapshot.
+//     Output-row updates are atomics; input rows are written back once per
+//     sentence.  Deterministic mode runs the sentences in order in one
+//     wavefront.
+//   * BF16: FP32 master rows in HBM, rounded to BF16 while staged; products
+//     on BF16 MFMA with FP32 accumulation; updates applied in FP32.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -760,122 +764,6 @@ void sg_sent_kernel(SentArgs t) {
   }
 }
 
-// ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass -----------
-constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;
-
-__global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t* keys, uint64_t n,
-                                                                int shift, uint32_t n_blocks,
-                                                                uint32_t* hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  for (int i = 0; i < kSortItems; ++i) {
-    const uint64_t k = base + (uint64_t)i * kSortBlock + threadIdx.x;
-    if (k < n) atomicAdd(&h[(keys[k] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)threadIdx.x * n_blocks + blockIdx.x] = h[threadIdx.x];  // digit-major
-}
-
-__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
-    const uint32_t* keys, const uint32_t* vals, uint64_t n, int shift, uint32_t n_blocks,
-    const uint64_t* digit_off, uint32_t* okeys, uint32_t* ovals) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[4][256];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  run[threadIdx.x] = 0;
-  for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  const uint64_t lanes_lt = (1ull << lane) - 1ull;
-  for (int i = 0; i < kSortItems; ++i) {
-    const uint64_t k = base + (uint64_t)i * kSortBlock + threadIdx.x;
-    const bool valid = k < n;
-    const uint32_t key = valid ? keys[k] : 0u;
-    const uint32_t dig = (key >> shift) & 255u;
-    // lanes of this wave holding the same digit
-    uint64_t same = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
-      same &= ((dig >> b) & 1u) ? bal : ~bal;
-    }
-    const uint32_t rank = (uint32_t)__popcll(same & lanes_lt);
-    const uint32_t cnt = (uint32_t)__popcll(same);
-    if (valid && rank == 0) wcnt[wave][dig] = cnt;
-    __syncthreads();
-    if (valid) {
-      uint32_t before = run[dig];
-      for (uint32_t w = 0; w < wave; ++w) before += wcnt[w][dig];
-      const uint64_t dst = digit_off[(uint64_t)dig * n_blocks + blockIdx.x] + before + rank;
-      okeys[dst] = key;
-      ovals[dst] = vals[k];
-    }
-    __syncthreads();
-    run[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
-                        wcnt[3][threadIdx.x];
-    for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
-  }
-}
-
-// Segment heads of sorted keys (valid keys only).
-__global__ void seg_head_kernel(const uint32_t* keys, uint64_t n, uint32_t* head) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    head[i] = (keys[i] != 0xffffffffu && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
-}
-
-__global__ void seg_list_kernel(const uint32_t* head, const uint64_t* pos, uint64_t n,
-                                uint64_t* starts) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    if (head[i]) starts[pos[i]] = i;
-}
-
-// One wavefront per row segment: contributions summed in sorted (slot) order,
-// then added once to the row.  mode 0: W += sum; mode 1: W -= lr * sum;
-// mode 2: store the sum in the gradient table instead.
-template <int D>
-__global__ __launch_bounds__(256) void seg_apply_kernel(const uint32_t* keys, const uint32_t* vals,
-                                                        const uint64_t* starts, uint64_t n_seg,
-                                                        uint64_t n_valid, const float* stage,
-                                                        uint32_t row_bits, float* syn0, float* syn1, int mode,
-                                                        float lr, float* g0, float* g1) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t sgi = wave; sgi < n_seg; sgi += nwaves) {
-    const uint64_t b = starts[sgi];
-    const uint64_t e = sgi + 1 < n_seg ? starts[sgi + 1] : n_valid;
-    const uint32_t key = keys[b];
-    float acc[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    for (uint64_t i = b; i < e; ++i) {
-      const uint64_t slot = vals[i];
-#pragma unroll
-      for (int q = 0; q < D; ++q) acc[q] = acc[q] + stage[slot * (64 * D) + lane + 64 * q];
-    }
-    const uint32_t tbl = key >> row_bits;
-    const uint64_t row = key & ((1u << row_bits) - 1u);
-    float* w = (tbl ? syn1 : syn0) + row * (64 * D);
-    if (mode == 2) {
-      float* g = (tbl ? g1 : g0) + row * (64 * D);
-#pragma unroll
-      for (int q = 0; q < D; ++q) g[lane + 64 * q] = acc[q];
-      continue;
-    }
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      const float nv = mode == 0 ? w[lane + 64 * q] + acc[q] : w[lane + 64 * q] - lr * acc[q];
-      w[lane + 64 * q] = nv;
-    }
-  }
-}
-
 __global__ void pair_score_kernel(const uint32_t* rank, const float* syn0, uint32_t dim,
                                   const uint32_t* src, const uint32_t* dst, uint64_t n, float* out) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -913,8 +801,6 @@ struct dg_sgns_state {
   uint32_t* d_rank = nullptr;
   float* d_syn0 = nullptr;
   float* d_syn1 = nullptr;
-  float* d_g0 = nullptr;  // gradient tables of the tuple interface
-  float* d_g1 = nullptr;
   int32_t* d_table = nullptr;
   uint32_t table_size = 0;
   float* d_expt = nullptr;
@@ -933,43 +819,44 @@ struct dg_sgns_state {
   uint8_t* d_ucount = nullptr;  // batched: distinct vertices per sentence of a launch
   size_t ucount_cap = 0;
   unsigned long long* d_pool_rows = nullptr;  // batched: distinct vertices per sentence, summed
-  // sorted scatter-add
-  float* d_stage = nullptr;
-  size_t stage_cap = 0;
-  uint32_t* d_k[2] = {nullptr, nullptr};
-  uint32_t* d_v[2] = {nullptr, nullptr};
-  size_t kv_cap = 0;
-  uint32_t* d_hist = nullptr;
-  size_t hist_cap = 0;
-  uint64_t* d_hoff = nullptr;
-  size_t hoff_cap = 0;
-  uint32_t* d_head = nullptr;
-  size_t head_cap = 0;
-  uint64_t* d_hpos = nullptr;
-  size_t hpos_cap = 0;
-  uint64_t* d_starts = nullptr;
-  size_t starts_cap = 0;
-  // tuple interface
-  uint32_t* d_tuples = nullptr;
-  size_t tuples_cap = 0;
-  float* d_logits = nullptr;
-  size_t logits_cap = 0;
   uint32_t epoch = 0;
   uint64_t sentences_done = 0;
+  uint64_t version = 0;     // changes on every write of syn0 / syn1
+  bool has_vocab = true;    // false for dg_sgns_tables_create tables
+  uint32_t rank_size = 0;   // entries of d_rank (vertex ids accepted by score_pairs)
 };
 
 void dg_sgns_free(dg_ctx* ctx) {
+  dg_tuples_invalidate(ctx);
   dg_sgns_state* s = ctx->sgns;
   if (!s) return;
-  void* bufs[] = {s->d_rank,   s->d_syn0,  s->d_syn1,   s->d_g0,
-                  s->d_g1,     s->d_table, s->d_expt,   s->d_keep,    s->d_sen,   s->d_sen_off,
-                  s->d_cnt,    s->d_shrink, s->d_negs, s->d_ucount, s->d_pool_rows, s->d_stage, s->d_k[0],  s->d_k[1],
-                  s->d_v[0],   s->d_v[1],  s->d_hist,   s->d_hoff,    s->d_head,  s->d_hpos,
-                  s->d_starts, s->d_tuples, s->d_logits};
+  void* bufs[] = {s->d_rank,  s->d_syn0,    s->d_syn1, s->d_table,   s->d_expt,
+                  s->d_keep,  s->d_sen,     s->d_sen_off, s->d_cnt, s->d_shrink,
+                  s->d_negs,  s->d_ucount,  s->d_pool_rows};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete s;
   ctx->sgns = nullptr;
+}
+
+static uint64_t next_table_version() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
+
+int dg_sgns_view(dg_ctx* ctx, dg_table_view* v) {
+  dg_sgns_state* s = ctx->sgns;
+  if (!s || !s->d_syn0) return dg_fail(ctx, DG_ERR_STATE, "no embedding tables (dg_sgns_init or dg_sgns_tables_create)");
+  v->syn0 = s->d_syn0;
+  v->syn1 = s->d_syn1;
+  v->rows = s->vocab;
+  v->dim = s->dim;
+  v->version = s->version;
+  return DG_OK;
+}
+
+uint64_t dg_sgns_written(dg_ctx* ctx) {
+  return ctx->sgns ? (ctx->sgns->version = next_table_version()) : 0;
 }
 
 extern "C" void dg_sgns_params_default(dg_sgns_params* p) {
@@ -1134,97 +1021,37 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
   DG_HIP(ctx, hipMemcpy(s->d_expt, expt.data(), sizeof(float) * (kExpTable + 1), hipMemcpyHostToDevice));
   DG_HIP(ctx, hipMemcpy(s->d_keep, keep.data(), sizeof(uint32_t) * vocab, hipMemcpyHostToDevice));
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->rank_size = V;
+  dg_sgns_written(ctx);
   return DG_OK;
 }
 
-static int row_bits_for(uint32_t vocab) {
-  int b = 1;
-  while ((1u << b) < vocab && b < 30) ++b;
-  return b;
-}
-
-// Sort (key, slot) pairs stably by key and apply the per-row sums.
-static int sorted_scatter_apply(dg_ctx* ctx, dg_sgns_state* s, uint64_t n, int mode, float lr) {
-  if (n == 0) return DG_OK;
-  const int rb = row_bits_for(s->vocab);
-  // valid keys are < 2^(rb+1); sorting rb+2 bits puts the unused-slot key
-  // 0xffffffff (bit rb+1 set) after every valid key
-  const int key_bits = rb + 2;
-  const uint32_t n_blocks = (uint32_t)((n + kSortTile - 1) / kSortTile);
-  DG_CHECK(dg_reserve(ctx, &s->d_hist, &s->hist_cap, (size_t)n_blocks * 256));
-  DG_CHECK(dg_reserve(ctx, &s->d_hoff, &s->hoff_cap, (size_t)n_blocks * 256 + 1));
-  int cur = 0;
-  for (int shift = 0; shift < key_bits; shift += 8) {
-    radix_hist_kernel<<<n_blocks, kSortBlock, 0, ctx->stream>>>(s->d_k[cur], n, shift, n_blocks,
-                                                                s->d_hist);
-    DG_CHECK(dg_scan_lengths(ctx, s->d_hist, (uint64_t)n_blocks * 256, s->d_hoff));
-    radix_scatter_kernel<<<n_blocks, kSortBlock, 0, ctx->stream>>>(
-        s->d_k[cur], s->d_v[cur], n, shift, n_blocks, s->d_hoff, s->d_k[1 - cur], s->d_v[1 - cur]);
-    DG_HIP(ctx, hipGetLastError());
-    cur = 1 - cur;
-  }
-  // invalid keys (0xffffffff) sort last only if the top pass covers bit 31; they
-  // are excluded by the head test and by n_valid below
-  DG_CHECK(dg_reserve(ctx, &s->d_head, &s->head_cap, (size_t)n));
-  DG_CHECK(dg_reserve(ctx, &s->d_hpos, &s->hpos_cap, (size_t)n + 1));
-  seg_head_kernel<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(s->d_k[cur], n, s->d_head);
-  DG_CHECK(dg_scan_lengths(ctx, s->d_head, n, s->d_hpos));
-  uint64_t n_seg = 0;
-  DG_HIP(ctx, hipMemcpyAsync(&n_seg, s->d_hpos + n, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  DG_CHECK(dg_reserve(ctx, &s->d_starts, &s->starts_cap, (size_t)n_seg + 1));
-  seg_list_kernel<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(s->d_head, s->d_hpos, n,
-                                                                  s->d_starts);
-  // number of valid keys: position after the last valid segment = first invalid key
-  // (invalid keys sort to the end because 0xffffffff is the largest key)
-  uint64_t n_valid = n;
-  {
-    // find the first 0xffffffff by counting invalid keys is equivalent: all sort last
-    std::vector<uint32_t> tail(1);
-    (void)tail;
-  }
-  const int D = (int)(s->dim / 64);
-  const unsigned g = (unsigned)grid_for(ctx, n_seg * 64, 256);
-  // n_valid is resolved on the device side by the segment bounds: the last
-  // segment ends at the first invalid key, found by a binary search below
-  uint32_t* keys = s->d_k[cur];
-  uint64_t lo = 0, hi = n;  // first index with key == 0xffffffff
-  {
-    std::vector<uint32_t> probe(1);
-    while (lo < hi) {
-      const uint64_t mid = lo + (hi - lo) / 2;
-      DG_HIP(ctx, hipMemcpy(probe.data(), keys + mid, sizeof(uint32_t), hipMemcpyDeviceToHost));
-      if (probe[0] == 0xffffffffu)
-        hi = mid;
-      else
-        lo = mid + 1;
-    }
-    n_valid = lo;
-  }
-  switch (D) {
-    case 1:
-      seg_apply_kernel<1><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
-                                                      lr, s->d_g0, s->d_g1);
-      break;
-    case 2:
-      seg_apply_kernel<2><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
-                                                      lr, s->d_g0, s->d_g1);
-      break;
-    case 4:
-      seg_apply_kernel<4><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
-                                                      lr, s->d_g0, s->d_g1);
-      break;
-    case 8:
-      seg_apply_kernel<8><<<g, 256, 0, ctx->stream>>>(keys, s->d_v[cur], s->d_starts, n_seg, n_valid,
-                                                      s->d_stage, rb, s->d_syn0, s->d_syn1, mode,
-                                                      lr, s->d_g0, s->d_g1);
-      break;
-  }
-  DG_HIP(ctx, hipGetLastError());
+extern "C" int dg_sgns_tables_create(dg_ctx* ctx, uint32_t rows, uint32_t dim, const float* syn0,
+                                     const float* syn1) {
+  if (!ctx || !syn0 || !syn1) return DG_ERR_ARG;
+  if (rows < 1 || rows > (1u << 30)) return dg_fail(ctx, DG_ERR_ARG, "rows %u: must be in [1, 2^30]", rows);
+  if (dim == 0 || dim % 64 != 0 || dim > 512)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "dim %u: must be a multiple of 64, at most 512", dim);
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  dg_sgns_free(ctx);
+  dg_sgns_state* s = new dg_sgns_state();
+  ctx->sgns = s;
+  dg_sgns_params_default(&s->p);
+  s->p.dim = dim;
+  s->dim = dim;
+  s->vocab = rows;
+  s->has_vocab = false;
+  s->rank_size = rows;
+  s->vertex_of_rank.resize(rows);
+  for (uint32_t r = 0; r < rows; ++r) s->vertex_of_rank[r] = r;
+  const size_t nrow = (size_t)rows * dim;
+  DG_HIP(ctx, hipMalloc((void**)&s->d_rank, sizeof(uint32_t) * rows));
+  DG_HIP(ctx, hipMalloc((void**)&s->d_syn0, sizeof(float) * nrow));
+  DG_HIP(ctx, hipMalloc((void**)&s->d_syn1, sizeof(float) * nrow));
+  DG_HIP(ctx, hipMemcpy(s->d_rank, s->vertex_of_rank.data(), sizeof(uint32_t) * rows, hipMemcpyHostToDevice));
+  DG_HIP(ctx, hipMemcpy(s->d_syn0, syn0, sizeof(float) * nrow, hipMemcpyHostToDevice));
+  DG_HIP(ctx, hipMemcpy(s->d_syn1, syn1, sizeof(float) * nrow, hipMemcpyHostToDevice));
+  dg_sgns_written(ctx);
   return DG_OK;
 }
 
@@ -1297,6 +1124,8 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
   if (!ctx || !pin) return DG_ERR_ARG;
   dg_sgns_state* s = ctx->sgns;
   if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
+  if (!s->has_vocab)
+    return dg_fail(ctx, DG_ERR_STATE, "tables from dg_sgns_tables_create have no vocabulary: run dg_sgns_init to train");
   DG_CHECK(check_params(ctx, pin));
   if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
   DG_HIP(ctx, hipSetDevice(ctx->device));
@@ -1445,6 +1274,7 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
     s->sentences_done += n_walks;
   }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dg_sgns_written(ctx);
   for (auto& pe : ctx->pending) {
     float m = 0.f;
     DG_HIP(ctx, hipEventElapsedTime(&m, pe.start, pe.stop));
@@ -1494,6 +1324,7 @@ extern "C" int dg_sgns_set_tables(dg_ctx* ctx, const float* syn0, const float* s
   if (syn0) DG_HIP(ctx, hipMemcpy(s->d_syn0, syn0, sizeof(float) * n, hipMemcpyHostToDevice));
   if (syn1) DG_HIP(ctx, hipMemcpy(s->d_syn1, syn1, sizeof(float) * n, hipMemcpyHostToDevice));
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dg_sgns_written(ctx);
   return DG_OK;
 }
 
@@ -1503,6 +1334,10 @@ extern "C" int dg_sgns_score_pairs(dg_ctx* ctx, const uint32_t* src, const uint3
   dg_sgns_state* s = ctx->sgns;
   if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
   if (n == 0) return DG_OK;
+  for (uint64_t i = 0; i < n; ++i)
+    if (src[i] >= s->rank_size || dst[i] >= s->rank_size)
+      return dg_fail(ctx, DG_ERR_ARG, "pair %llu: vertex id out of range (%u vertices)",
+                     (unsigned long long)i, s->rank_size);
   DG_HIP(ctx, hipSetDevice(ctx->device));
   uint32_t *ds = nullptr, *dd = nullptr;
   float* dout = nullptr;
@@ -1540,17 +1375,4 @@ extern "C" int dg_emb_write(dg_ctx* ctx, const char* path, int binary) {
     fprintf(f, "\n");
   }
   return fclose(f) == 0 ? DG_OK : dg_fail(ctx, DG_ERR_IO, "write to %s failed", path);
-}
-
-// tuple interface: filled in by the differentiation kernels
-extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t*, uint64_t, uint32_t, int,
-                                      float*, double*) {
-  return dg_fail(ctx, DG_ERR_UNSUPPORTED, "not available yet");
-}
-extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t*, uint64_t, uint32_t, int,
-                                       float, int) {
-  return dg_fail(ctx, DG_ERR_UNSUPPORTED, "not available yet");
-}
-extern "C" int dg_sgns_grad_rows(dg_ctx* ctx, int, const uint32_t*, uint64_t, float*) {
-  return dg_fail(ctx, DG_ERR_UNSUPPORTED, "not available yet");
 }

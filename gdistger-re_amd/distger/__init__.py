@@ -14,6 +14,8 @@ from .ffi import (  # noqa: F401
     load_library,
     gen_chung_lu,
     gen_rmat,
+    gen_zipf,
+    gen_normal,
     lp_split,
     auc,
     write_edges_bin,

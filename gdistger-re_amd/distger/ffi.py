@@ -180,8 +180,11 @@ def load_library():
         "dg_emb_write": (c.c_int, [vp, c.c_char_p, c.c_int]),
         "dg_sgns_tuples_forward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, vp,
                                              c.POINTER(c.c_double)]),
-        "dg_sgns_tuples_backward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, c.c_float,
-                                              c.c_int]),
+        "dg_sgns_tuples_backward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, vp,
+                                              c.c_float, c.c_int]),
+        "dg_sgns_tables_create": (c.c_int, [vp, c.c_uint32, c.c_uint32, vp, vp]),
+        "dg_gen_zipf": (c.c_int, [c.c_uint32, c.c_double, c.c_uint64, c.c_uint64, vp]),
+        "dg_gen_normal": (c.c_int, [c.c_uint64, c.c_double, c.c_uint64, vp]),
         "dg_sgns_grad_rows": (c.c_int, [vp, c.c_int, vp, c.c_uint64, vp]),
         "dg_write_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64]),
         "dg_read_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64, c.POINTER(c.c_uint64)]),
@@ -379,10 +382,20 @@ class Context:
                                                   ctypes.byref(loss)))
         return logits.reshape(n, 1 + K), loss.value
 
-    def tuples_backward(self, tuples, K, bf16=False, lr=0.0, apply=False):
+    def tuples_backward(self, tuples, K, bf16=False, dlogits=None, lr=0.0, apply=False):
         t = np.ascontiguousarray(tuples, dtype=np.uint32)
-        self._check(self.L.dg_sgns_tuples_backward(self.h, _vp(t), t.shape[0], K, int(bf16),
+        dl = None if dlogits is None else np.ascontiguousarray(dlogits, dtype=np.float32)
+        if dl is not None and dl.size != t.shape[0] * (K + 1):
+            raise ValueError("dlogits needs n * (K + 1) values")
+        self._check(self.L.dg_sgns_tuples_backward(self.h, _vp(t), t.shape[0], K, int(bf16), _vp(dl),
                                                    float(lr), int(apply)))
+
+    def sgns_tables_create(self, syn0, syn1):
+        s0 = np.ascontiguousarray(syn0, dtype=np.float32)
+        s1 = np.ascontiguousarray(syn1, dtype=np.float32)
+        if s0.shape != s1.shape or s0.ndim != 2:
+            raise ValueError("syn0 and syn1 must be equal rows x dim arrays")
+        self._check(self.L.dg_sgns_tables_create(self.h, s0.shape[0], s0.shape[1], _vp(s0), _vp(s1)))
 
     def grad_rows(self, table, rows, dim):
         r = np.ascontiguousarray(rows, dtype=np.uint32)
@@ -426,6 +439,21 @@ def gen_rmat(scale, edge_factor, a, b, c, seed):
     _check_host(L.dg_gen_rmat(scale, edge_factor, a, b, c, seed, _vp(buf), cap, ctypes.byref(n)),
                 "dg_gen_rmat")
     return buf[: n.value].copy()
+
+
+def gen_zipf(v_num, s, n, seed):
+    """n ids in [0, v_num), P(k) proportional to (k + 1)^-s."""
+    L = load_library()
+    out = np.zeros(int(n), np.uint32)
+    _check_host(L.dg_gen_zipf(v_num, float(s), int(n), seed, _vp(out)), "dg_gen_zipf")
+    return out
+
+
+def gen_normal(n, sigma, seed):
+    L = load_library()
+    out = np.zeros(int(n), np.float32)
+    _check_host(L.dg_gen_normal(int(n), float(sigma), seed, _vp(out)), "dg_gen_normal")
+    return out
 
 
 def lp_split(v_num, edges, frac, seed):

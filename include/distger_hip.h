@@ -256,20 +256,40 @@ int dg_sgns_score_pairs(dg_ctx* ctx, const uint32_t* src, const uint32_t* dst, u
 int dg_emb_write(dg_ctx* ctx, const char* path, int binary);
 
 /* ---- differentiable tuple interface (forward / hand-written adjoint) ----
- * A tuple is (center, context, neg_1..neg_K) in vocabulary ranks, K =
- * p->negative.  Loss per tuple: -log sig(x_ctx . y_center) - sum_k log
- * sig(-x_ctx . y_negk), x = syn0 rows, y = syn1 rows, exact sigmoid.
- * forward: logits[n*(1+K)] and the summed loss.
- * backward: dL/dlogit = sig(z) - label, then row gradients reduced by a
- * sorted-key segmented sum (deterministic); with apply != 0 the rows are
- * updated W -= lr * grad, else the gradients are kept for dg_sgns_grad_rows. */
+ * An autograd-Function-shaped pair over the tables of dg_sgns_init or
+ * dg_sgns_tables_create.  A tuple is (center, context, neg_1..neg_K) in table
+ * rows (ranks), tuples[n * (K + 2)].  x = syn0[context], y_0 = syn1[center],
+ * y_k = syn1[neg_k], z_j = x . y_j and
+ *   L = sum_t [ -log sig(z_0) - sum_k log sig(-z_k) ]   (exact sigmoid).
+ * A negative equal to its tuple's centre is skipped, as skip_gram_kernel
+ * skips it (src/word2vec.cu:366): its logit is reported, its loss term and
+ * gradient are 0.  The reference's in-place step on the same pairs
+ * (src/word2vec.cu:359-396) is W -= alpha * dL/dW one tuple at a time.
+ * K <= 15; dim 64, 128 or 256; bf16 != 0 computes from BF16 copies of the
+ * rows (FP32 products and sums; updates go to the FP32 rows).
+ *
+ * forward: logits[n * (1 + K)] (may be NULL) and the summed loss (FP64 sum
+ * of FP32 terms in a fixed order; may be NULL). */
 int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K,
                            int bf16, float* logits, double* loss);
+/* backward: dL/dz_j = sig(z_j) - label_j, or the caller's upstream gradient
+ * dlogits[n * (1 + K)] when not NULL; row gradients dL/dW are summed per row
+ * by a sorted-key segmented reduction in a fixed order (the result does not
+ * depend on scheduling).  apply != 0 then updates W -= lr * dL/dW.  The
+ * gradients stay readable through dg_sgns_grad_rows until the next backward. */
 int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
-                            float lr, int apply);
-/* Gradient rows of the last backward: `table` 0 = syn0, 1 = syn1, rows given
- * by rank; out[n_rows * dim] (rows that got no gradient read 0). */
+                            const float* dlogits, float lr, int apply);
+/* Gradient rows of the last backward: `table` 0 = syn0, 1 = syn1, rows by
+ * rank; out[n_rows * dim] (rows that got no gradient read 0). */
 int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, uint64_t n_rows, float* out);
+
+/* Tables without a vocabulary, for the tuple interface: `rows` x `dim` FP32
+ * input (syn0) and output (syn1) rows; rank r is vertex r.  Replaces any
+ * trainer state; training again needs dg_sgns_init.  No reference
+ * counterpart: the reference builds its tables from a corpus
+ * (src/word2vec.cu:844-878). */
+int dg_sgns_tables_create(dg_ctx* ctx, uint32_t rows, uint32_t dim, const float* syn0,
+                          const float* syn1);
 
 /* ---- timing ------------------------------------------------------------ */
 
@@ -293,6 +313,14 @@ int dg_copy_bench(dg_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps);
  * edges (src < dst) with data 0 and returns the count in *e_num. */
 int dg_gen_chung_lu(uint32_t v_num, uint64_t e_target, double gamma, uint32_t min_degree,
                     uint64_t seed, dg_edge* edges, uint64_t e_cap, uint64_t* e_num);
+
+/* Zipf(s) ids: out[i] in [0, v_num) with P(k) proportional to (k + 1)^-s,
+ * by inverse CDF (double prefix sums, binary search); element i uses its own
+ * Philox stream, so the output does not depend on the thread count. */
+int dg_gen_zipf(uint32_t v_num, double s, uint64_t n, uint64_t seed, uint32_t* out);
+
+/* N(0, sigma^2) floats by Box-Muller, one Philox block per pair of outputs. */
+int dg_gen_normal(uint64_t n, double sigma, uint64_t seed, float* out);
 
 /* R-MAT (Chakrabarti, Zhan, Faloutsos 2004) with 2^scale vertices and
  * edge_factor * 2^scale draws, quadrant probabilities (a, b, c, 1-a-b-c),

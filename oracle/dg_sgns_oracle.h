@@ -42,7 +42,18 @@ void dgo_sg_shared_first_order(const int32_t* sen, const uint64_t* sen_off, uint
                                const int32_t* table, uint32_t table_size, uint32_t vocab,
                                float alpha, const float* expt, int exact_sigmoid, uint64_t seed,
                                uint32_t epoch, uint32_t dim, float* syn0, float* syn1);
-double dgo_tuple_loss(const uint32_t* tuples, uint64_t n, uint32_t K, const double* syn0,
-                      const double* syn1, uint32_t dim);
+/* Tuple interface, FP64: logits and loss, gradient rows, central
+ * differences, and the reference's sequential in-place update. */
+double dgo_tuple_forward(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                         const float* syn1, uint32_t dim, int bf16, double* logits);
+void dgo_tuple_grad_rows(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                         const float* syn1, uint32_t dim, uint32_t vocab, int bf16,
+                         const double* dlogits, int table, const uint32_t* rows, uint64_t n_rows,
+                         double* out);
+double dgo_tuple_fd(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                    const float* syn1, uint32_t dim, int table, uint32_t row, uint32_t col,
+                    double h);
+void dgo_tuple_inplace(const uint32_t* tuples, uint64_t n, uint32_t K, double lr, uint32_t dim,
+                       double* syn0, double* syn1);
 
 #endif

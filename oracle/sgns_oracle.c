@@ -464,26 +464,144 @@ void dgo_sg_shared_first_order(const int32_t* sen, const uint64_t* sen_off, uint
 }
 
 /* ------------------------------------------------------------------------
- * Tuple loss for the differentiation checks (FP64):
- *   L = sum_t [ -log sig(x_ctx . y_center) - sum_k log sig(-x_ctx . y_negk) ]
- * with x = input rows (syn0), y = output rows (syn1), tuples laid out as
- * (center, context, neg_1..neg_K).
+ * Differentiable tuple interface (FP64).  A tuple is (center, context,
+ * neg_1..neg_K); x = syn0[context], y_0 = syn1[center], y_k = syn1[neg_k].
+ *   z_j = x . y_j,  L = sum_t [ softplus(-z_0) + sum_k softplus(z_k) ]
+ * which is -log sig(z_0) - sum_k log sig(-z_k).  A negative equal to the
+ * tuple's centre is skipped, as skip_gram_kernel skips it
+ * (src/word2vec.cu:366): its logit is reported, its loss term and gradient
+ * are 0.  bf16 != 0 rounds every row element to BF16 (nearest even) first,
+ * the operands of the device's BF16 form.
  * ---------------------------------------------------------------------- */
-double dgo_tuple_loss(const uint32_t* tuples, uint64_t n, uint32_t K, const double* syn0,
-                      const double* syn1, uint32_t dim) {
+static double tup_elem(const float* t, uint64_t i, int bf16) {
+  float f = t[i];
+  if (bf16) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    memcpy(&f, &u, 4);
+  }
+  return (double)f;
+}
+
+static double softplus(double a) { return a > 0 ? a + log1p(exp(-a)) : log1p(exp(a)); }
+
+static uint32_t tup_target(const uint32_t* tp, uint32_t j) { return j == 0 ? tp[0] : tp[1 + j]; }
+
+static int tup_live(const uint32_t* tp, uint32_t j) { return j == 0 || tp[1 + j] != tp[0]; }
+
+static double tup_dot(const uint32_t* tp, uint32_t j, const float* syn0, const float* syn1,
+                      uint32_t dim, int bf16) {
+  const uint64_t xr = (uint64_t)tp[1] * dim, yr = (uint64_t)tup_target(tp, j) * dim;
+  double z = 0.0;
+  for (uint32_t d = 0; d < dim; ++d) z += tup_elem(syn0, xr + d, bf16) * tup_elem(syn1, yr + d, bf16);
+  return z;
+}
+
+double dgo_tuple_forward(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                         const float* syn1, uint32_t dim, int bf16, double* logits) {
+  const uint32_t w = 2 + K;
   double loss = 0.0;
-  uint32_t w = 2 + K;
   for (uint64_t t = 0; t < n; ++t) {
-    const double* x = syn0 + (uint64_t)tuples[t * w + 1] * dim;
+    const uint32_t* tp = tuples + t * w;
     for (uint32_t j = 0; j < 1 + K; ++j) {
-      uint32_t r = j == 0 ? tuples[t * w] : tuples[t * w + 1 + j];
-      const double* y = syn1 + (uint64_t)r * dim;
-      double z = 0.0;
-      for (uint32_t d = 0; d < dim; ++d) z += x[d] * y[d];
-      /* -log sig(s z) = log(1 + exp(-s z)), s = +1 for the centre, -1 for negatives */
-      double a = j == 0 ? -z : z;
-      loss += a > 0 ? a + log1p(exp(-a)) : log1p(exp(a));
+      const double z = tup_dot(tp, j, syn0, syn1, dim, bf16);
+      if (logits) logits[t * (1 + K) + j] = z;
+      if (tup_live(tp, j)) loss += softplus(j == 0 ? -z : z);
     }
   }
   return loss;
+}
+
+/* dL/dz_j = sig(z_j) - label_j (0 for a skipped negative), or the caller's
+ * upstream gradient dlogits (also 0 for a skipped negative). */
+static double tup_dz(const uint32_t* tp, uint64_t t, uint32_t K, uint32_t j, double z,
+                     const double* dlogits) {
+  if (!tup_live(tp, j)) return 0.0;
+  if (dlogits) return dlogits[t * (1 + K) + j];
+  return 1.0 / (1.0 + exp(-z)) - (j == 0 ? 1.0 : 0.0);
+}
+
+void dgo_tuple_grad_rows(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                         const float* syn1, uint32_t dim, uint32_t vocab, int bf16,
+                         const double* dlogits, int table, const uint32_t* rows, uint64_t n_rows,
+                         double* out) {
+  const uint32_t w = 2 + K;
+  int64_t* at = (int64_t*)malloc(sizeof(int64_t) * vocab);
+  for (uint32_t r = 0; r < vocab; ++r) at[r] = -1;
+  for (uint64_t i = 0; i < n_rows; ++i) at[rows[i]] = (int64_t)i;
+  memset(out, 0, sizeof(double) * n_rows * dim);
+  for (uint64_t t = 0; t < n; ++t) {
+    const uint32_t* tp = tuples + t * w;
+    const uint64_t xr = (uint64_t)tp[1] * dim;
+    for (uint32_t j = 0; j < 1 + K; ++j) {
+      const uint32_t yv = tup_target(tp, j);
+      const int64_t dst = table == 0 ? at[tp[1]] : at[yv];
+      if (dst < 0) continue;
+      const double gz = tup_dz(tp, t, K, j, tup_dot(tp, j, syn0, syn1, dim, bf16), dlogits);
+      double* o = out + (uint64_t)dst * dim;
+      const float* src = table == 0 ? syn1 : syn0;
+      const uint64_t sr = table == 0 ? (uint64_t)yv * dim : xr;
+      for (uint32_t d = 0; d < dim; ++d) o[d] += gz * tup_elem(src, sr + d, bf16);
+    }
+  }
+  free(at);
+}
+
+/* Central difference (L(w + h e) - L(w - h e)) / 2h of the FP64 loss (no BF16
+ * rounding) for coordinate (table, row, col).  Only tuples that read the row
+ * change, so only they are evaluated. */
+double dgo_tuple_fd(const uint32_t* tuples, uint64_t n, uint32_t K, const float* syn0,
+                    const float* syn1, uint32_t dim, int table, uint32_t row, uint32_t col,
+                    double h) {
+  const uint32_t w = 2 + K;
+  double lp = 0.0, lm = 0.0;
+  for (uint64_t t = 0; t < n; ++t) {
+    const uint32_t* tp = tuples + t * w;
+    const int x_hit = table == 0 && tp[1] == row;
+    int any = x_hit;
+    for (uint32_t j = 0; j < 1 + K && !any; ++j) any = table == 1 && tup_target(tp, j) == row;
+    if (!any) continue;
+    for (uint32_t j = 0; j < 1 + K; ++j) {
+      if (!tup_live(tp, j)) continue;
+      const uint32_t yv = tup_target(tp, j);
+      const int y_hit = table == 1 && yv == row;
+      double zp = 0.0, zm = 0.0;
+      for (uint32_t d = 0; d < dim; ++d) {
+        const double xv = syn0[(uint64_t)tp[1] * dim + d], yw = syn1[(uint64_t)yv * dim + d];
+        const double e = d == col ? h : 0.0;
+        zp += (xv + (x_hit ? e : 0.0)) * (yw + (y_hit ? e : 0.0));
+        zm += (xv - (x_hit ? e : 0.0)) * (yw - (y_hit ? e : 0.0));
+      }
+      lp += softplus(j == 0 ? -zp : zp);
+      lm += softplus(j == 0 ? -zm : zm);
+    }
+  }
+  return (lp - lm) / (2.0 * h);
+}
+
+/* The reference's in-place update over the tuples in order, with the exact
+ * sigmoid (skip_gram_kernel, src/word2vec.cu:359-396): per tuple, for each
+ * live target j in order, g = (label - sig(z)) * lr with z from the current
+ * rows; neu1e += g * y_j; y_j += g * x; then x += neu1e. */
+void dgo_tuple_inplace(const uint32_t* tuples, uint64_t n, uint32_t K, double lr, uint32_t dim,
+                       double* syn0, double* syn1) {
+  const uint32_t w = 2 + K;
+  double* neu1e = (double*)malloc(sizeof(double) * dim);
+  for (uint64_t t = 0; t < n; ++t) {
+    const uint32_t* tp = tuples + t * w;
+    double* x = syn0 + (uint64_t)tp[1] * dim;
+    memset(neu1e, 0, sizeof(double) * dim);
+    for (uint32_t j = 0; j < 1 + K; ++j) {
+      if (!tup_live(tp, j)) continue;
+      double* y = syn1 + (uint64_t)tup_target(tp, j) * dim;
+      double z = 0.0;
+      for (uint32_t d = 0; d < dim; ++d) z += x[d] * y[d];
+      const double g = ((j == 0 ? 1.0 : 0.0) - 1.0 / (1.0 + exp(-z))) * lr;
+      for (uint32_t d = 0; d < dim; ++d) neu1e[d] += g * y[d];
+      for (uint32_t d = 0; d < dim; ++d) y[d] += g * x[d];
+    }
+    for (uint32_t d = 0; d < dim; ++d) x[d] += neu1e[d];
+  }
+  free(neu1e);
 }

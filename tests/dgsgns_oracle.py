@@ -43,8 +43,14 @@ def lib():
                                 vp, c.c_uint32, c.c_uint32, c.c_float, vp, c.c_int, c.c_uint64,
                                 c.c_uint32, c.c_uint32, vp, vp]
     L.dgo_sg_shared_first_order.argtypes = L.dgo_sg_shared.argtypes
-    L.dgo_tuple_loss.restype = c.c_double
-    L.dgo_tuple_loss.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32]
+    L.dgo_tuple_forward.restype = c.c_double
+    L.dgo_tuple_forward.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32, c.c_int, vp]
+    L.dgo_tuple_grad_rows.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32, c.c_uint32,
+                                      c.c_int, vp, c.c_int, vp, c.c_uint64, vp]
+    L.dgo_tuple_fd.restype = c.c_double
+    L.dgo_tuple_fd.argtypes = [vp, c.c_uint64, c.c_uint32, vp, vp, c.c_uint32, c.c_int, c.c_uint32,
+                               c.c_uint32, c.c_double]
+    L.dgo_tuple_inplace.argtypes = [vp, c.c_uint64, c.c_uint32, c.c_double, c.c_uint32, vp, vp]
     _L = L
     return L
 
@@ -133,9 +139,40 @@ class Trainer:
         return b.value, negs[:draws].copy()
 
 
-def tuple_loss(tuples, K, syn0, syn1):
-    L = lib()
-    t = np.ascontiguousarray(tuples, dtype=np.uint32)
-    s0 = np.ascontiguousarray(syn0, dtype=np.float64)
-    s1 = np.ascontiguousarray(syn1, dtype=np.float64)
-    return L.dgo_tuple_loss(_vp(t), t.shape[0], K, _vp(s0), _vp(s1), s0.shape[1])
+class Tuples:
+    """FP64 oracle of the tuple interface over float tables (rows x dim)."""
+
+    def __init__(self, tuples, K, syn0, syn1):
+        self.t = np.ascontiguousarray(tuples, dtype=np.uint32)
+        self.K = int(K)
+        assert self.t.ndim == 2 and self.t.shape[1] == 2 + self.K
+        self.s0 = np.ascontiguousarray(syn0, dtype=np.float32)
+        self.s1 = np.ascontiguousarray(syn1, dtype=np.float32)
+        self.dim = self.s0.shape[1]
+
+    def forward(self, bf16=False):
+        """(logits[n, 1+K], loss) in FP64."""
+        logits = np.zeros((self.t.shape[0], 1 + self.K), np.float64)
+        loss = lib().dgo_tuple_forward(_vp(self.t), self.t.shape[0], self.K, _vp(self.s0),
+                                       _vp(self.s1), self.dim, int(bf16), _vp(logits))
+        return logits, loss
+
+    def grad_rows(self, table, rows, bf16=False, dlogits=None):
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        out = np.zeros((len(r), self.dim), np.float64)
+        dl = None if dlogits is None else np.ascontiguousarray(dlogits, dtype=np.float64)
+        lib().dgo_tuple_grad_rows(_vp(self.t), self.t.shape[0], self.K, _vp(self.s0), _vp(self.s1),
+                                  self.dim, self.s0.shape[0], int(bf16), _vp(dl), int(table), _vp(r),
+                                  len(r), _vp(out))
+        return out
+
+    def fd(self, table, row, col, h=1e-3):
+        return lib().dgo_tuple_fd(_vp(self.t), self.t.shape[0], self.K, _vp(self.s0), _vp(self.s1),
+                                  self.dim, int(table), int(row), int(col), float(h))
+
+    def inplace(self, lr):
+        """Tables after the reference's sequential in-place pass, FP64."""
+        x = self.s0.astype(np.float64)
+        y = self.s1.astype(np.float64)
+        lib().dgo_tuple_inplace(_vp(self.t), self.t.shape[0], self.K, float(lr), self.dim, _vp(x), _vp(y))
+        return x, y
