@@ -109,15 +109,16 @@ __device__ __forceinline__ void store_row(float* p, const float (&v)[D]) {
   }
 }
 
-// Sum NV per-lane values over the 64 lanes at once.  Each butterfly step
-// exchanges half of the remaining values with the lane `off` away, so after
-// log2(NV) steps every lane holds one value index, idx = lane / (64 / NV),
-// summed over the lanes that differ in the exchanged bits; the remaining
-// steps finish the sum.  Every lane then holds the full sum of value idx.
+// Sum NV per-lane values over each half-wavefront (32 lanes) at once.  Each
+// butterfly step exchanges half of the remaining values with the lane `off`
+// away, so after log2(NV) steps every lane holds one value index,
+// idx = (lane % 32) / (32 / NV), summed over the lanes that differ in the
+// exchanged bits; the remaining steps finish the sum.  Every lane of a half
+// then holds its half's full sum of value idx.
 template <int NV>
-__device__ __forceinline__ float multi_sum(float (&p)[NV], uint32_t lane) {
+__device__ __forceinline__ float half_multi_sum(float (&p)[NV], uint32_t lane) {
 #pragma unroll
-  for (int h = NV / 2, off = 32; h >= 1; h >>= 1, off >>= 1) {
+  for (int h = NV / 2, off = 16; h >= 1; h >>= 1, off >>= 1) {
     const bool up = (lane & (uint32_t)off) != 0;
 #pragma unroll
     for (int i = 0; i < h; ++i) {
@@ -128,7 +129,7 @@ __device__ __forceinline__ float multi_sum(float (&p)[NV], uint32_t lane) {
   }
   float v = p[0];
 #pragma unroll
-  for (int off = 32 / NV; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 16 / NV; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
@@ -152,54 +153,77 @@ struct TupleArgs {
   uint32_t row_bits;
 };
 
-// One wavefront per tuple.  BWD = false: logits and per-tuple loss.
-// BWD = true: dL/dz, the staged context-row gradient and the sort pairs
-// (entries [0, n) for syn0 rows, [n, N) for syn1 rows, in tuple order).
+// Two tuples per wavefront, one per half (32 lanes, D = dim/32 consecutive
+// elements per lane, so a row is one 16-byte load per lane at dim 128).
+// The next pair's ids are loaded before the current pair is reduced.
+// BWD = false: logits and per-tuple loss.  BWD = true: dL/dz, the staged
+// context-row gradient and the sort pairs (entries [0, n) for syn0 rows,
+// [n, N) for syn1 rows, in tuple order).
 template <typename E, int DIM, int NV, bool BWD>
 __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
-  constexpr int D = DIM / 64;
-  constexpr int kStride = 64 / NV;  // lanes per value index after multi_sum
+  constexpr int D = DIM / 32;
+  constexpr int kStride = 32 / NV;          // lanes per value index after the sum
+  constexpr bool kKeepY = NV * D <= 64;     // else the stage pass reloads y rows
   const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t hl = lane & 31u, hb = lane & 32u;  // lane within the half, half base
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const E* xt = static_cast<const E*>(a.xt);
   const E* yt = static_cast<const E*>(a.yt);
   const uint32_t K1 = a.K + 1, W = a.K + 2;
-  const uint32_t idx = lane / kStride;  // value index this lane ends with
-  for (uint64_t t = wave; t < a.n; t += nwaves) {
-    const uint32_t id = lane < W ? a.tuples[t * W + lane] : 0u;
-    const uint32_t center = __shfl(id, 0, 64);
-    const uint32_t ctx = __shfl(id, 1, 64);
-    // this lane's value index: its target row and whether it is a live term
-    const uint32_t my_target = __shfl(id, idx == 0 ? 0 : (int)(1 + idx) & 63, 64);
-    const bool my_live = idx < K1 && (idx == 0 || my_target != center);
+  const uint32_t idx = hl / kStride;  // value index this lane ends with
+  const uint64_t pairs = (a.n + 1) / 2;
+  uint64_t pr = wave;
+  uint32_t id = 0;
+  if (pr < pairs) {
+    const uint64_t t = 2 * pr + (hb ? 1 : 0);
+    id = (t < a.n && hl < W) ? a.tuples[t * W + hl] : 0u;
+  }
+  for (; pr < pairs; pr += nwaves) {
+    const uint64_t t = 2 * pr + (hb ? 1 : 0);
+    const bool tv = t < a.n;  // this half has a tuple (the last pair may be odd)
+    const uint32_t center = __shfl(id, (int)hb, 64);
+    const uint32_t ctx = __shfl(id, (int)hb + 1, 64);
+    const uint32_t my_target = __shfl(id, (int)hb + (idx == 0 ? 0 : (int)(1 + idx) & 31), 64);
+    const bool my_live = tv && idx < K1 && (idx == 0 || my_target != center);
     float x[D];
-    RowIO<E, D>::load(xt + (uint64_t)ctx * DIM + lane * D, x);
-    float y[NV][D];
+    if (tv) RowIO<E, D>::load(xt + (uint64_t)ctx * DIM + hl * D, x);
+    float y[kKeepY ? NV : 1][D];
     float p[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       p[j] = 0.0f;
       if ((uint32_t)j < K1) {
-        const uint32_t r = __shfl(id, j == 0 ? 0 : 1 + j, 64);
-        RowIO<E, D>::load(yt + (uint64_t)r * DIM + lane * D, y[j]);
+        const uint32_t r = __shfl(id, (int)hb + (j == 0 ? 0 : 1 + j), 64);
+        float yj[D];
+        if (tv) {
+          RowIO<E, D>::load(yt + (uint64_t)r * DIM + hl * D, yj);
 #pragma unroll
-        for (int q = 0; q < D; ++q) p[j] += x[q] * y[j][q];
-      } else {
+          for (int q = 0; q < D; ++q) p[j] += x[q] * yj[q];
+        }
+        if constexpr (kKeepY) {
 #pragma unroll
-        for (int q = 0; q < D; ++q) y[j][q] = 0.0f;
+          for (int q = 0; q < D; ++q) y[j][q] = tv ? yj[q] : 0.0f;
+        }
       }
     }
-    const float z = multi_sum<NV>(p, lane);
-    const bool writer = (lane % kStride) == 0 && idx < K1;
+    // ids of the next pair, in flight during the sums below
+    const uint64_t pn = pr + nwaves;
+    uint32_t id_next = 0;
+    if (pn < pairs) {
+      const uint64_t tn = 2 * pn + (hb ? 1 : 0);
+      id_next = (tn < a.n && hl < W) ? a.tuples[tn * W + hl] : 0u;
+    }
+    const float z = half_multi_sum<NV>(p, lane);
+    const bool writer = tv && (hl % kStride) == 0 && idx < K1;
     if constexpr (!BWD) {
       if (writer) a.logits[t * K1 + idx] = z;
       // loss: one lane per live value index, summed in index order
-      float l = (writer && my_live) ? softplus_f(idx == 0 ? -z : z) : 0.0f;
+      const float l = (writer && my_live) ? softplus_f(idx == 0 ? -z : z) : 0.0f;
       float acc = 0.0f;
 #pragma unroll
-      for (int j = 0; j < NV; ++j) acc += __shfl(l, j * kStride, 64);
-      if (lane == 0) a.tloss[t] = acc;
+      for (int j = 0; j < NV; ++j) acc += __shfl(l, (int)hb + j * kStride, 64);
+      if (tv && hl == 0) a.tloss[t] = acc;
     } else {
       float dz = 0.0f;
       if (my_live) {
@@ -219,38 +243,66 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         if ((uint32_t)j < K1) {
-          const float gj = __shfl(dz, j * kStride, 64);
+          const float gj = __shfl(dz, (int)hb + j * kStride, 64);
+          if constexpr (kKeepY) {
 #pragma unroll
-          for (int q = 0; q < D; ++q) c[q] += gj * y[j][q];
+            for (int q = 0; q < D; ++q) c[q] += gj * y[j][q];
+          } else {
+            const uint32_t r = __shfl(id, (int)hb + (j == 0 ? 0 : 1 + j), 64);
+            if (tv) {
+              float yj[D];
+              RowIO<E, D>::load(yt + (uint64_t)r * DIM + hl * D, yj);
+#pragma unroll
+              for (int q = 0; q < D; ++q) c[q] += gj * yj[q];
+            }
+          }
         }
       }
-      store_row<D>(a.stage + t * DIM + lane * D, c);
-      if (lane == 0) {
-        a.keys[t] = ctx;
-        a.vals[t] = (uint32_t)t;
+      if (tv) {
+        store_row<D>(a.stage + t * DIM + hl * D, c);
+        if (hl == 0) {
+          a.keys[t] = ctx;
+          a.vals[t] = (uint32_t)t;
+        }
       }
     }
+    id = id_next;
   }
 }
 
-// Deterministic loss: thread i sums entries i, i + 256, ... in order, then a
-// fixed tree over the block.
-__global__ __launch_bounds__(256) void loss_sum_kernel(const float* tloss, uint64_t n, double* out) {
-  __shared__ double part[256];
-  double s = 0.0;
-  for (uint64_t i = threadIdx.x; i < n; i += 256) s += (double)tloss[i];
-  part[threadIdx.x] = s;
+// Deterministic loss in two levels: block k sums the k-th of kLossParts
+// contiguous chunks (threads strided inside it, then a fixed tree), and one
+// block sums the chunk totals with the same tree.
+constexpr int kLossParts = 256;
+
+__device__ double block_tree_sum(double v, double* part) {
+  part[threadIdx.x] = v;
   __syncthreads();
   for (int w = 128; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = part[0];
+  return part[0];
+}
+
+__global__ __launch_bounds__(256) void loss_part_kernel(const float* tloss, uint64_t n, double* parts) {
+  __shared__ double part[256];
+  const uint64_t lo = n * blockIdx.x / kLossParts, hi = n * (blockIdx.x + 1) / kLossParts;
+  double s = 0.0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) s += (double)tloss[i];
+  const double t = block_tree_sum(s, part);
+  if (threadIdx.x == 0) parts[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void loss_final_kernel(const double* parts, double* out) {
+  __shared__ double part[256];
+  const double t = block_tree_sum(parts[threadIdx.x], part);
+  if (threadIdx.x == 0) *out = t;
 }
 
 // ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass -----------
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortBlock * kSortItems;
 
 // Per-tile digit counts, stored digit-major so that one exclusive scan gives
@@ -270,22 +322,65 @@ __global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t* 
   hist[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Per digit: exclusive scan of its per-tile counts (contiguous in the
+// digit-major histogram) into offsets within the digit, and the digit total.
+// One workgroup per digit.
+__global__ __launch_bounds__(256) void radix_digit_scan_kernel(const uint32_t* hist, uint32_t n_tiles,
+                                                               uint32_t* tile_off, uint32_t* digit_total) {
+  __shared__ uint32_t sc[256];
+  const uint32_t d = blockIdx.x;
+  const uint32_t* h = hist + (uint64_t)d * n_tiles;
+  uint32_t* o = tile_off + (uint64_t)d * n_tiles;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < n_tiles; c0 += 256) {
+    const uint32_t k = c0 + threadIdx.x;
+    const uint32_t v = k < n_tiles ? h[k] : 0u;
+    sc[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 1; st < 256; st <<= 1) {
+      const uint32_t add = threadIdx.x >= (unsigned)st ? sc[threadIdx.x - st] : 0u;
+      __syncthreads();
+      sc[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (k < n_tiles) o[k] = carry + sc[threadIdx.x] - v;
+    carry += sc[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) digit_total[d] = carry;
+}
+
 // Scatter a tile in input order: within a wave, lanes with the same digit are
 // ranked by a ballot match on the digit bits; waves and rounds are ordered
-// through LDS counters, so equal keys keep their input order.
+// through LDS counters, so equal keys keep their input order.  The digit
+// bases are the exclusive scan of the 256 digit totals, done per workgroup.
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t* keys, const uint32_t* vals, uint64_t n, int shift, uint32_t n_tiles,
-    const uint64_t* digit_off, uint32_t* okeys, uint32_t* ovals) {
+    const uint32_t* tile_off, const uint32_t* digit_total, uint32_t* okeys, uint32_t* ovals) {
   __shared__ uint32_t run[256];
+  __shared__ uint32_t base[256];
   __shared__ uint32_t wcnt[kSortBlock / 64][256];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t tot = digit_total[threadIdx.x];
+  base[threadIdx.x] = tot;
   run[threadIdx.x] = 0;
   for (int w = 0; w < kSortBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  for (int st = 1; st < 256; st <<= 1) {
+    const uint32_t add = threadIdx.x >= (unsigned)st ? base[threadIdx.x - st] : 0u;
+    __syncthreads();
+    base[threadIdx.x] += add;
+    __syncthreads();
+  }
+  // inclusive -> exclusive, plus this tile's offset within the digit
+  const uint32_t my_base = base[threadIdx.x] - tot + tile_off[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+  __syncthreads();
+  base[threadIdx.x] = my_base;
+  __syncthreads();
+  const uint64_t tile0 = (uint64_t)blockIdx.x * kSortTile;
   const uint64_t lanes_lt = (1ull << lane) - 1ull;
   for (int i = 0; i < kSortItems; ++i) {
-    const uint64_t k = base + (uint64_t)i * kSortBlock + threadIdx.x;
+    const uint64_t k = tile0 + (uint64_t)i * kSortBlock + threadIdx.x;
     const bool valid = k < n;
     const uint32_t key = valid ? keys[k] : 0u;
     const uint32_t dig = (key >> shift) & 255u;
@@ -301,7 +396,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     if (valid) {
       uint32_t before = run[dig];
       for (uint32_t w = 0; w < wave; ++w) before += wcnt[w][dig];
-      const uint64_t dst = digit_off[(uint64_t)dig * n_tiles + blockIdx.x] + before + rank;
+      const uint64_t dst = (uint64_t)base[dig] + before + rank;
       okeys[dst] = key;
       ovals[dst] = vals[k];
     }
@@ -314,17 +409,25 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
   }
 }
 
-// 1 where a sorted key starts a new row segment.
-__global__ void seg_head_kernel(const uint32_t* keys, uint64_t n, uint32_t* head) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+// Row segments (runs of equal sorted keys) starting in each block of
+// kSegBlock sorted entries; their exclusive scan numbers the segments.
+__global__ __launch_bounds__(256) void seg_count_kernel(const uint32_t* keys, uint64_t N, uint32_t* bcnt) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
+  for (uint64_t b = wave; b < n_blocks; b += nwaves) {
+    const uint64_t i = b * kSegBlock + lane;
+    const bool head = i < N && (i == 0 || keys[i] != keys[i - 1]);
+    const uint64_t heads = __ballot(head);
+    if (lane == 0) bcnt[b] = (uint32_t)__popcll(heads);
+  }
 }
 
 struct SegArgs {
   const uint32_t* keys;   // sorted, N
   const uint32_t* vals;   // entry ids in sorted order
-  const uint64_t* off;    // exclusive scan of the segment heads, N + 1
+  const uint64_t* boff;   // per block: segments starting before it (n_blocks + 1)
   uint64_t N;
   uint64_t n;             // tuples (entries [0, n) are syn0 entries)
   uint32_t K;
@@ -334,15 +437,82 @@ struct SegArgs {
   const float* stage;
   const void* xt;         // syn0 rows for the syn1 entries (float or BF16)
   uint32_t* seg_key;      // per segment: its key
+  uint32_t* seg_first;    // per segment: sorted position of its first entry
   float* grad;            // per segment: the row gradient
   float* carry;           // per block: sum of its last run when that run continues
   float* headp;           // per block: sum of its first run when that run started earlier
   uint32_t* fin;          // per block: 1 if headp finishes a row
 };
 
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float lane_f32(float v, uint32_t l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+
+constexpr int kRowsInFlight = 8;
+
+// Wave-uniform state of one block's walk over its sorted entries.
+struct SegWalk {
+  uint64_t b, base;
+  uint32_t nv, pkey0, after;
+  uint64_t ends;          // bit e: entry e ends a run of equal keys
+  uint32_t piece_start;
+  bool fin_here;
+};
+
+// Sum entries [lo, hi) of the block, all reading rows of element type T at
+// `rows + src_e`; kRowsInFlight row loads are issued before they are added,
+// in entry order.  A run end stores the run's sum: as a finished row
+// gradient, as the block's carry (the run continues in the next block) or
+// as the block's head part (the run started in an earlier block).
+template <typename T, int DIM>
+__device__ __forceinline__ void seg_run(const SegArgs& a, SegWalk& w, const T* rows, uint32_t lo,
+                                        uint32_t hi, uint32_t lane, uint32_t key, uint32_t src,
+                                        float scale, uint32_t ord, float (&acc)[DIM / 64]) {
+  constexpr int D = DIM / 64;
+  for (uint32_t e0 = lo; e0 < hi; e0 += kRowsInFlight) {
+    float v[kRowsInFlight][D];
+#pragma unroll
+    for (int u = 0; u < kRowsInFlight; ++u) {
+      if (e0 + u < hi) {
+        const uint64_t se = (uint64_t)lane_u32(src, e0 + u) * DIM;
+        RowIO<T, D>::load(rows + se + lane * D, v[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowsInFlight; ++u) {
+      const uint32_t e = e0 + u;
+      if (e < hi) {
+        const float ce = lane_f32(scale, e);
+#pragma unroll
+        for (int q = 0; q < D; ++q) acc[q] += ce * v[u][q];
+        if ((w.ends >> e) & 1ull) {
+          const uint32_t ke = lane_u32(key, e);
+          const bool continued = w.piece_start == 0 && w.base > 0 && w.pkey0 == ke;
+          const bool open = e + 1 == w.nv && w.after == ke;
+          if (!continued && !open) {
+            store_row<D>(a.grad + (uint64_t)lane_u32(ord, e) * DIM + lane * D, acc);
+          } else if (open) {
+            store_row<D>(a.carry + w.b * DIM + lane * D, acc);
+          } else {
+            store_row<D>(a.headp + w.b * DIM + lane * D, acc);
+            w.fin_here = true;
+          }
+#pragma unroll
+          for (int q = 0; q < D; ++q) acc[q] = 0.0f;
+          w.piece_start = e + 1;
+        }
+      }
+    }
+  }
+}
+
 // Entry e of the sorted list contributes
 //   syn0 entry (tuple t):        stage[t]               (sum_j dz_tj y_tj)
 //   syn1 entry (t, j):           dz_tj * x[context_t]
+// The table bit is the key's top bit, so a block's syn0 entries come first.
 template <typename E, int DIM>
 __global__ __launch_bounds__(256) void tuple_seg_kernel(SegArgs a) {
   constexpr int D = DIM / 64;
@@ -353,102 +523,110 @@ __global__ __launch_bounds__(256) void tuple_seg_kernel(SegArgs a) {
   const E* xt = static_cast<const E*>(a.xt);
   const uint32_t K1 = a.K + 1, W = a.K + 2;
   for (uint64_t b = wave; b < n_blocks; b += nwaves) {
-    const uint64_t base = b * kSegBlock;
-    const uint32_t nv = (uint32_t)(a.N - base < (uint64_t)kSegBlock ? a.N - base : kSegBlock);
-    const uint64_t i = base + lane;
-    const bool valid = lane < nv;
+    SegWalk w;
+    w.b = b;
+    w.base = b * kSegBlock;
+    w.nv = (uint32_t)(a.N - w.base < (uint64_t)kSegBlock ? a.N - w.base : kSegBlock);
+    const uint64_t i = w.base + lane;
+    const bool valid = lane < w.nv;
     const uint32_t key = valid ? a.keys[i] : 0xffffffffu;
     const uint32_t val = valid ? a.vals[i] : 0u;
     const uint32_t nkey = (i + 1 < a.N) ? a.keys[i + 1] : 0xffffffffu;
-    const uint32_t pkey0 = base > 0 ? a.keys[base - 1] : 0xffffffffu;  // wave-uniform
-    const uint32_t after = base + nv < a.N ? a.keys[base + nv] : 0xffffffffu;
+    w.pkey0 = w.base > 0 ? a.keys[w.base - 1] : 0xffffffffu;
+    w.after = w.base + w.nv < a.N ? a.keys[w.base + w.nv] : 0xffffffffu;
     const uint32_t up_key = __shfl_up(key, 1, 64);  // every lane takes part
-    const uint32_t prev_key = lane == 0 ? pkey0 : up_key;
-    if (valid && (i == 0 || prev_key != key)) a.seg_key[a.off[i]] = key;
-    // wave-uniform per entry: source row (offset in elements) and scale
+    const uint32_t prev_key = lane == 0 ? w.pkey0 : up_key;
+    const bool head = valid && (i == 0 || prev_key != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t lanes_lt = (1ull << lane) - 1ull;
+    const uint64_t bo = a.boff[b];
+    if (head) {
+      const uint64_t o = bo + (uint64_t)__popcll(heads & lanes_lt);
+      a.seg_key[o] = key;
+      a.seg_first[o] = (uint32_t)i;
+    }
+    // ordinal of this entry's segment: heads up to and including this lane,
+    // after the bo segments that started in earlier blocks
+    const uint32_t ord = valid ? (uint32_t)(bo + (uint64_t)__popcll(heads & (lanes_lt | (1ull << lane))) - 1) : 0u;
     const bool is0 = (key >> a.row_bits) == 0u;
-    uint64_t src = 0;
+    uint32_t src = 0;  // row index into the stage (syn0 entries) or into syn0 (syn1 entries)
     float scale = 0.0f;
     if (valid) {
       if (is0) {
-        src = (uint64_t)val * DIM;
+        src = val;
         scale = 1.0f;
       } else {
-        const uint64_t t = val / K1;
-        src = (uint64_t)a.tuples[t * W + 1] * DIM;
+        const uint32_t t = val / K1;
+        src = a.tuples[(uint64_t)t * W + 1];
         scale = a.dz[val];
       }
     }
-    const uint64_t ends = __ballot(valid && (key != nkey || lane + 1 == nv));
+    w.ends = __ballot(valid && (key != nkey || lane + 1 == w.nv));
+    const uint32_t n0 = (uint32_t)__popcll(__ballot(valid && is0));
+    w.piece_start = 0;
+    w.fin_here = false;
     float acc[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    uint32_t piece_start = 0;
-    bool fin_here = false;
-    for (uint32_t e = 0; e < nv; ++e) {
-      const uint32_t ke = __shfl(key, (int)e, 64);
-      const uint64_t se = __shfl(src, (int)e, 64);
-      const float ce = __shfl(scale, (int)e, 64);
-      float v[D];
-      if ((ke >> a.row_bits) == 0u)
-        RowIO<float, D>::load(a.stage + se + lane * D, v);
-      else
-        RowIO<E, D>::load(xt + se + lane * D, v);
-#pragma unroll
-      for (int q = 0; q < D; ++q) acc[q] += ce * v[q];
-      if ((ends >> e) & 1ull) {
-        const bool continued = piece_start == 0 && base > 0 && pkey0 == ke;
-        const bool open = e + 1 == nv && after == ke;
-        if (!continued && !open) {
-          const uint64_t ord = a.off[base + e + 1] - 1;
-          store_row<D>(a.grad + ord * DIM + lane * D, acc);
-        } else if (open) {
-          store_row<D>(a.carry + b * DIM + lane * D, acc);
-        } else {
-          store_row<D>(a.headp + b * DIM + lane * D, acc);
-          fin_here = true;
-        }
-#pragma unroll
-        for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-        piece_start = e + 1;
-      }
-    }
-    if (lane == 0) a.fin[b] = fin_here ? 1u : 0u;
+    seg_run<float, DIM>(a, w, a.stage, 0, n0, lane, key, src, scale, ord, acc);
+    seg_run<E, DIM>(a, w, xt, n0, w.nv, lane, key, src, scale, ord, acc);
+    if (lane == 0) a.fin[b] = w.fin_here ? 1u : 0u;
   }
 }
 
-// A row whose entries cross block edges: its partial sums, block by block in
-// increasing order, then the closing block's first run.
+// A row whose entries cross block edges: the carries of blocks b0 .. b-1
+// (b0 = the block of the row's first entry) and the head part of block b.
+// Sixteen wavefronts sum fixed contiguous chunks of the carries, in order;
+// the chunk sums and the head part are then added in chunk order.
+constexpr int kFixWaves = 16;
+
 template <int DIM>
-__global__ __launch_bounds__(256) void tuple_fix_kernel(SegArgs a) {
+__global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
   constexpr int D = DIM / 64;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  __shared__ float part[kFixWaves][DIM];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint64_t n_blocks = (a.N + kSegBlock - 1) / kSegBlock;
-  for (uint64_t b = wave; b < n_blocks; b += nwaves) {
-    if (!a.fin[b]) continue;  // wave-uniform
-    // first block of the row: walk back while block b0 lies wholly inside
-    // the row (its first entry has the row's key and so does the entry
-    // before it); b >= 1 because block b's first run continues a row
-    const uint32_t row_key = a.keys[b * kSegBlock];
-    uint64_t b0 = b - 1;
-    while (b0 > 0 && a.keys[b0 * kSegBlock] == row_key && a.keys[b0 * kSegBlock - 1] == row_key) --b0;
+  for (uint64_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
+    if (!a.fin[b]) continue;  // uniform over the workgroup
+    // block b's first entry continues a segment that started earlier: the
+    // last one counted before block b
+    const uint64_t ord = a.boff[b] - 1;
+    const uint64_t b0 = a.seg_first[ord] / kSegBlock;
+    const uint64_t cnt = b - b0;
+    const uint64_t lo = b0 + cnt * wv / kFixWaves, hi = b0 + cnt * (wv + 1) / kFixWaves;
     float acc[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    for (uint64_t bb = b0; bb < b; ++bb) {
-      float v[D];
-      RowIO<float, D>::load(a.carry + bb * DIM + lane * D, v);
+    for (uint64_t c0 = lo; c0 < hi; c0 += kRowsInFlight) {
+      float v[kRowsInFlight][D];
 #pragma unroll
-      for (int q = 0; q < D; ++q) acc[q] += v[q];
+      for (int u = 0; u < kRowsInFlight; ++u)
+        if (c0 + u < hi) RowIO<float, D>::load(a.carry + (c0 + u) * DIM + lane * D, v[u]);
+#pragma unroll
+      for (int u = 0; u < kRowsInFlight; ++u)
+        if (c0 + u < hi) {
+#pragma unroll
+          for (int q = 0; q < D; ++q) acc[q] += v[u][q];
+        }
     }
-    float v[D];
-    RowIO<float, D>::load(a.headp + b * DIM + lane * D, v);
 #pragma unroll
-    for (int q = 0; q < D; ++q) acc[q] += v[q];
-    const uint64_t ord = a.off[b * kSegBlock + 1] - 1;
-    store_row<D>(a.grad + ord * DIM + lane * D, acc);
+    for (int q = 0; q < D; ++q) part[wv][lane * D + q] = acc[q];
+    __syncthreads();
+    if (wv == 0) {
+      float tot[D];
+      RowIO<float, D>::load(a.headp + b * DIM + lane * D, tot);
+      float s[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) s[q] = 0.0f;
+      for (int k = 0; k < kFixWaves; ++k) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) s[q] += part[k][lane * D + q];
+      }
+#pragma unroll
+      for (int q = 0; q < D; ++q) s[q] += tot[q];
+      store_row<D>(a.grad + ord * DIM + lane * D, s);
+    }
+    __syncthreads();
   }
 }
 
@@ -548,14 +726,19 @@ struct dg_tuple_state {
   size_t v_cap[2] = {0, 0};
   uint32_t* d_hist = nullptr;
   size_t hist_cap = 0;
-  uint64_t* d_hoff = nullptr;
+  uint32_t* d_hoff = nullptr;   // per (digit, tile): offset within the digit
   size_t hoff_cap = 0;
+  uint32_t* d_dtot = nullptr;   // per digit: total count
+  size_t dtot_cap = 0;
   uint32_t* d_head = nullptr;
   size_t head_cap = 0;
   uint64_t* d_off = nullptr;
   size_t off_cap = 0;
   uint32_t* d_seg_key = nullptr;
   size_t seg_key_cap = 0;
+  uint32_t* d_seg_first = nullptr;
+  size_t seg_first_cap = 0;
+  double* d_loss_parts = nullptr;
   float* d_grad = nullptr;
   size_t grad_cap = 0;
   float* d_carry = nullptr;
@@ -575,6 +758,7 @@ struct dg_tuple_state {
   uint32_t grad_dim = 0;
   uint32_t grad_rows = 0;
   uint64_t grad_N = 0;
+  uint64_t grad_blocks = 0;  // segment count is d_off[grad_blocks]
   int sorted = 0;  // which of d_k / d_v holds the sorted pairs
 };
 
@@ -583,7 +767,7 @@ void dg_tuples_free(dg_ctx* ctx) {
   if (!t) return;
   void* bufs[] = {t->d_tuples, t->d_logits, t->d_tloss, t->d_loss,  t->d_dl,   t->d_dz,
                   t->d_stage,  t->d_k[0],   t->d_k[1],  t->d_v[0],  t->d_v[1], t->d_hist,
-                  t->d_hoff,   t->d_head,   t->d_off,   t->d_seg_key, t->d_grad, t->d_carry,
+                  t->d_hoff,   t->d_dtot,   t->d_head,   t->d_off,   t->d_seg_key, t->d_seg_first, t->d_loss_parts, t->d_grad, t->d_carry,
                   t->d_headp,  t->d_fin,    t->d_bf0,   t->d_bf1};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -647,7 +831,7 @@ int ensure_bf16(dg_ctx* ctx, dg_tuple_state* st, const dg_table_view& tv) {
 
 template <typename E, int DIM, bool BWD>
 void launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
-  const int g = waves_grid(ctx, a.n);
+  const int g = waves_grid(ctx, (a.n + 1) / 2);  // a tuple per half-wavefront
   if (a.K + 1 <= 8)
     tuple_fwd_kernel<E, DIM, 8, BWD><<<g, 256, 0, ctx->stream>>>(a);
   else
@@ -668,16 +852,16 @@ void launch_fwd_form(dg_ctx* ctx, const TupleArgs& a, int bf16, uint32_t dim) {
 }
 
 template <typename E>
-void launch_seg(dg_ctx* ctx, const SegArgs& a, uint32_t dim, int g) {
+void launch_seg(dg_ctx* ctx, const SegArgs& a, uint32_t dim, int g, int gf) {
   if (dim == 64) {
     tuple_seg_kernel<E, 64><<<g, 256, 0, ctx->stream>>>(a);
-    tuple_fix_kernel<64><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fix_kernel<64><<<gf, 64 * kFixWaves, 0, ctx->stream>>>(a);
   } else if (dim == 128) {
     tuple_seg_kernel<E, 128><<<g, 256, 0, ctx->stream>>>(a);
-    tuple_fix_kernel<128><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fix_kernel<128><<<gf, 64 * kFixWaves, 0, ctx->stream>>>(a);
   } else {
     tuple_seg_kernel<E, 256><<<g, 256, 0, ctx->stream>>>(a);
-    tuple_fix_kernel<256><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fix_kernel<256><<<gf, 64 * kFixWaves, 0, ctx->stream>>>(a);
   }
 }
 
@@ -696,6 +880,7 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
   DG_CHECK(dg_reserve(ctx, &st->d_logits, &st->logits_cap, (size_t)(n * K1)));
   DG_CHECK(dg_reserve(ctx, &st->d_tloss, &st->tloss_cap, (size_t)n));
   if (!st->d_loss) DG_HIP(ctx, hipMalloc((void**)&st->d_loss, sizeof(double)));
+  if (!st->d_loss_parts) DG_HIP(ctx, hipMalloc((void**)&st->d_loss_parts, sizeof(double) * kLossParts));
   TupleArgs a{};
   a.tuples = st->d_tuples;
   a.n = n;
@@ -707,7 +892,8 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
   size_t tk;
   DG_CHECK(dg_time_begin(ctx, "tuple_fwd", &tk));
   launch_fwd_form<false>(ctx, a, bf16, tv.dim);
-  loss_sum_kernel<<<1, 256, 0, ctx->stream>>>(st->d_tloss, n, st->d_loss);
+  loss_part_kernel<<<kLossParts, 256, 0, ctx->stream>>>(st->d_tloss, n, st->d_loss_parts);
+  loss_final_kernel<<<1, 256, 0, ctx->stream>>>(st->d_loss_parts, st->d_loss);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   if (logits)
@@ -765,28 +951,30 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   // sort (key, entry) pairs on rb + 1 key bits
   const uint32_t n_tiles = (uint32_t)((N + kSortTile - 1) / kSortTile);
   DG_CHECK(dg_reserve(ctx, &st->d_hist, &st->hist_cap, (size_t)n_tiles * 256));
-  DG_CHECK(dg_reserve(ctx, &st->d_hoff, &st->hoff_cap, (size_t)n_tiles * 256 + 1));
-  DG_CHECK(dg_reserve(ctx, &st->d_head, &st->head_cap, (size_t)N));
-  DG_CHECK(dg_reserve(ctx, &st->d_off, &st->off_cap, (size_t)N + 1));
+  DG_CHECK(dg_reserve(ctx, &st->d_hoff, &st->hoff_cap, (size_t)n_tiles * 256));
+  DG_CHECK(dg_reserve(ctx, &st->d_dtot, &st->dtot_cap, (size_t)256));
+  DG_CHECK(dg_reserve(ctx, &st->d_head, &st->head_cap, (size_t)(N / kSegBlock + 1)));
+  DG_CHECK(dg_reserve(ctx, &st->d_off, &st->off_cap, (size_t)(N / kSegBlock + 2)));
   DG_CHECK(dg_time_begin(ctx, "tuple_sort", &tk));
   int cur = 0;
   for (int shift = 0; shift < rb + 1; shift += 8) {
     radix_hist_kernel<<<n_tiles, kSortBlock, 0, ctx->stream>>>(st->d_k[cur], N, shift, n_tiles,
                                                                st->d_hist);
-    DG_CHECK(dg_scan_lengths(ctx, st->d_hist, (uint64_t)n_tiles * 256, st->d_hoff));
+    radix_digit_scan_kernel<<<256, 256, 0, ctx->stream>>>(st->d_hist, n_tiles, st->d_hoff, st->d_dtot);
     radix_scatter_kernel<<<n_tiles, kSortBlock, 0, ctx->stream>>>(
-        st->d_k[cur], st->d_v[cur], N, shift, n_tiles, st->d_hoff, st->d_k[1 - cur],
+        st->d_k[cur], st->d_v[cur], N, shift, n_tiles, st->d_hoff, st->d_dtot, st->d_k[1 - cur],
         st->d_v[1 - cur]);
     DG_HIP(ctx, hipGetLastError());
     cur = 1 - cur;
   }
-  seg_head_kernel<<<waves_grid(ctx, N / 64 + 1), 256, 0, ctx->stream>>>(st->d_k[cur], N, st->d_head);
-  DG_CHECK(dg_scan_lengths(ctx, st->d_head, N, st->d_off));
+  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
+  seg_count_kernel<<<waves_grid(ctx, n_blocks), 256, 0, ctx->stream>>>(st->d_k[cur], N, st->d_head);
+  DG_CHECK(dg_scan_lengths(ctx, st->d_head, n_blocks, st->d_off));
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   // segmented sums
-  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
   DG_CHECK(dg_reserve(ctx, &st->d_seg_key, &st->seg_key_cap, (size_t)N));
+  DG_CHECK(dg_reserve(ctx, &st->d_seg_first, &st->seg_first_cap, (size_t)N));
   DG_CHECK(dg_reserve(ctx, &st->d_grad, &st->grad_cap, (size_t)(N * dim)));
   DG_CHECK(dg_reserve(ctx, &st->d_carry, &st->carry_cap, (size_t)(n_blocks * dim)));
   DG_CHECK(dg_reserve(ctx, &st->d_headp, &st->headp_cap, (size_t)(n_blocks * dim)));
@@ -794,7 +982,7 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   SegArgs sa{};
   sa.keys = st->d_k[cur];
   sa.vals = st->d_v[cur];
-  sa.off = st->d_off;
+  sa.boff = st->d_off;
   sa.N = N;
   sa.n = n;
   sa.K = K;
@@ -804,16 +992,18 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   sa.stage = st->d_stage;
   sa.xt = bf16 ? (const void*)st->d_bf0 : (const void*)tv.syn0;
   sa.seg_key = st->d_seg_key;
+  sa.seg_first = st->d_seg_first;
   sa.grad = st->d_grad;
   sa.carry = st->d_carry;
   sa.headp = st->d_headp;
   sa.fin = st->d_fin;
   const int g = waves_grid(ctx, n_blocks);
+  const int gf = (int)std::min<uint64_t>(n_blocks, (uint64_t)ctx->num_cus * 4);
   DG_CHECK(dg_time_begin(ctx, "tuple_seg", &tk));
   if (bf16)
-    launch_seg<uint16_t>(ctx, sa, dim, g);
+    launch_seg<uint16_t>(ctx, sa, dim, g, gf);
   else
-    launch_seg<float>(ctx, sa, dim, g);
+    launch_seg<float>(ctx, sa, dim, g, gf);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   st->sorted = cur;
@@ -821,6 +1011,7 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   st->grad_dim = dim;
   st->grad_rows = tv.rows;
   st->grad_N = N;
+  st->grad_blocks = n_blocks;
   if (apply) {
     DG_CHECK(dg_time_begin(ctx, "tuple_apply", &tk));
     const bool shadow = st->d_bf0 && st->bf_version == tv.version;
@@ -828,13 +1019,13 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
     uint16_t* b1 = shadow ? st->d_bf1 : nullptr;
     const int ga = waves_grid(ctx, N);
     if (dim == 64)
-      tuple_apply_kernel<64><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + N, rb,
+      tuple_apply_kernel<64><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
                                                           lr, tv.syn0, tv.syn1, b0, b1);
     else if (dim == 128)
-      tuple_apply_kernel<128><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + N, rb,
+      tuple_apply_kernel<128><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
                                                            lr, tv.syn0, tv.syn1, b0, b1);
     else
-      tuple_apply_kernel<256><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + N, rb,
+      tuple_apply_kernel<256><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
                                                            lr, tv.syn0, tv.syn1, b0, b1);
     DG_HIP(ctx, hipGetLastError());
     DG_CHECK(dg_time_end(ctx, tk));
@@ -865,7 +1056,7 @@ extern "C" int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, u
   hipError_t e = hipMemcpy(d_rows, rows, sizeof(uint32_t) * n_rows, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     grad_rows_kernel<<<waves_grid(ctx, n_rows), 256, 0, ctx->stream>>>(
-        st->d_seg_key, st->d_grad, st->d_off + st->grad_N, st->grad_rb, (uint32_t)table, d_rows, n_rows,
+        st->d_seg_key, st->d_grad, st->d_off + st->grad_blocks, st->grad_rb, (uint32_t)table, d_rows, n_rows,
         st->grad_dim, d_out);
     e = hipGetLastError();
   }
