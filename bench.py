@@ -98,6 +98,44 @@ def auc_of(ctx, pos, neg):
     return distger.auc(ps, ns)
 
 
+def tuple_workload(ctx, iters=20, warmup=3):
+    """Differentiable interface: minibatches of 65,536 (center, context, 5
+    negatives) tuples, ids Zipf(1.1) over 2,238,731 rows, dim 128, rows
+    N(0, 0.1^2).  One step = forward + backward + apply.  Algorithmic bytes
+    per step: the unique rows U = U_in + U_out read once at storage
+    precision s and the FP32 master rows read and written once,
+    2 U d s + 2 U d 4."""
+    rows, n, K, dim = 2238731, 65536, 5, 128
+    t = distger.gen_zipf(rows, 1.1, n * (K + 2), 2024).reshape(n, K + 2)
+    s0 = distger.gen_normal(rows * dim, 0.1, 99).reshape(rows, dim)
+    s1 = distger.gen_normal(rows * dim, 0.1, 100).reshape(rows, dim)
+    u_in = len(np.unique(t[:, 1]))
+    u_out = len(np.unique(np.concatenate([t[:, :1], t[:, 2:]], axis=1)))
+    ctx.sgns_tables_create(s0, s1)
+    del s0, s1
+    res = {"tuples": n, "negatives": K, "rows": rows, "dim": dim, "unique_in": u_in, "unique_out": u_out}
+    phases = ("tuple_fwd", "tuple_prep", "tuple_sort", "tuple_seg", "tuple_apply")
+    for bf16 in (False, True):
+        for _ in range(warmup):
+            ctx.tuples_forward(t, K, bf16=bf16)
+            ctx.tuples_backward(t, K, bf16=bf16, lr=0.0, apply=True)
+        ctx.timing_reset()
+        for _ in range(iters):
+            ctx.tuples_forward(t, K, bf16=bf16)
+            ctx.tuples_backward(t, K, bf16=bf16, lr=0.0, apply=True)
+        us = {ph: 1e3 * ctx.timing(ph)[1] / iters for ph in phases}
+        step_us = sum(us.values())
+        alg = 2 * (u_in + u_out) * dim * (2 if bf16 else 4) + 2 * (u_in + u_out) * dim * 4
+        gbs = alg / (step_us * 1e-6) / 1e9
+        res["bf16" if bf16 else "fp32"] = {
+            "step_us": step_us, "tuples_per_s": n / (step_us * 1e-6),
+            "phase_us": {k.replace("tuple_", ""): v for k, v in us.items()},
+            "algorithmic_bytes_per_step": alg, "achieved_gbs": gbs, "frac_hbm": gbs / HBM_GBS}
+        log(f"tuples {'bf16' if bf16 else 'fp32'}: {step_us:.1f} us per forward+backward+apply "
+            f"of {n} tuples ({n / (step_us * 1e-6):.3e} tuples/s)")
+    return res
+
+
 def cpu_baseline(ctx, walk_sample, sgns_sample, threads, counts, seed):
     """The CPU oracle on a bounded sample of the same pipeline: walks on
     `threads` host threads, then the plain SGNS restatement on one thread."""
@@ -144,6 +182,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-vs-BF16 AUC run")
     ap.add_argument("--cpu-walks", type=int, default=400_000)
     ap.add_argument("--cpu-sgns-walks", type=int, default=20_000)
+    ap.add_argument("--no-tuples", action="store_true",
+                    help="skip the differentiable tuple workload (65,536 Zipf tuples)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -259,6 +299,8 @@ def main():
                                counts, args.seed)
         copy_gbs = ctx.copy_bench(1 << 30, 3)
         out["copy_kernel_gbs"] = copy_gbs
+        if not args.no_tuples:
+            out["tuples"] = tuple_workload(ctx)
 
     line = {
         "metric": "walk steps/s + SGNS words/s at equal AUC; BF16-MFMA vs FP32 plain, 1 GPU",

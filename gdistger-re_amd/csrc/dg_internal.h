@@ -27,6 +27,10 @@ struct dg_walk_queue {
   unsigned long long tokens;  // tokens written
 };
 
+// Hot-vertex set for visit counting: at most kHotMax vertices.
+constexpr uint32_t kHotMax = 4096;
+constexpr uint16_t kNotHot = 0xffffu;
+
 #define DG_QFLAG_TOKEN_OVERFLOW 1u
 #define DG_QFLAG_LENGTH_LIMIT 2u
 
@@ -58,6 +62,11 @@ struct dg_ctx {
   uint32_t* d_cn = nullptr;   // common-neighbour counts
   uint2* d_adj = nullptr;     // walker view {neighbour, bounce threshold}
   uint32_t* d_deg = nullptr;
+  // the highest-degree vertices, counted per workgroup in LDS when visit
+  // counts are taken (their global counters would be contended)
+  uint16_t* d_hot_slot = nullptr;  // V: slot in [0, n_hot) or kNotHot
+  uint32_t* d_hot_id = nullptr;    // n_hot: vertex of each slot
+  uint32_t n_hot = 0;
 
   // walker buffers
   dg_walk_queue* d_queue = nullptr;
@@ -66,6 +75,8 @@ struct dg_ctx {
   uint64_t freq_total = 0;           // sum of d_freq
   uint32_t* d_freq_round = nullptr;  // visits of the round being walked
   size_t freq_round_cap = 0;
+  uint32_t* d_hot_part = nullptr;    // per pack workgroup: hot-vertex counts
+  size_t hot_part_cap = 0;
   uint32_t* d_tok_raw = nullptr;     // tokens in completion order
   size_t tok_raw_cap = 0;
   uint64_t* d_walk_off = nullptr;    // per-walk offset into d_tok_raw

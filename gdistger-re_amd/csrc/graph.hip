@@ -204,9 +204,13 @@ int grid_for(dg_ctx* ctx, uint64_t n, int block) {
 }  // namespace
 
 void dg_graph_free(dg_ctx* ctx) {
-  void* bufs[] = {ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, ctx->d_adj, ctx->d_deg};
+  void* bufs[] = {ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, ctx->d_adj, ctx->d_deg, ctx->d_hot_slot,
+                  ctx->d_hot_id};
   for (void* p : bufs)
     if (p) hipFree(p);
+  ctx->d_hot_slot = nullptr;
+  ctx->d_hot_id = nullptr;
+  ctx->n_hot = 0;
   ctx->d_row_ptr = nullptr;
   ctx->d_nbr = nullptr;
   ctx->d_cn = nullptr;
@@ -229,6 +233,24 @@ static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>
   DG_HIP(ctx, hipMalloc((void**)&ctx->d_deg, sizeof(uint32_t) * v_num));
   DG_HIP(ctx, hipMemcpyAsync(ctx->d_row_ptr, rp.data(), sizeof(uint64_t) * (v_num + 1),
                              hipMemcpyHostToDevice, ctx->stream));
+  {
+    // hot set for visit counting: the kHotMax highest-degree vertices (ties
+    // by id); walks visit vertices in proportion to degree
+    const uint32_t n_hot = v_num < kHotMax ? v_num : kHotMax;
+    std::vector<uint32_t> order(v_num);
+    for (uint32_t v = 0; v < v_num; ++v) order[v] = v;
+    auto deg_of = [&](uint32_t v) { return rp[v + 1] - rp[v]; };
+    std::partial_sort(order.begin(), order.begin() + n_hot, order.end(), [&](uint32_t a, uint32_t b) {
+      return deg_of(a) != deg_of(b) ? deg_of(a) > deg_of(b) : a < b;
+    });
+    std::vector<uint16_t> slot(v_num, kNotHot);
+    for (uint32_t k = 0; k < n_hot; ++k) slot[order[k]] = (uint16_t)k;
+    DG_HIP(ctx, hipMalloc((void**)&ctx->d_hot_slot, sizeof(uint16_t) * (v_num ? v_num : 1)));
+    DG_HIP(ctx, hipMalloc((void**)&ctx->d_hot_id, sizeof(uint32_t) * (n_hot ? n_hot : 1)));
+    DG_HIP(ctx, hipMemcpy(ctx->d_hot_slot, slot.data(), sizeof(uint16_t) * v_num, hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpy(ctx->d_hot_id, order.data(), sizeof(uint32_t) * n_hot, hipMemcpyHostToDevice));
+    ctx->n_hot = n_hot;
+  }
   if (m)
     DG_HIP(ctx, hipMemcpyAsync(ctx->d_nbr, nbr.data(), sizeof(uint32_t) * m,
                                hipMemcpyHostToDevice, ctx->stream));

@@ -16,12 +16,17 @@
 //     stride so lane-strided and cooperative accesses are bank-conflict free)
 //     and spills longer walks to a private global area.  The visit count f_i
 //     of walk.hpp:816 (an unordered_map per walk) is a scan of that buffer.
-//   * A finished walk is written out cooperatively by its wave: one atomic
-//     reserves space for all lanes that finished together, then the wave
-//     writes each walk with coalesced 64-lane stores.
+//   * A finished walk is written out cooperatively by its wave into its own
+//     slot (walk index * kPathLds tokens) with one 64-lane store; only walks
+//     longer than a slot take space from a shared overflow cursor.
+//   * Visit counts are not taken in the walk loop: the packing pass counts
+//     every token, the highest-degree vertices in per-workgroup LDS counters
+//     and the rest with uncontended global atomics.
 //   * Every draw comes from the walk's own Philox stream (dg_philox.h), and the
 //     FP64 entropy/acceptance math uses dg_math.h, so the corpus is the same
 //     bits as the CPU oracle's whatever the schedule.
+#include <algorithm>
+
 #include "dg_internal.h"
 #include "dg_math.h"
 #include "dg_philox.h"
@@ -55,7 +60,6 @@ struct WalkArgs {
   uint64_t* walk_off;
   uint32_t* walk_len;
   uint32_t* spill;
-  uint32_t* freq;
   dg_walk_queue* q;
 };
 
@@ -297,24 +301,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
         // lane j < nl holds token j (nl <= kPathLds < 64)
         const uint32_t v = (uint32_t)lane < nl ? src[lane] : 0xffffffffu;
         if ((uint32_t)lane < nl) a.tokens[off + lane] = v;
-        // visit counts (path.hpp:91,95, context_map_freq / vertex_cn): one
-        // atomic per distinct vertex of the walk, adding its multiplicity
-        // (walks revisit their hub at every other step)
-        uint64_t rem = __ballot((uint32_t)lane < nl);
-        while (rem != 0) {
-          const int ld = __builtin_ctzll(rem);
-          const uint32_t vl = __shfl(v, ld, 64);
-          const uint64_t same = __ballot(v == vl) & rem;
-          if (lane == ld) atomicAdd(&a.freq[vl], (uint32_t)__popcll(same));
-          rem &= ~same;
-        }
       }
       if (finish && fits)  // the spilled tail, by its own lane
-        for (uint32_t j = kPathLds; j < len; ++j) {
-          const uint32_t v = spill[j - kPathLds];
-          a.tokens[my_off + j] = v;
-          atomicAdd(&a.freq[v], 1u);
-        }
+        for (uint32_t j = kPathLds; j < len; ++j) a.tokens[my_off + j] = spill[j - kPathLds];
       if (finish) active = false;
     }
   }
@@ -434,44 +423,116 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_apply(const uint32_t* le
   if (blockIdx.x == 0 && threadIdx.x == 0) off[n] = tile_prefix[n_tiles];
 }
 
-// ---- corpus packing: walk-id order, plus cumulative visit counts -----------
-// Each block takes 256 consecutive walks; their source and destination
-// offsets go to LDS and every thread copies single tokens, finding its walk by
-// bisection over the block's offsets, so loads and stores are coalesced.
-__global__ __launch_bounds__(256) void pack_kernel(const uint32_t* raw, const uint64_t* walk_off,
-                                                   const uint64_t* out_off, uint64_t n_walks,
-                                                   uint32_t* tokens) {
-  __shared__ uint64_t s_out[257];
-  __shared__ uint64_t s_src[256];
-  for (uint64_t w0 = (uint64_t)blockIdx.x * 256; w0 < n_walks; w0 += (uint64_t)gridDim.x * 256) {
-    const uint32_t nw = (uint32_t)(n_walks - w0 < 256 ? n_walks - w0 : 256);
-    if (threadIdx.x < nw) {
-      s_out[threadIdx.x] = out_off[w0 + threadIdx.x];
-      s_src[threadIdx.x] = walk_off[w0 + threadIdx.x];
+// ---- corpus packing in walk-id order, and the round's visit counts ------
+// (path.hpp:91,95: context_map_freq / vertex_cn).  A half-wavefront (32
+// lanes) copies one walk at a time, two walks per pass, in chunks of 32
+// tokens.  It sorts the chunk across its lanes (bitonic network), so equal
+// vertices become runs, and counts each distinct vertex once with its
+// multiplicity (walks revisit their hub at every other step): a hot vertex
+// (hot_slot != kNotHot) in this workgroup's LDS counters, written out as one
+// partial row at the end, any other vertex with a global atomic.  The
+// partial rows are gridDim.x x n_hot.
+constexpr int kPackBlock = 256;
+constexpr int kPackWalks = 2;  // walks per half-wavefront per pass
+
+__device__ __forceinline__ uint32_t bitonic32(uint32_t x, uint32_t hl) {
+#pragma unroll
+  for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t y = __shfl_xor(x, j, 32);
+      const bool up = (hl & (uint32_t)k) == 0;
+      const bool lower = (hl & (uint32_t)j) == 0;
+      x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
     }
-    if (threadIdx.x == 0) s_out[nw] = out_off[w0 + nw];
-    __syncthreads();
-    const uint64_t base = s_out[0];
-    const uint64_t total = s_out[nw] - base;
-    for (uint64_t t = threadIdx.x; t < total; t += 256) {
-      const uint64_t pos = base + t;
-      uint32_t lo = 0, hi = nw - 1;  // last walk k with s_out[k] <= pos
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (s_out[mid] <= pos)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      tokens[pos] = raw[s_src[lo] + (pos - s_out[lo])];
-    }
-    __syncthreads();
   }
+  return x;
+}
+
+__global__ __launch_bounds__(kPackBlock) void pack_count_kernel(
+    const uint32_t* raw, const uint64_t* walk_off, const uint32_t* walk_len, const uint64_t* out_off,
+    uint64_t n_walks, uint32_t* tokens, const uint16_t* hot_slot, uint32_t n_hot, uint32_t* freq,
+    uint32_t* hot_part) {
+  __shared__ uint32_t s_hot[kHotMax];
+  for (uint32_t k = threadIdx.x; k < n_hot; k += kPackBlock) s_hot[k] = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u, hl = lane & 31u, half = lane >> 5;
+  const uint64_t wave = ((uint64_t)blockIdx.x * kPackBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kPackBlock) >> 6;
+  const uint64_t per_pass = 2 * kPackWalks;  // walks per wavefront per pass
+  for (uint64_t w0 = wave * per_pass; w0 < n_walks; w0 += nwaves * per_pass) {
+    uint64_t w[kPackWalks], src[kPackWalks], dst[kPackWalks];
+    uint32_t len[kPackWalks];
+#pragma unroll
+    for (int u = 0; u < kPackWalks; ++u) {
+      w[u] = w0 + (uint64_t)(2 * u + half);
+      const bool ok = w[u] < n_walks;
+      len[u] = ok ? walk_len[w[u]] : 0u;
+      src[u] = ok ? walk_off[w[u]] : 0ull;
+      dst[u] = ok ? out_off[w[u]] : 0ull;
+    }
+    // chunks of 32 tokens; the count is made wave-uniform so that both halves
+    // take part in every shuffle
+    uint32_t nch = 0;
+#pragma unroll
+    for (int u = 0; u < kPackWalks; ++u) nch = max(nch, (len[u] + 31u) >> 5);
+    nch = max(nch, (uint32_t)__shfl_xor((int)nch, 32, 64));
+    for (uint32_t c = 0; c < nch; ++c) {
+      uint32_t x[kPackWalks], nvalid[kPackWalks];
+#pragma unroll
+      for (int u = 0; u < kPackWalks; ++u) {
+        const uint32_t j = c * 32u + hl;
+        nvalid[u] = len[u] > c * 32u ? min(len[u] - c * 32u, 32u) : 0u;
+        x[u] = 0xffffffffu;
+        if (j < len[u]) {
+          x[u] = raw[src[u] + j];
+          tokens[dst[u] + j] = x[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPackWalks; ++u) {
+        const uint32_t y = bitonic32(x[u], hl);
+        const uint32_t prev = __shfl_up(y, 1, 32);
+        const bool head = hl < nvalid[u] && (hl == 0 || prev != y);
+        const uint32_t hmask = (uint32_t)(__ballot(head) >> (32 * half));
+        if (head) {
+          const uint32_t above = hmask & ~((2u << hl) - 1u);
+          const uint32_t next = above ? (uint32_t)__builtin_ctz(above) : nvalid[u];
+          const uint32_t cnt = next - hl;
+          const uint16_t hs = hot_slot[y];
+          if (hs != kNotHot)
+            atomicAdd(&s_hot[hs], cnt);
+          else
+            atomicAdd(&freq[y], cnt);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < n_hot; k += kPackBlock)
+    hot_part[(uint64_t)blockIdx.x * n_hot + k] = s_hot[k];
+}
+
+// Hot-vertex counts: the partial rows summed per slot.  Grid (slots / 64,
+// rows / 64): each thread adds 64 partial rows of one slot (coalesced across
+// the slots of a wavefront) and adds that to the vertex's count.
+constexpr uint32_t kHotRows = 64;
+
+__global__ __launch_bounds__(64) void hot_reduce_kernel(const uint32_t* hot_part, uint32_t n_parts,
+                                                        uint32_t n_hot, const uint32_t* hot_id,
+                                                        uint32_t* freq) {
+  const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= n_hot) return;
+  const uint32_t b0 = blockIdx.y * kHotRows;
+  const uint32_t b1 = b0 + kHotRows < n_parts ? b0 + kHotRows : n_parts;
+  uint32_t c = 0;
+  for (uint32_t b = b0; b < b1; ++b) c += hot_part[(uint64_t)b * n_hot + k];
+  if (c) atomicAdd(&freq[hot_id[k]], c);
 }
 
 // ---- relative entropy (walk.hpp:693-725) in a fixed summation order -------
 // terms[i] = p_i (ln p_i - ln q_i); chunk sums of 1024 consecutive terms added
-// left to right; the chunk sums then added left to right by one thread.
+// left to right, then the chunk sums added left to right.
 constexpr uint32_t kReChunk = 1024;
 
 __global__ void re_terms_kernel(const uint32_t* deg, const uint32_t* freq, uint32_t v_num,
@@ -491,23 +552,41 @@ __global__ void re_terms_kernel(const uint32_t* deg, const uint32_t* freq, uint3
   }
 }
 
-__global__ void re_chunks_kernel(const double* terms, const uint32_t* deg, const uint32_t* freq,
-                                 uint32_t v_num, double* chunks, uint32_t n_chunks) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_chunks) return;
+// One wavefront per chunk: the 64 lanes load 64 consecutive terms at a time
+// (coalesced) and the sum is taken in index order from the lane registers,
+// left to right as the order above requires.
+__global__ __launch_bounds__(256) void re_chunks_kernel(const double* terms, const uint32_t* deg,
+                                                        const uint32_t* freq, uint32_t v_num,
+                                                        double* chunks, uint32_t n_chunks) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (c >= n_chunks) return;  // wave-uniform
   const uint32_t lo = c * kReChunk;
   const uint32_t hi = lo + kReChunk < v_num ? lo + kReChunk : v_num;
   double part = 0.0;
-  for (uint32_t i = lo; i < hi; ++i)
-    if (deg[i] != 0 && freq[i] != 0) part = part + terms[i];
-  chunks[c] = part;
+  for (uint32_t i0 = lo; i0 < hi; i0 += 64) {
+    const uint32_t i = i0 + lane;
+    const bool in = i < hi && deg[i] != 0 && freq[i] != 0;
+    const double t = in ? terms[i] : 0.0;
+    const uint64_t live = __ballot(in);
+    const uint32_t n = hi - i0 < 64u ? hi - i0 : 64u;
+    for (uint32_t k = 0; k < n; ++k)
+      if ((live >> k) & 1ull) part = part + __shfl(t, (int)k, 64);
+  }
+  if (lane == 0) chunks[c] = part;
 }
 
-__global__ void re_final_kernel(const double* chunks, uint32_t n_chunks, dg_walk_queue* q) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The chunk sums left to right, by one wavefront in the same way.
+__global__ __launch_bounds__(64) void re_final_kernel(const double* chunks, uint32_t n_chunks,
+                                                      dg_walk_queue* q) {
+  const uint32_t lane = threadIdx.x & 63u;
   double total = 0.0;
-  for (uint32_t c = 0; c < n_chunks; ++c) total = total + chunks[c];
-  q->re = total;
+  for (uint32_t c0 = 0; c0 < n_chunks; c0 += 64) {
+    const double t = c0 + lane < n_chunks ? chunks[c0 + lane] : 0.0;
+    const uint32_t n = n_chunks - c0 < 64u ? n_chunks - c0 : 64u;
+    for (uint32_t k = 0; k < n; ++k) total = total + __shfl(t, (int)k, 64);
+  }
+  if (lane == 0) q->re = total;
 }
 
 __global__ void offsets_shift_kernel(const uint64_t* src, uint64_t n, uint64_t add,
@@ -646,10 +725,8 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   a.adj = ctx->d_adj;
   a.term_table = ctx->d_term_table;
   // this round's visits go to a separate count array, added to the cumulative
-  // counts only once the round succeeded (a retry after a token-buffer
-  // overflow must not count twice)
+  // counts once the round is packed
   DG_CHECK(dg_reserve(ctx, &ctx->d_freq_round, &ctx->freq_round_cap, (size_t)ctx->v_num));
-  a.freq = ctx->d_freq_round;
   a.v_num = ctx->v_num;
   a.round = round;
   a.seed = p->seed;
@@ -670,8 +747,6 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
     a.tokens = ctx->d_tok_raw;
     a.tok_cap = ctx->tok_raw_cap;
     queue_reset_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
-    fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq_round,
-                                                                             ctx->v_num, 0u);
     size_t tk;
     DG_CHECK(dg_time_begin(ctx, "walk", &tk));
     walk_kernel<<<(unsigned)blocks, kWalkBlock, 0, ctx->stream>>>(a);
@@ -695,13 +770,31 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   const uint64_t draws = ctx->h_queue->draws;
   const uint64_t moves = ctx->h_queue->moves;
 
-  // pack in walk-id order and count visits
+  // pack in walk-id order and count the round's visits
   DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, (size_t)n_tok));
   size_t tp;
   DG_CHECK(dg_time_begin(ctx, "pack", &tp));
   DG_CHECK(dg_scan_lengths(ctx, ctx->d_walk_len, walker_num, ctx->d_offsets));
-  pack_kernel<<<grid_for(ctx, (walker_num + 255) / 256, 1), 256, 0, ctx->stream>>>(
-      ctx->d_tok_raw, ctx->d_walk_off, ctx->d_offsets, walker_num, ctx->d_tokens);
+  {
+    // every resident workgroup slot (LDS: n_hot counters each), fewer for small rounds
+    int per_cu = 0;
+    DG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pack_count_kernel, kPackBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    const uint64_t per_wg = (uint64_t)(kPackBlock / 64) * 2 * kPackWalks;  // walks per workgroup pass
+    const uint64_t need = ((uint64_t)walker_num + per_wg - 1) / per_wg;
+    uint32_t parts = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, need ? need : 1);
+    DG_CHECK(dg_reserve(ctx, &ctx->d_hot_part, &ctx->hot_part_cap, (size_t)parts * (ctx->n_hot ? ctx->n_hot : 1)));
+    fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq_round,
+                                                                             ctx->v_num, 0u);
+    pack_count_kernel<<<parts, kPackBlock, 0, ctx->stream>>>(
+        ctx->d_tok_raw, ctx->d_walk_off, ctx->d_walk_len, ctx->d_offsets, walker_num, ctx->d_tokens,
+        ctx->d_hot_slot, ctx->n_hot, ctx->d_freq_round, ctx->d_hot_part);
+    if (ctx->n_hot) {
+      const dim3 grid((ctx->n_hot + 63) / 64, (parts + kHotRows - 1) / kHotRows);
+      hot_reduce_kernel<<<grid, 64, 0, ctx->stream>>>(ctx->d_hot_part, parts, ctx->n_hot,
+                                                      ctx->d_hot_id, ctx->d_freq_round);
+    }
+  }
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tp));
   add_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
@@ -720,7 +813,7 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   DG_CHECK(dg_time_begin(ctx, "re", &tr));
   re_terms_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
       ctx->d_deg, ctx->d_freq, ctx->v_num, ctx->m, ctx->freq_total, ctx->d_re_terms, ctx->d_queue);
-  re_chunks_kernel<<<(n_chunks + 63) / 64, 64, 0, ctx->stream>>>(ctx->d_re_terms, ctx->d_deg,
+  re_chunks_kernel<<<(n_chunks + 3) / 4, 256, 0, ctx->stream>>>(ctx->d_re_terms, ctx->d_deg,
                                                                 ctx->d_freq, ctx->v_num, chunks,
                                                                 n_chunks);
   re_final_kernel<<<1, 64, 0, ctx->stream>>>(chunks, n_chunks, ctx->d_queue);
