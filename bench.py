@@ -47,20 +47,19 @@ def log(msg):
 
 
 def walker_bytes(stats_list):
-    """Algorithmic HBM bytes of the walker kernel: 16 B per row-offset pair read
-    (walk start and every move), 8 B per adjacency unit drawn, 4 B per vertex
-    written, 8 B per visit-count update, 12 B per walk record."""
-    walks = sum(s.walkers for s in stats_list)
+    """Algorithmic bytes of the walker kernel by the per-step figure of the
+    precomputed-common-neighbour data path (the reference's, walk.hpp:920):
+    two dependent random 64-B transactions per walk step (the drawn adjacency
+    unit and the candidate's row offsets) plus 4 B per emitted vertex."""
     tokens = sum(s.tokens for s in stats_list)
-    draws = sum(s.draws for s in stats_list)
-    moves = sum(s.moves for s in stats_list)
-    return 16 * moves + 8 * draws + 12 * tokens + 12 * walks
+    steps = sum(s.steps for s in stats_list)
+    return 2 * 64 * steps + 4 * tokens
 
 
 def walker_transaction_bytes(stats_list):
-    """The walker per SURVEY's secondary-data-path figure: every random access
-    (an adjacency entry drawn, a row-offset pair read) moves one 64-byte
-    transaction, plus 4 B per token written."""
+    """Random 64-B transactions the kernel actually issues (a bounce-back
+    append reads nothing): one per drawn adjacency unit and one per row-offset
+    pair read (walk start and every move), plus 4 B per emitted vertex."""
     tokens = sum(s.tokens for s in stats_list)
     draws = sum(s.draws for s in stats_list)
     moves = sum(s.moves for s in stats_list)
@@ -261,10 +260,12 @@ def main():
         out["walk"] = {"walk_steps_per_s": steps / (walk_ms * 1e-3),
                        "round_steps_per_s": steps / ((walk_ms + pack_ms + re_ms) * 1e-3),
                        "kernel_ms": walk_ms / n_walk, "pack_ms": pack_ms / n_walk,
-                       "re_ms": re_ms / n_walk, "bytes_per_launch": walk_bytes,
+                       "re_ms": re_ms / n_walk,
+                       "algorithmic_bytes_per_launch": walk_bytes,
                        "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS,
-                       "transaction_bytes_per_launch": walk_tbytes,
-                       "transaction_gbs": walk_tgbs, "transaction_frac_hbm": walk_tgbs / HBM_GBS}
+                       "issued_transaction_bytes_per_launch": walk_tbytes,
+                       "issued_transaction_gbs": walk_tgbs,
+                       "issued_transaction_frac_hbm": walk_tgbs / HBM_GBS}
         out["sgns_bf16_pipeline"] = {"words_per_s": words / ((bf_ms + prep_ms) * 1e-3),
                                      "kernel_ms_per_launch": bf_ms / n_bf, "launches": n_bf,
                                      "prep_ms_per_launch": prep_ms / n_bf,

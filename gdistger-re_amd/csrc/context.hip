@@ -1,6 +1,8 @@
 // context.hip - context lifetime, error reporting and kernel timing.
 #include "dg_internal.h"
+#include "dg_math.h"
 
+#include <algorithm>
 #include <cstring>
 
 int dg_fail(dg_ctx* ctx, int code, const char* fmt, ...) {
@@ -181,4 +183,52 @@ extern "C" int dg_copy_bench(dg_ctx* ctx, uint64_t bytes, uint32_t iters, double
   if (rc != DG_OK) return rc;
   if (gbps) *gbps = launches ? (2.0 * (double)(n * 16) * (double)launches) / (ms * 1e-3) / 1e9 : 0.0;
   return DG_OK;
+}
+
+namespace {
+
+__global__ void math_eval_kernel(int fn, const double* x, const double* y, uint64_t n, double* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const double a = x[i];
+    double r = 0.0;
+    switch (fn) {
+      case DG_MATH_SQRT: r = dg_sqrt(a); break;
+      case DG_MATH_LOG2: r = dg_log2(a); break;
+      case DG_MATH_LOG: r = dg_log(a); break;
+      case DG_MATH_TANH: r = dg_tanh(a); break;
+      default: r = dg_pow(a, y[i]); break;
+    }
+    out[i] = r;
+  }
+}
+
+}  // namespace
+
+extern "C" int dg_math_eval(dg_ctx* ctx, int fn, const double* x, const double* y, uint64_t n,
+                            double* out) {
+  if (!ctx || fn < DG_MATH_SQRT || fn > DG_MATH_POW || (n && (!x || !out)) ||
+      (fn == DG_MATH_POW && n && !y))
+    return DG_ERR_ARG;
+  if (n == 0) return DG_OK;
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  int rc = DG_OK;
+  hipError_t e = hipMalloc((void**)&dx, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dout, sizeof(double) * n);
+  if (e == hipSuccess && fn == DG_MATH_POW) e = hipMalloc((void**)&dy, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && dy) e = hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
+    math_eval_kernel<<<(unsigned)blocks, 256, 0, ctx->stream>>>(fn, dx, dy, n, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = dg_fail(ctx, DG_ERR_HIP, "dg_math_eval: %s", hipGetErrorString(e));
+  hipFree(dx);
+  hipFree(dy);
+  hipFree(dout);
+  return rc;
 }

@@ -297,30 +297,11 @@ DG_HD double dg_tanh(double x) {
   return dg_tanh_pos(x);
 }
 
-/* Square root for x > 0 finite: Newton iterations from a linear seed on
- * [1, 4), then one correction with the fma residual x - y*y. */
-DG_HD double dg_sqrt(double x) {
-  if (!(x > 0.0)) return x == 0.0 ? x : (x - x) / (x - x);
-  uint64_t b = dg_bits(x);
-  int eb = (int)((b >> 52) & 0x7ff);
-  int adj = 0;
-  if (eb == 0) {
-    x = x * 0x1p54;
-    b = dg_bits(x);
-    eb = (int)((b >> 52) & 0x7ff);
-    adj = -54;
-  }
-  int e = eb - 1023 + adj;
-  int half = (e >= 0) ? e / 2 : -((1 - e) / 2); /* floor(e / 2) */
-  int rem = e - 2 * half;                       /* 0 or 1 */
-  double m = dg_from_bits((b & 0x000fffffffffffffULL) | ((uint64_t)(1023 + rem) << 52));
-  double y = 0.5 + 0.4 * m;
-  for (int i = 0; i < 5; ++i) {
-    y = 0.5 * (y + m / y);
-  }
-  double r = DG_FMA(-y, y, m);
-  y = y + r / (y + y);
-  return y * dg_pow2i(half);
-}
+/* Square root: IEEE-754 requires it correctly rounded, and both compilers
+ * emit a correctly rounded sequence (sqrtsd on the host; on gfx950 the
+ * v_rsq_f64 estimate refined by fma Newton steps with a final residual
+ * correction), so host and device agree bit for bit; the device side is
+ * checked against the host on sampled inputs by the tests. */
+DG_HD double dg_sqrt(double x) { return __builtin_sqrt(x); }
 
 #endif /* DG_MATH_H */

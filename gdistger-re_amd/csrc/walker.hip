@@ -67,7 +67,7 @@ struct WalkArgs {
 // both equal the number of vertices on the walk, which the kernel keeps as an
 // integer, so only the seven FP64 moments live in registers.
 struct WalkState {
-  double H, EH, ES, EHS, EHH, ESS, R;
+  double H, EH, ES, EHS, EHH, ESS;
 };
 
 // log2 term of Correlation::iterateHn with step_k = 1 (include/walk.hpp:
@@ -83,27 +83,29 @@ __device__ __forceinline__ double entropy_term(double Sn, double fi) {
   return dg_log2(prod * fac);
 }
 
-// Rest of Correlation::iterate (include/walk.hpp:317-354), same operation order.
+// Rest of Correlation::iterate (include/walk.hpp:317-354), same operation
+// order.  The correlation R itself is only read by the termination test
+// (walk.hpp:979-985), which runs after min_length steps, so it is formed from
+// the moments there (state_corr) instead of on every step.
 __device__ __forceinline__ void state_iterate(WalkState& w, double Sn, double t) {
   const double one = 1.0;
   const double H1 = (Sn * w.H - t) / (Sn + one);
   const double S1 = Sn + one;
   const double N1 = Sn + 1.0;
-  const double EH = w.EH + (H1 - w.EH) / N1;
-  const double ES = w.ES + (S1 - w.ES) / N1;
-  const double EHS = w.EHS + (H1 * S1 - w.EHS) / N1;
-  const double EHH = w.EHH + (H1 * H1 - w.EHH) / N1;
-  const double ESS = w.ESS + (S1 * S1 - w.ESS) / N1;
-  const double DH = EHH - EH * EH;
-  const double DS = ESS - ES * ES;
-  const double cov = EHS - EH * ES;
-  w.R = cov == 0.0 ? 0.0 : cov / dg_sqrt(DH * DS);
+  w.EH = w.EH + (H1 - w.EH) / N1;
+  w.ES = w.ES + (S1 - w.ES) / N1;
+  w.EHS = w.EHS + (H1 * S1 - w.EHS) / N1;
+  w.EHH = w.EHH + (H1 * H1 - w.EHH) / N1;
+  w.ESS = w.ESS + (S1 * S1 - w.ESS) / N1;
   w.H = H1;
-  w.EH = EH;
-  w.ES = ES;
-  w.EHS = EHS;
-  w.EHH = EHH;
-  w.ESS = ESS;
+}
+
+// Pearson R of the (H, S) history from the running moments (walk.hpp:342-354).
+__device__ __forceinline__ double state_corr(const WalkState& w) {
+  const double DH = w.EHH - w.EH * w.EH;
+  const double DS = w.ESS - w.ES * w.ES;
+  const double cov = w.EHS - w.EH * w.ES;
+  return cov == 0.0 ? 0.0 : cov / dg_sqrt(DH * DS);
 }
 
 // term_table[S * kTermStride + f] = entropy_term(S, f) for 1 <= S < kTermTable,
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
   uint64_t tot_draw = 0, tot_move = 0, tot_tok = 0;
   uint32_t max_l = 0;
   bool pending_back = false;
-  WalkState ws = {0, 0, 0, 0, 0, 0, 0};
+  WalkState ws = {0, 0, 0, 0, 0, 0};
   dg_stream rs;
   dg_stream_init(&rs, a.seed, 0, 0, DG_TAG_WALK);
 
@@ -190,7 +192,6 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
           ws.EHS = 0.0;
           ws.EHH = 0.0;
           ws.ESS = 1.0;
-          ws.R = 1.0;
           dg_stream_init(&rs, a.seed, w, a.round, DG_TAG_WALK);
           active = true;
         }
@@ -211,6 +212,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
         bool append = false;
         bool trace = false;
         uint32_t v_app = 0;
+        uint64_t next_rb = 0, next_re = 0;  // row of the vertex moved to, loaded early
         if (pending_back) {
           v_app = cur;  // walk.hpp:986-1017: back to the current vertex
           append = true;
@@ -227,6 +229,10 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
             trace = (u >= e.y);                    // walk.hpp:933-938 via the threshold
             v_app = e.x;
             append = true;
+            if (!trace) {  // the walk moves to e.x unless it ends: its row, in flight during the math
+              next_rb = a.row_ptr[e.x];
+              next_re = a.row_ptr[e.x + 1];
+            }
           }
         }
         if (append) {
@@ -244,7 +250,12 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
             spill[len - kPathLds] = v_app;
           ++len;
           const uint32_t step = len - 1;
-          if (step > a.min_length && (ws.R * ws.R < a.mu || ws.R < 0.0)) {
+          bool ends = false;
+          if (step > a.min_length) {
+            const double R = state_corr(ws);
+            ends = R * R < a.mu || R < 0.0;
+          }
+          if (ends) {
             finish = true;
           } else if (a.max_length != 0 && len >= a.max_length) {
             finish = true;
@@ -257,8 +268,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
             pending_back = true;
           } else {
             cur = v_app;  // walk.hpp:1018-1023
-            cur_rb = a.row_ptr[cur];
-            cur_deg = (uint32_t)(a.row_ptr[cur + 1] - cur_rb);
+            cur_rb = next_rb;
+            cur_deg = (uint32_t)(next_re - next_rb);
             ++nmove;
           }
         }

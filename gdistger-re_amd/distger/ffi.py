@@ -18,6 +18,7 @@ DG_CN_FROM_DATA = 0
 DG_CN_COMPUTE = 1
 DG_ROUNDS_REFERENCE = 0
 DG_ROUNDS_CONVERGE = 1
+DG_MATH_SQRT, DG_MATH_LOG2, DG_MATH_LOG, DG_MATH_TANH, DG_MATH_POW = 0, 1, 2, 3, 4
 
 # 12-byte binary edge record (include/type.hpp Edge<float>)
 EDGE_DTYPE = np.dtype([("src", "<u4"), ("dst", "<u4"), ("data", "<f4")])
@@ -169,6 +170,7 @@ def load_library():
                                   c.POINTER(c.c_uint64), vp, c.c_uint64, c.POINTER(c.c_uint64)]),
         "dg_auc": (c.c_double, [vp, c.c_uint64, vp, c.c_uint64]),
         "dg_copy_bench": (c.c_int, [vp, c.c_uint64, c.c_uint32, c.POINTER(c.c_double)]),
+        "dg_math_eval": (c.c_int, [vp, c.c_int, vp, vp, c.c_uint64, vp]),
         "dg_sgns_params_default": (None, [c.POINTER(SgnsParams)]),
         "dg_sgns_init": (c.c_int, [vp, c.POINTER(SgnsParams)]),
         "dg_sgns_train": (c.c_int, [vp, c.c_int, c.POINTER(SgnsParams), c.c_float,
@@ -401,6 +403,13 @@ class Context:
         r = np.ascontiguousarray(rows, dtype=np.uint32)
         out = np.zeros((len(r), dim), np.float32)
         self._check(self.L.dg_sgns_grad_rows(self.h, table, _vp(r), len(r), _vp(out)))
+        return out
+
+    def math_eval(self, fn, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._check(self.L.dg_math_eval(self.h, int(fn), _vp(x), _vp(yy), len(x), _vp(out)))
         return out
 
     def copy_bench(self, nbytes, iters=3):

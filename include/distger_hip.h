@@ -304,6 +304,18 @@ int dg_timing_reset(dg_ctx* ctx);
  * other kernels.  No reference counterpart. */
 int dg_copy_bench(dg_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps);
 
+/* The walker's FP64 functions evaluated on the device, for checking that
+ * host and device round identically (the walk-termination and
+ * convergence decisions depend on it): fn DG_MATH_SQRT, _LOG2, _LOG, _TANH
+ * take x; DG_MATH_POW takes x and y.  out[i] = f(x[i] [, y[i]]).  No
+ * reference counterpart. */
+#define DG_MATH_SQRT 0
+#define DG_MATH_LOG2 1
+#define DG_MATH_LOG 2
+#define DG_MATH_TANH 3
+#define DG_MATH_POW 4
+int dg_math_eval(dg_ctx* ctx, int fn, const double* x, const double* y, uint64_t n, double* out);
+
 /* ---- host data utilities (synthetic inputs, file formats) -------------- */
 
 /* Chung-Lu power-law graph: expected degrees w_i ~ (i+1)^(-1/(gamma-1))

@@ -9,6 +9,7 @@ import pytest
 
 import dgoracle
 import distger
+from distger import ffi
 
 pytestmark = pytest.mark.gpu
 
@@ -203,3 +204,33 @@ def test_bounce_thresholds_definition(c1_ctx):
         t = int(thr[e])
         assert pn <= t / M
         assert t == 0 or not (pn <= (t - 1) / M)
+
+
+def test_device_math_matches_host(ctx):
+    """The walker's FP64 functions round identically on the device and in the
+    oracle (host): sqrt, log2, log and tanh on log-uniform and walker-range
+    inputs, pow on the Correlation forms (S/(S+1))^S and ((f+1)/f)^f."""
+    import ctypes
+    L = dgoracle.load("dg")
+    rng = np.random.default_rng(17)
+    n = 200_000
+    wide = np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n))
+    unit = rng.uniform(0.0, 4.0, n)
+    cases = [
+        (ffi.DG_MATH_SQRT, np.concatenate([wide[: n // 2], unit[: n // 2]]), None, L.dgo_sqrt),
+        (ffi.DG_MATH_LOG2, np.concatenate([wide[: n // 2], unit[: n // 2] + 1e-9]), None, L.dgo_log2),
+        (ffi.DG_MATH_LOG, np.concatenate([wide[: n // 2], unit[: n // 2] + 1e-9]), None, L.dgo_log),
+        (ffi.DG_MATH_TANH, rng.uniform(0.0, 20.0, n), None, L.dgo_tanh),
+    ]
+    s = rng.integers(1, 100_000, n).astype(np.float64)
+    cases.append((ffi.DG_MATH_POW, np.concatenate([s[: n // 2] / (s[: n // 2] + 1), (s[n // 2:] + 1) / s[n // 2:]]),
+                  s, L.dgo_pow))
+    for fn, x, y, host in cases:
+        print(f"[walker-gpu] device math fn={fn}", flush=True)
+        dev = ctx.math_eval(fn, x, y)
+        if y is None:
+            ref = np.array([host(ctypes.c_double(v)) for v in x])
+        else:
+            ref = np.array([host(ctypes.c_double(a), ctypes.c_double(b)) for a, b in zip(x, y)])
+        bad = np.flatnonzero(dev.view(np.uint64) != ref.view(np.uint64))
+        assert len(bad) == 0, f"fn {fn}: {len(bad)} of {len(x)} differ, e.g. x={x[bad[0]]!r}"
