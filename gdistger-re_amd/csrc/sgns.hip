@@ -601,66 +601,92 @@ void sg_sent_kernel(SentArgs t) {
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        // ---- the recursion, one lane per output column
+        // ---- the recursion over the contexts in order.  Lane l works on
+        // output column n = l % KC and pool vertices v = l / KC + G j (G =
+        // 64 / KC groups, J = P / G vertices per lane): corr[j] and gp[j] are
+        // the running corrections and g sums of (n, v).  Per context u the
+        // column's corr[n][u] comes from its owner lane (n, u % G) by one
+        // shuffle, gp[k][u] of every column k by readlanes of that group.
         int slot = 0;
         {
-          const int n = lane;
-          const bool live = n <= K && (n == 0 || orow != row0);
+          constexpr int G = 64 / KC;
+          constexpr int J = P / G;
+          const int n = lane % KC, grp = lane / KC;
+          const int orow_n = __shfl(orow, n, 64);
+          const bool live = n <= K && (n == 0 || orow_n != row0);
           const float lab = n == 0 ? 1.0f : 0.0f;
           const float wt = n == 0 ? t.alpha : t.alpha * t.wneg;
-          const int nc = n < KC ? n : 0;
           float gyr[KC];
 #pragma unroll
-          for (int k = 0; k < KC; ++k) gyr[k] = L.Gy[k * (KC + 1) + nc];
-          float corr[P], gp[P], mrow[KC];
+          for (int k = 0; k < KC; ++k) gyr[k] = L.Gy[k * (KC + 1) + n];
+          float corr[J], gp[J], mrow[KC];
 #pragma unroll
-          for (int v = 0; v < P; ++v) {
-            corr[v] = 0.0f;
-            gp[v] = 0.0f;
+          for (int j = 0; j < J; ++j) {
+            corr[j] = 0.0f;
+            gp[j] = 0.0f;
           }
 #pragma unroll
           for (int k = 0; k < KC; ++k) mrow[k] = 0.0f;
+          // the position's contexts in order: lane i holds the pool index of
+          // context i (the window minus the position itself)
           const int w = W - b;
           const int c_lo = q - w > 0 ? q - w : 0;
           const int c_hi = q + w < Ls - 1 ? q + w : Ls - 1;
-          for (int c = c_lo; c <= c_hi; ++c) {
-            if (c == q) continue;
-            const int u = L.p2p[c - lo];
-            float cu = 0.0f, gpu = 0.0f;
+          const int nctx = c_hi - c_lo;  // c_hi - c_lo + 1 positions, q excluded
+          int uctx = 0;
+          {
+            const int c = c_lo + lane + (c_lo + lane >= q ? 1 : 0);
+            if (lane < nctx) uctx = L.p2p[c - lo];
+          }
+          // first context's operands
+          int u = nctx > 0 ? __builtin_amdgcn_readlane(uctx, 0) : 0;
+          float s_nu = L.S[u * (KC + 1) + n];
+          float gxu[J];
 #pragma unroll
-            for (int v = 0; v < P; ++v) {
-              cu = v == u ? corr[v] : cu;
-              gpu = v == u ? gp[v] : gpu;
+          for (int j = 0; j < J; ++j) gxu[j] = L.Gx[u * GXL + grp + G * j];  // Gx symmetric
+          for (int i = 0; i < nctx; ++i) {
+            // next context's operands, in flight during this one
+            const int un = i + 1 < nctx ? __builtin_amdgcn_readlane(uctx, i + 1) : u;
+            const float s_next = L.S[un * (KC + 1) + n];
+            float gx_next[J];
+#pragma unroll
+            for (int j = 0; j < J; ++j) gx_next[j] = L.Gx[un * GXL + grp + G * j];
+            const int ou = u % G, ju = u / G;
+            float c_own = corr[0], g_own = gp[0];
+#pragma unroll
+            for (int j = 1; j < J; ++j) {
+              c_own = j == ju ? corr[j] : c_own;
+              g_own = j == ju ? gp[j] : g_own;
             }
-            float sv = L.S[u * (KC + 1) + nc] + cu;
+            const float cu = __shfl(c_own, n + KC * ou, 64);
+            float sv = s_nu + cu;
             float gpk[KC];  // g of the earlier occurrences of this vertex, per column
 #pragma unroll
             for (int k = 0; k < KC; ++k) {
-              gpk[k] = k <= K ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gpu), k)) : 0.0f;
+              gpk[k] = k <= K ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_own), k + KC * ou))
+                              : 0.0f;
               sv += gpk[k] * gyr[k];
             }
             float g = 0.0f;
             if (live) g = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
 #pragma unroll
             for (int k = 0; k < KC; ++k) mrow[k] += g * gpk[k];
-            if (n < KC) L.Gs[slot * KC + n] = g;
+            if (lane < KC) L.Gs[slot * KC + n] = g;
             if (lane == 0) L.su[slot] = u;
             ++slot;
-            const float4* gxr = reinterpret_cast<const float4*>(L.Gx + u * GXL);  // Gx symmetric
 #pragma unroll
-            for (int v4 = 0; v4 < P / 4; ++v4) {
-              const float4 gq = gxr[v4];
-              corr[4 * v4 + 0] += g * gq.x;
-              corr[4 * v4 + 1] += g * gq.y;
-              corr[4 * v4 + 2] += g * gq.z;
-              corr[4 * v4 + 3] += g * gq.w;
+            for (int j = 0; j < J; ++j) {
+              corr[j] += g * gxu[j];
+              gp[j] += (j == ju && grp == ou) ? g : 0.0f;
             }
+            u = un;
+            s_nu = s_next;
 #pragma unroll
-            for (int v = 0; v < P; ++v) gp[v] += v == u ? g : 0.0f;
+            for (int j = 0; j < J; ++j) gxu[j] = gx_next[j];
           }
-          if (n < KC) {
 #pragma unroll
-            for (int v = 0; v < P; ++v) L.Gp[v * GPL + n] = gp[v];
+          for (int j = 0; j < J; ++j) L.Gp[(grp + G * j) * GPL + n] = gp[j];
+          if (lane < KC) {
 #pragma unroll
             for (int k = 0; k < KC; ++k) L.Mm[n * GPL + k] = mrow[k];
           }
