@@ -157,12 +157,14 @@ __global__ void degree_kernel(const uint64_t* rp, uint32_t v_num, uint32_t* deg)
 }
 
 // Smallest u in [0, 2^31 - 1] with tanh(pa) <= u / (2^31 - 1), pa as in
-// walk.hpp:933; 2^31 (never reached by a 31-bit draw) if no u qualifies.
-__device__ __forceinline__ uint32_t bounce_threshold(int sd, int dd, int cnv) {
+// walk.hpp:933 (times the edge weight w on a weighted graph: tanh(pa * w));
+// 2^31 (never reached by a 31-bit draw) if no u qualifies.
+__device__ __forceinline__ uint32_t bounce_threshold(int sd, int dd, int cnv, const float* w,
+                                                     uint64_t e) {
   const double M = 2147483647.0;
   const double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) /
                     (double)(sd < dd ? sd : dd);
-  const double pn = dg_tanh(pa);
+  const double pn = dg_tanh(w ? pa * (double)w[e] : pa);
   if (pn != pn) return 0x80000000u;
   if (!(pn <= 1.0)) return 0x80000000u;
   uint32_t lo = 0, hi = 2147483647u;
@@ -177,7 +179,8 @@ __device__ __forceinline__ uint32_t bounce_threshold(int sd, int dd, int cnv) {
 }
 
 __global__ void threshold_kernel(const uint64_t* rp, const uint32_t* nbr, const uint32_t* cn,
-                                 const uint32_t* deg, uint32_t v_num, uint64_t m, uint2* adj) {
+                                 const float* w, const uint32_t* deg, uint32_t v_num, uint64_t m,
+                                 uint2* adj) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
        e += (uint64_t)gridDim.x * blockDim.x) {
     // source vertex of entry e: last u with rp[u] <= e
@@ -190,7 +193,7 @@ __global__ void threshold_kernel(const uint64_t* rp, const uint32_t* nbr, const 
         hi = mid - 1;
     }
     const uint32_t v = nbr[e];
-    adj[e] = make_uint2(v, bounce_threshold((int)deg[lo], (int)deg[v], (int)cn[e]));
+    adj[e] = make_uint2(v, bounce_threshold((int)deg[lo], (int)deg[v], (int)cn[e], w, e));
   }
 }
 
@@ -223,7 +226,7 @@ void dg_graph_free(dg_ctx* ctx) {
 // Device CSR from the host arrays; CN from the edge data or computed here.
 static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>& rp,
                         const std::vector<uint32_t>& nbr, const std::vector<uint32_t>& cn_data,
-                        int cn_mode) {
+                        const std::vector<float>& weight, int cn_mode) {
   dg_graph_free(ctx);
   const uint64_t m = rp[v_num];
   DG_HIP(ctx, hipMalloc((void**)&ctx->d_row_ptr, sizeof(uint64_t) * (v_num + 1)));
@@ -259,7 +262,7 @@ static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>
   degree_kernel<<<grid_for(ctx, v_num, 256), 256, 0, ctx->stream>>>(ctx->d_row_ptr, v_num,
                                                                     ctx->d_deg);
   DG_HIP(ctx, hipGetLastError());
-  if (cn_mode == DG_CN_COMPUTE) {
+  if (cn_mode == DG_CN_COMPUTE || cn_mode == DG_CN_COMPUTE_WEIGHTED) {
     std::vector<uint32_t> big;
     for (uint32_t v = 0; v < v_num; ++v)
       if (rp[v + 1] - rp[v] > (uint64_t)kSmallDeg) big.push_back(v);
@@ -285,21 +288,29 @@ static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>
     DG_HIP(ctx, hipMemcpyAsync(ctx->d_cn, cn_data.data(), sizeof(uint32_t) * m,
                                hipMemcpyHostToDevice, ctx->stream));
   }
+  // weights are only needed for the thresholds, so they are not kept
+  float* d_w = nullptr;
+  if (cn_mode == DG_CN_COMPUTE_WEIGHTED && m) {
+    DG_HIP(ctx, hipMalloc((void**)&d_w, sizeof(float) * m));
+    DG_HIP(ctx, hipMemcpyAsync(d_w, weight.data(), sizeof(float) * m, hipMemcpyHostToDevice,
+                               ctx->stream));
+  }
   size_t tt;
   DG_CHECK(dg_time_begin(ctx, "threshold", &tt));
   if (m)
     threshold_kernel<<<grid_for(ctx, m, 256), 256, 0, ctx->stream>>>(
-        ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, ctx->d_deg, v_num, m, ctx->d_adj);
+        ctx->d_row_ptr, ctx->d_nbr, ctx->d_cn, d_w, ctx->d_deg, v_num, m, ctx->d_adj);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tt));
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (d_w) hipFree(d_w);
   return DG_OK;
 }
 
 extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* edges,
                                    uint64_t e_num, int undirected, int cn_mode) {
   if (!ctx || (!edges && e_num) || v_num == 0) return DG_ERR_ARG;
-  if (cn_mode != DG_CN_FROM_DATA && cn_mode != DG_CN_COMPUTE)
+  if (cn_mode != DG_CN_FROM_DATA && cn_mode != DG_CN_COMPUTE && cn_mode != DG_CN_COMPUTE_WEIGHTED)
     return dg_fail(ctx, DG_ERR_ARG, "unknown cn_mode %d", cn_mode);
   DG_HIP(ctx, hipSetDevice(ctx->device));
   for (uint64_t i = 0; i < e_num; ++i)
@@ -317,21 +328,26 @@ extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* e
   std::vector<uint64_t> fill(rp.begin(), rp.end() - 1);
   std::vector<uint32_t> nbr(m);
   std::vector<uint32_t> cn;
-  if (cn_mode == DG_CN_FROM_DATA) cn.resize(m);
+  std::vector<float> weight;
+  const bool from_data = cn_mode == DG_CN_FROM_DATA, weighted = cn_mode == DG_CN_COMPUTE_WEIGHTED;
+  if (from_data) cn.resize(m);
+  if (weighted) weight.resize(m);
   for (uint64_t i = 0; i < e_num; ++i) {
     const uint64_t p = fill[edges[i].src]++;
     nbr[p] = edges[i].dst;
     // walk.hpp:920 converts the float edge data to int
-    const uint32_t c = cn_mode == DG_CN_FROM_DATA ? (uint32_t)(int)edges[i].data : 0u;
-    if (cn_mode == DG_CN_FROM_DATA) cn[p] = c;
+    const uint32_t c = from_data ? (uint32_t)(int)edges[i].data : 0u;
+    if (from_data) cn[p] = c;
+    if (weighted) weight[p] = edges[i].data;
     if (undirected) {
       const uint64_t q = fill[edges[i].dst]++;
       nbr[q] = edges[i].src;
-      if (cn_mode == DG_CN_FROM_DATA) cn[q] = c;
+      if (from_data) cn[q] = c;
+      if (weighted) weight[q] = edges[i].data;
     }
   }
   dg_walk_free(ctx);  // walker state belongs to the previous graph
-  return finish_graph(ctx, v_num, rp, nbr, cn, cn_mode);
+  return finish_graph(ctx, v_num, rp, nbr, cn, weight, cn_mode);
 }
 
 extern "C" int dg_graph_load_bin(dg_ctx* ctx, const char* path, uint32_t v_num, int undirected,

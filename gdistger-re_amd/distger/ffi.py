@@ -16,6 +16,7 @@ _LIB_NAME = "libdistger_hip.so"
 DG_OK = 0
 DG_CN_FROM_DATA = 0
 DG_CN_COMPUTE = 1
+DG_CN_COMPUTE_WEIGHTED = 2
 DG_ROUNDS_REFERENCE = 0
 DG_ROUNDS_CONVERGE = 1
 DG_MATH_SQRT, DG_MATH_LOG2, DG_MATH_LOG, DG_MATH_TANH, DG_MATH_POW = 0, 1, 2, 3, 4

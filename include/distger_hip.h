@@ -71,6 +71,11 @@ int dg_sync(dg_ctx* ctx);
                               reference's binary input does (gcn -> gconverter,
                               walk.hpp:920 reads candidate->data) */
 #define DG_CN_COMPUTE 1    /* compute |N(u) & N(v) \ {u,v}| on the device */
+#define DG_CN_COMPUTE_WEIGHTED 2 /* compute the counts on the device; the edge data
+                                    is the weight w(u,v) and the walker accepts a
+                                    move with tanh(pa * w(u,v)), the paper's weighted
+                                    form (DistGER_Full_Version.pdf p.3, sec. 2.1; the
+                                    reference code has no weighted walk) */
 
 /* Build the device CSR from `e_num` edges.  Replaces GraphEngine::load_graph
  * (include/graph.hpp:642-824) for one partition: optional undirected doubling

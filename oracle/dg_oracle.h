@@ -37,6 +37,7 @@ typedef struct {
   const uint64_t* row_ptr; /* v_num + 1 */
   const uint32_t* nbr;     /* m */
   const uint32_t* cn;      /* m, common-neighbour count per entry */
+  const float* w;          /* m edge weights, or NULL: unweighted acceptance */
 } dgo_graph;
 
 typedef struct {

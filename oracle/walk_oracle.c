@@ -238,7 +238,9 @@ int64_t dgo_walk_one(const dgo_graph* g, const dgo_walk_cfg* cfg, uint32_t round
     int sd = (int)deg;
     int dd = (int)(g->row_ptr[cand + 1] - g->row_ptr[cand]);
     double pa = (1.0 / (double)(sd - cnv)) * (double)(sd > dd ? sd : dd) / (double)(sd < dd ? sd : dd);
-    double pn = O_TANH(pa);                                /* :933-934 */
+    /* :933-934; a weighted graph accepts with tanh(pa * w(u, v)), the paper's
+     * weighted form (DistGER_Full_Version.pdf p.3 sec. 2.1), not in the code */
+    double pn = O_TANH(g->w ? pa * (double)g->w[e] : pa);
     int trace = (pn <= p);                                 /* :938 */
     if (len + 2 > cap) return -1;
     ++step;                                                /* :961-970 */

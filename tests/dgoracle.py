@@ -22,6 +22,7 @@ class _Graph(ctypes.Structure):
         ("row_ptr", ctypes.c_void_p),
         ("nbr", ctypes.c_void_p),
         ("cn", ctypes.c_void_p),
+        ("w", ctypes.c_void_p),
     ]
 
 
@@ -79,7 +80,7 @@ def load(variant="dg"):
 class OracleGraph:
     """CSR built by the oracle from 12-byte edges (reference neighbour order)."""
 
-    def __init__(self, v_num, edges, undirected=True, cn=None, variant="dg"):
+    def __init__(self, v_num, edges, undirected=True, cn=None, variant="dg", weighted=False):
         L = load(variant)
         self.L = L
         edges = np.ascontiguousarray(edges, dtype=EDGE_DTYPE)
@@ -100,8 +101,10 @@ class OracleGraph:
             self.cn = self.data.astype(np.int64).astype(np.uint32)
         else:
             self.cn = np.ascontiguousarray(cn, dtype=np.uint32)
+        # weighted: the edge data is w(u, v) and the acceptance tanh(pa * w)
+        self.w = self.data if weighted else None
         self._g = _Graph(v_num, m, self.row_ptr.ctypes.data, self.nbr.ctypes.data,
-                         self.cn.ctypes.data)
+                         self.cn.ctypes.data, self.w.ctypes.data if weighted else None)
 
     @property
     def degrees(self):

@@ -328,3 +328,45 @@ def test_reference_parameters_fix_walk_length(c1_edges):
     keys = set(zip(g.nbr.tolist(), np.repeat(np.arange(10000), np.diff(g.row_ptr).astype(np.int64)).tolist()))
     sel = np.nonzero(same_walk)[0][:20000]
     assert all((int(dst[i]), int(src[i])) in keys for i in sel)
+
+
+def test_weighted_acceptance_closed_form():
+    """Weighted graphs accept a move u->v with tanh(pa * w(u,v)).  On a star
+    with centre 0, leaves 1..4 and weight w on every edge, a walk from a leaf
+    moves to the centre (pa = 4 / (1 * (1 - 0)) = 4) and bounces back with
+    probability 1 - tanh(4 w); after a move the centre draws the start leaf
+    again with probability 1/4.  So P(path[2] = start) = 1 - tanh(4w) +
+    tanh(4w) / 4: 0.3747 at w = 0.3 (0.2505 if the weight were ignored)."""
+    w = 0.3
+    e = edges_of([(0, 1), (0, 2), (0, 3), (0, 4)])
+    e["data"] = w
+    g = dgoracle.OracleGraph(5, e, weighted=True)
+    cfg = g.cfg(seed=21, min_length=100, max_length=3)
+    n = 40000
+    same = 0
+    for k in range(n):
+        wid = 5 * k + 1 + (k % 4)  # start at leaf 1..4
+        path, _ = g.walk_one(cfg, 1, wid)
+        same += int(path[2]) == wid % 5
+    expect = 1.0 - math.tanh(4 * w) + math.tanh(4 * w) / 4
+    sigma = math.sqrt(expect * (1 - expect) / n)
+    assert abs(same / n - expect) < 5 * sigma, (same / n, expect)
+
+
+def test_weighted_unit_weights_equal_unweighted():
+    """w = 1 on every edge reproduces the unweighted corpus exactly; R-MAT
+    weights U(0, 1] change it."""
+    import distger
+    edges = distger.gen_rmat(10, 8, 0.57, 0.19, 0.19, 4)
+    ones = edges.copy()
+    ones["data"] = 1.0
+    V = 1024
+    cfg_args = dict(seed=5, min_length=20, max_length=0, mu=0.999)
+    gu = dgoracle.OracleGraph(V, ones)
+    g1 = dgoracle.OracleGraph(V, ones, weighted=True)
+    gw = dgoracle.OracleGraph(V, edges, weighted=True)
+    tu, ou = gu.walk_round(gu.cfg(**cfg_args), 1, 0, 3 * V, threads=4)
+    t1, o1 = g1.walk_round(g1.cfg(**cfg_args), 1, 0, 3 * V, threads=4)
+    tw, ow = gw.walk_round(gw.cfg(**cfg_args), 1, 0, 3 * V, threads=4)
+    assert np.array_equal(ou, o1) and np.array_equal(tu, t1)
+    assert not (np.array_equal(ou, ow) and np.array_equal(tu, tw))

@@ -234,3 +234,53 @@ def test_device_math_matches_host(ctx):
             ref = np.array([host(ctypes.c_double(a), ctypes.c_double(b)) for a, b in zip(x, y)])
         bad = np.flatnonzero(dev.view(np.uint64) != ref.view(np.uint64))
         assert len(bad) == 0, f"fn {fn}: {len(bad)} of {len(x)} differ, e.g. x={x[bad[0]]!r}"
+
+
+@pytest.fixture(scope="module")
+def rmat_weighted():
+    """R-MAT scale 12, edge factor 8, (0.57, 0.19, 0.19, 0.05), weights
+    U(0, 1], seed 4 (the weighted-walk configuration at small scale)."""
+    return distger.gen_rmat(12, 8, 0.57, 0.19, 0.19, 4)
+
+
+def test_weighted_round_bit_exact(ctx, rmat_weighted):
+    """Weighted walks (acceptance tanh(pa * w)) on the device against the
+    oracle: corpus and visit counts bit for bit."""
+    print("[walker-gpu] weighted R-MAT round parity", flush=True)
+    V = 1 << 12
+    ctx.graph_from_edges(V, rmat_weighted, undirected=True, cn_mode=ffi.DG_CN_COMPUTE_WEIGHTED)
+    g = dgoracle.OracleGraph(V, rmat_weighted, weighted=True)
+    rp, nbr, cn = ctx.graph_csr()
+    assert np.array_equal(rp, g.row_ptr) and np.array_equal(nbr, g.nbr[: g.m])
+    assert np.array_equal(cn, g.cn[: g.m])
+    p = ctx.default_walk_params(seed=13)
+    ctx.walk_reset()
+    try:  # isolated R-MAT vertices are never visited: relative entropy undefined
+        ctx.walk_round(p, 1, 0, 5 * V)
+    except distger.DgError as e:
+        assert e.code == -7
+    tok, off = ctx.corpus()
+    otok, ooff = g.walk_round(g.cfg(13, 20, 0, 0.999), 1, 0, 5 * V, threads=8)
+    assert np.array_equal(off, ooff) and np.array_equal(tok, otok)
+    assert np.array_equal(ctx.vertex_freq(), np.bincount(otok, minlength=V).astype(np.uint32))
+
+
+def test_weighted_thresholds_definition(ctx, rmat_weighted):
+    """thr = min{u : tanh(pa * w) <= u / (2^31-1)} with the oracle's tanh."""
+    V = 1 << 12
+    ctx.graph_from_edges(V, rmat_weighted, undirected=True, cn_mode=ffi.DG_CN_COMPUTE_WEIGHTED)
+    g = dgoracle.OracleGraph(V, rmat_weighted, weighted=True)
+    L = dgoracle.load("dg")
+    rp, nbr, cn = ctx.graph_csr()
+    thr = ctx.graph_thresholds()
+    deg = np.diff(rp).astype(np.int64)
+    src = np.repeat(np.arange(len(deg)), deg)
+    M = 2147483647.0
+    rng = np.random.default_rng(1)
+    for e in rng.choice(len(nbr), 4000, replace=False):
+        sd, dd, c = int(deg[src[e]]), int(deg[nbr[e]]), int(cn[e])
+        pa = (1.0 / float(sd - c)) * float(max(sd, dd)) / float(min(sd, dd))
+        pn = L.dgo_tanh(pa * float(g.w[e]))
+        t = int(thr[e])
+        assert pn <= t / M
+        assert t == 0 or not (pn <= (t - 1) / M)
