@@ -20,10 +20,12 @@
 //
 // Common neighbours (src/tools/gcn.cpp:250-373, the offline tool whose
 // output the reference walker reads back as edge data, walk.hpp:920) are
-// computed on the device: one wavefront per source vertex hashes N(u) into
-// LDS, then probes each neighbour's list 64 entries at a time and counts hits
-// with a ballot; sources with more than kSmallDeg neighbours use a whole
-// workgroup and a larger LDS table, in chunks if needed.
+// computed on the device over the distinct neighbours: one wavefront per
+// source vertex hashes N(u) into LDS, then probes the neighbour lists 64
+// entries at a time and counts hits with a ballot; a source with more than
+// kSmallDeg neighbours gets a hash table in global memory and its entries are
+// shared out to many wavefronts.  On a simple undirected graph each pair is
+// counted once, from its larger-degree endpoint scanning the smaller list.
 //
 // Acceptance threshold: the walker bounces back when tanh(pa) <= p with
 // p = u / (2^31 - 1), u the 31-bit draw (walk.hpp:928-938).  pa depends only
@@ -32,6 +34,7 @@
 // division), and the walker tests `u >= threshold` - the same decision bit
 // for bit, without tanh or the neighbour's degree in the step loop.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <thread>
 
@@ -43,11 +46,12 @@ namespace {
 constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr int kSmallDeg = 1024;          // wave path: hash of 2048 slots per wave
 constexpr int kSmallSlots = 2048;
-constexpr int kBigSlots = 16384;         // block path: 64 KiB table, chunks of 8192
-constexpr int kBigChunk = 8192;
 
+// Multiplicative (Fibonacci) hashing keeps the high bits of the product:
+// the low bits depend only on the low bits of w, and vertex ids of generated
+// graphs (R-MAT in particular) are far from uniform in their low bits.
 __device__ __forceinline__ uint32_t slot_hash(uint32_t w, uint32_t mask) {
-  return (w * 2654435761u) & mask;
+  return (w * 2654435761u) >> __builtin_clz(mask);  // mask = 2^k - 1, k >= 6
 }
 
 __device__ __forceinline__ void hash_insert(uint32_t* tbl, uint32_t mask, uint32_t w) {
@@ -69,9 +73,24 @@ __device__ __forceinline__ bool hash_find(const uint32_t* tbl, uint32_t mask, ui
   }
 }
 
+// A pair's count is |N(u) & N(v) \ {u, v}| over the distinct neighbours.
+// With SYM (simple undirected graph, rev[k] = position of the reverse entry
+// of k) each pair is counted once, from its larger-degree endpoint (ties by
+// id): the smaller list is scanned against the hash of the larger one and
+// the count written to both entries.  Cost: the sum over edges of
+// min(deg u, deg v) instead of the sum of squared degrees.
+template <bool SYM>
+__device__ __forceinline__ bool other_side(const uint64_t* rp, uint32_t u, uint32_t du, uint32_t v) {
+  if (!SYM || v == u) return false;
+  const uint32_t dv = (uint32_t)(rp[v + 1] - rp[v]);
+  return dv > du || (dv == du && v > u);
+}
+
 // One wavefront per source vertex with degree <= kSmallDeg.
+template <bool SYM>
 __global__ __launch_bounds__(256) void cn_small_kernel(const uint64_t* rp, const uint32_t* nbr,
-                                                       uint32_t v_num, uint32_t* cn) {
+                                                       uint32_t v_num, const uint64_t* rev,
+                                                       uint32_t* cn) {
   __shared__ uint32_t s_tbl[4][kSmallSlots];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t* tbl = s_tbl[wave];
@@ -94,6 +113,7 @@ __global__ __launch_bounds__(256) void cn_small_kernel(const uint64_t* rp, const
     __builtin_amdgcn_wave_barrier();
     for (uint64_t k = b; k < e; ++k) {
       const uint32_t v = nbr[k];
+      if (other_side<SYM>(rp, u, du, v)) continue;  // wave-uniform
       const uint64_t vb = rp[v], ve = rp[v + 1];
       uint32_t c = 0;
       for (uint64_t j0 = vb; j0 < ve; j0 += 64) {  // wave-uniform trip count
@@ -104,51 +124,80 @@ __global__ __launch_bounds__(256) void cn_small_kernel(const uint64_t* rp, const
         }
         c += (uint32_t)__popcll(__ballot(hit));
       }
-      if (lane == 0) cn[k] = c;
+      if (lane == 0) {
+        cn[k] = c;
+        if (SYM && v != u) cn[rev[k]] = c;
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-// One workgroup per source vertex with degree > kSmallDeg; N(u) is hashed in
-// chunks of kBigChunk entries and the counts of all chunks are added.
-__global__ __launch_bounds__(256) void cn_big_kernel(const uint64_t* rp, const uint32_t* nbr,
-                                                     const uint32_t* big, uint32_t n_big,
-                                                     uint32_t* cn) {
-  __shared__ uint32_t tbl[kBigSlots];
-  __shared__ uint32_t s_cnt[4];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+// Sources with degree > kSmallDeg (hubs): N(u) goes into a hash table of its
+// own in global memory (hub_hash_kernel, one workgroup per hub), and the
+// hub's entries are cut into items of kHubItem consecutive entries, one
+// wavefront per item, so a hub's work spreads over many wavefronts and every
+// neighbour list is scanned once.
+constexpr uint32_t kHubItem = 64;
+
+__global__ __launch_bounds__(256) void hub_hash_kernel(const uint64_t* rp, const uint32_t* nbr,
+                                                       const uint32_t* big, uint32_t n_big,
+                                                       const uint64_t* toff, const uint32_t* tsize,
+                                                       uint32_t* tab) {
   for (uint32_t bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
     const uint32_t u = big[bi];
-    const uint64_t b = rp[u], e = rp[u + 1];
-    for (uint64_t c0 = b; c0 < e; c0 += kBigChunk) {
-      const uint64_t c1 = c0 + kBigChunk < e ? c0 + kBigChunk : e;
-      for (uint32_t i = threadIdx.x; i < (uint32_t)kBigSlots; i += 256) tbl[i] = kEmpty;
-      __syncthreads();
-      for (uint64_t i = c0 + threadIdx.x; i < c1; i += 256) {
-        const uint32_t w = nbr[i];
-        if (w != u) hash_insert(tbl, kBigSlots - 1, w);
-      }
-      __syncthreads();
-      // each wave takes neighbours k = b + wave, b + wave + 4, ...
-      for (uint64_t k = b + wave; k < e; k += 4) {
-        const uint32_t v = nbr[k];
-        const uint64_t vb = rp[v], ve = rp[v + 1];
-        uint32_t c = 0;
-        for (uint64_t j = vb; j < ve; j += 64) {
-          bool hit = false;
-          if (j + lane < ve) {
-            const uint32_t w = nbr[j + lane];
-            hit = (w != v) && (w != u) && hash_find(tbl, kBigSlots - 1, w);
-          }
-          c += (uint32_t)__popcll(__ballot(hit));
-        }
-        if (lane == 0) cn[k] = (c0 == b ? 0u : cn[k]) + c;
-      }
-      __syncthreads();
+    uint32_t* t = tab + toff[bi];
+    const uint32_t mask = tsize[bi] - 1;
+    for (uint64_t i = rp[u] + threadIdx.x; i < rp[u + 1]; i += 256) {
+      const uint32_t w = nbr[i];
+      if (w != u) hash_insert(t, mask, w);
     }
   }
-  (void)s_cnt;
+}
+
+template <bool SYM>
+__global__ __launch_bounds__(256) void hub_cn_kernel(const uint64_t* rp, const uint32_t* nbr,
+                                                     const uint32_t* big, const uint64_t* toff,
+                                                     const uint32_t* tsize, const uint32_t* tab,
+                                                     const uint32_t* it_hub, const uint64_t* it_k0,
+                                                     uint64_t n_items, const uint64_t* rev, uint32_t* cn) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t it = wave; it < n_items; it += nwaves) {
+    const uint32_t bi = it_hub[it];
+    const uint32_t u = big[bi];
+    const uint64_t b = rp[u], e = rp[u + 1];
+    const uint32_t du = (uint32_t)(e - b);
+    const uint32_t* t = tab + toff[bi];
+    const uint32_t mask = tsize[bi] - 1;
+    const uint64_t k0 = it_k0[it];
+    const uint64_t k1 = k0 + kHubItem < e ? k0 + kHubItem : e;
+    for (uint64_t k = k0; k < k1; ++k) {
+      const uint32_t v = nbr[k];
+      if (other_side<SYM>(rp, u, du, v)) continue;  // wave-uniform
+      const uint64_t vb = rp[v], ve = rp[v + 1];
+      uint32_t c = 0;
+      for (uint64_t j = vb; j < ve; j += 64) {
+        bool hit = false;
+        if (j + lane < ve) {
+          const uint32_t w = nbr[j + lane];
+          hit = (w != v) && (w != u) && hash_find(t, mask, w);
+        }
+        c += (uint32_t)__popcll(__ballot(hit));
+      }
+      if (lane == 0) {
+        cn[k] = c;
+        if (SYM && v != u) cn[rev[k]] = c;
+      }
+    }
+  }
+}
+
+__global__ void fill_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 __global__ void degree_kernel(const uint64_t* rp, uint32_t v_num, uint32_t* deg) {
@@ -224,9 +273,177 @@ void dg_graph_free(dg_ctx* ctx) {
 }
 
 // Device CSR from the host arrays; CN from the edge data or computed here.
+// Rows in parallel on the host: lo .. hi of [0, n) per thread.
+template <typename F>
+static void host_rows(uint32_t n, F f) {
+  unsigned t = std::thread::hardware_concurrency();
+  t = t == 0 ? 1 : (t > 16 ? 16 : t);
+  if (n < 4096) t = 1;
+  std::vector<std::thread> th;
+  for (unsigned i = 0; i < t; ++i) {
+    const uint32_t lo = (uint32_t)((uint64_t)n * i / t), hi = (uint32_t)((uint64_t)n * (i + 1) / t);
+    th.emplace_back([=] { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// True if no row lists a neighbour twice or the vertex itself.
+static bool rows_simple(uint32_t v_num, const std::vector<uint64_t>& rp, const std::vector<uint32_t>& nbr) {
+  std::atomic<bool> simple{true};
+  host_rows(v_num, [&](uint32_t lo, uint32_t hi) {
+    std::vector<uint32_t> row;
+    for (uint32_t u = lo; u < hi && simple.load(std::memory_order_relaxed); ++u) {
+      row.assign(nbr.begin() + rp[u], nbr.begin() + rp[u + 1]);
+      std::sort(row.begin(), row.end());
+      for (size_t i = 0; i < row.size(); ++i)
+        if (row[i] == u || (i && row[i] == row[i - 1])) {
+          simple = false;
+          break;
+        }
+    }
+  });
+  return simple;
+}
+
+// Common-neighbour counts on the device.  A simple undirected graph counts
+// each pair once (SYM, with the reverse positions `rev`); a simple directed
+// graph counts every entry; any other graph is counted on its distinct
+// neighbour lists (sorted, without the vertex itself), and each entry takes
+// the count of its pair - the set semantics of gcn.cpp:329-346.
+static int compute_cn(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>& rp,
+                      const std::vector<uint32_t>& nbr, const std::vector<uint64_t>& rev,
+                      bool undirected) {
+  const bool simple = rows_simple(v_num, rp, nbr);
+  const std::vector<uint64_t>* crp = &rp;
+  const std::vector<uint32_t>* cnbr = &nbr;
+  std::vector<uint64_t> srp;
+  std::vector<uint32_t> snbr;
+  if (!simple) {  // distinct neighbour lists
+    srp.assign(v_num + 1, 0);
+    std::vector<std::vector<uint32_t>> rows(v_num);
+    host_rows(v_num, [&](uint32_t lo, uint32_t hi) {
+      for (uint32_t u = lo; u < hi; ++u) {
+        std::vector<uint32_t>& r = rows[u];
+        r.assign(nbr.begin() + rp[u], nbr.begin() + rp[u + 1]);
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        r.erase(std::remove(r.begin(), r.end(), u), r.end());
+      }
+    });
+    for (uint32_t u = 0; u < v_num; ++u) srp[u + 1] = srp[u] + rows[u].size();
+    snbr.resize(srp[v_num]);
+    for (uint32_t u = 0; u < v_num; ++u) std::copy(rows[u].begin(), rows[u].end(), snbr.begin() + srp[u]);
+    crp = &srp;
+    cnbr = &snbr;
+  }
+  const uint64_t cm = (*crp)[v_num];
+  const bool sym = simple && undirected;
+  uint64_t *d_rp = ctx->d_row_ptr, *d_rev = nullptr;
+  uint32_t *d_nbr = ctx->d_nbr, *d_cnv = ctx->d_cn;
+  if (!simple) {
+    DG_HIP(ctx, hipMalloc((void**)&d_rp, sizeof(uint64_t) * (v_num + 1)));
+    DG_HIP(ctx, hipMalloc((void**)&d_nbr, sizeof(uint32_t) * (cm ? cm : 1)));
+    DG_HIP(ctx, hipMalloc((void**)&d_cnv, sizeof(uint32_t) * (cm ? cm : 1)));
+    DG_HIP(ctx, hipMemcpy(d_rp, crp->data(), sizeof(uint64_t) * (v_num + 1), hipMemcpyHostToDevice));
+    if (cm) DG_HIP(ctx, hipMemcpy(d_nbr, cnbr->data(), sizeof(uint32_t) * cm, hipMemcpyHostToDevice));
+  }
+  if (sym && cm) {
+    DG_HIP(ctx, hipMalloc((void**)&d_rev, sizeof(uint64_t) * cm));
+    DG_HIP(ctx, hipMemcpyAsync(d_rev, rev.data(), sizeof(uint64_t) * cm, hipMemcpyHostToDevice, ctx->stream));
+  }
+  // hubs: hash-table offsets and work items
+  std::vector<uint32_t> big, tsize, it_hub;
+  std::vector<uint64_t> toff, it_k0;
+  uint64_t tslots = 0;
+  for (uint32_t v = 0; v < v_num; ++v) {
+    const uint64_t d = (*crp)[v + 1] - (*crp)[v];
+    if (d <= (uint64_t)kSmallDeg) continue;
+    uint32_t sz = 64;
+    while (sz < 2 * d) sz <<= 1;
+    const uint32_t bi = (uint32_t)big.size();
+    big.push_back(v);
+    tsize.push_back(sz);
+    toff.push_back(tslots);
+    tslots += sz;
+    for (uint64_t k0 = (*crp)[v]; k0 < (*crp)[v + 1]; k0 += kHubItem) {
+      it_hub.push_back(bi);
+      it_k0.push_back(k0);
+    }
+  }
+  uint32_t *d_big = nullptr, *d_tsize = nullptr, *d_tab = nullptr, *d_it_hub = nullptr;
+  uint64_t *d_toff = nullptr, *d_it_k0 = nullptr;
+  if (!big.empty()) {
+    DG_HIP(ctx, hipMalloc((void**)&d_big, sizeof(uint32_t) * big.size()));
+    DG_HIP(ctx, hipMalloc((void**)&d_tsize, sizeof(uint32_t) * big.size()));
+    DG_HIP(ctx, hipMalloc((void**)&d_toff, sizeof(uint64_t) * big.size()));
+    DG_HIP(ctx, hipMalloc((void**)&d_tab, sizeof(uint32_t) * tslots));
+    DG_HIP(ctx, hipMalloc((void**)&d_it_hub, sizeof(uint32_t) * it_hub.size()));
+    DG_HIP(ctx, hipMalloc((void**)&d_it_k0, sizeof(uint64_t) * it_k0.size()));
+    DG_HIP(ctx, hipMemcpy(d_big, big.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpy(d_tsize, tsize.data(), sizeof(uint32_t) * big.size(), hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpy(d_toff, toff.data(), sizeof(uint64_t) * big.size(), hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpy(d_it_hub, it_hub.data(), sizeof(uint32_t) * it_hub.size(), hipMemcpyHostToDevice));
+    DG_HIP(ctx, hipMemcpy(d_it_k0, it_k0.data(), sizeof(uint64_t) * it_k0.size(), hipMemcpyHostToDevice));
+  }
+  size_t tc;
+  DG_CHECK(dg_time_begin(ctx, "cn", &tc));
+  const unsigned gs = (unsigned)grid_for(ctx, (uint64_t)v_num, 4);
+  if (!big.empty()) {
+    fill_kernel<<<grid_for(ctx, tslots, 256), 256, 0, ctx->stream>>>(d_tab, tslots, kEmpty);
+    hub_hash_kernel<<<(unsigned)std::min<size_t>(big.size(), (size_t)ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+        d_rp, d_nbr, d_big, (uint32_t)big.size(), d_toff, d_tsize, d_tab);
+  }
+  const unsigned gh = (unsigned)grid_for(ctx, (uint64_t)it_hub.size() * 64, 256);
+  if (sym) {
+    cn_small_kernel<true><<<gs, 256, 0, ctx->stream>>>(d_rp, d_nbr, v_num, d_rev, d_cnv);
+    if (!big.empty())
+      hub_cn_kernel<true><<<gh, 256, 0, ctx->stream>>>(d_rp, d_nbr, d_big, d_toff, d_tsize, d_tab, d_it_hub,
+                                                       d_it_k0, it_hub.size(), d_rev, d_cnv);
+  } else {
+    cn_small_kernel<false><<<gs, 256, 0, ctx->stream>>>(d_rp, d_nbr, v_num, nullptr, d_cnv);
+    if (!big.empty())
+      hub_cn_kernel<false><<<gh, 256, 0, ctx->stream>>>(d_rp, d_nbr, d_big, d_toff, d_tsize, d_tab, d_it_hub,
+                                                        d_it_k0, it_hub.size(), nullptr, d_cnv);
+  }
+  DG_HIP(ctx, hipGetLastError());
+  DG_CHECK(dg_time_end(ctx, tc));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  void* hub_bufs[] = {d_big, d_tsize, d_toff, d_tab, d_it_hub, d_it_k0};
+  for (void* q : hub_bufs)
+    if (q) hipFree(q);
+  if (d_rev) hipFree(d_rev);
+  if (!simple) {
+    // each original entry takes the count of its pair in the distinct lists
+    std::vector<uint32_t> scn(cm), cn(rp[v_num]);
+    if (cm) DG_HIP(ctx, hipMemcpy(scn.data(), d_cnv, sizeof(uint32_t) * cm, hipMemcpyDeviceToHost));
+    host_rows(v_num, [&](uint32_t lo, uint32_t hi) {
+      for (uint32_t u = lo; u < hi; ++u) {
+        const uint32_t* row = snbr.data() + srp[u];
+        const uint64_t len = srp[u + 1] - srp[u];
+        for (uint64_t k = rp[u]; k < rp[u + 1]; ++k) {
+          const uint32_t v = nbr[k];
+          if (v == u) {  // |N(u) \ {u}|
+            cn[k] = (uint32_t)len;
+            continue;
+          }
+          const uint32_t* it = std::lower_bound(row, row + len, v);
+          cn[k] = scn[srp[u] + (uint64_t)(it - row)];
+        }
+      }
+    });
+    hipFree(d_rp);
+    hipFree(d_nbr);
+    hipFree(d_cnv);
+    if (rp[v_num])
+      DG_HIP(ctx, hipMemcpy(ctx->d_cn, cn.data(), sizeof(uint32_t) * rp[v_num], hipMemcpyHostToDevice));
+  }
+  return DG_OK;
+}
+
 static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>& rp,
                         const std::vector<uint32_t>& nbr, const std::vector<uint32_t>& cn_data,
-                        const std::vector<float>& weight, int cn_mode) {
+                        const std::vector<float>& weight, const std::vector<uint64_t>& rev,
+                        bool undirected, int cn_mode) {
   dg_graph_free(ctx);
   const uint64_t m = rp[v_num];
   DG_HIP(ctx, hipMalloc((void**)&ctx->d_row_ptr, sizeof(uint64_t) * (v_num + 1)));
@@ -263,27 +480,8 @@ static int finish_graph(dg_ctx* ctx, uint32_t v_num, const std::vector<uint64_t>
                                                                     ctx->d_deg);
   DG_HIP(ctx, hipGetLastError());
   if (cn_mode == DG_CN_COMPUTE || cn_mode == DG_CN_COMPUTE_WEIGHTED) {
-    std::vector<uint32_t> big;
-    for (uint32_t v = 0; v < v_num; ++v)
-      if (rp[v + 1] - rp[v] > (uint64_t)kSmallDeg) big.push_back(v);
-    size_t tc;
-    DG_CHECK(dg_time_begin(ctx, "cn", &tc));
-    cn_small_kernel<<<grid_for(ctx, (uint64_t)v_num, 4), 256, 0, ctx->stream>>>(
-        ctx->d_row_ptr, ctx->d_nbr, v_num, ctx->d_cn);
-    DG_HIP(ctx, hipGetLastError());
-    if (!big.empty()) {
-      uint32_t* d_big = nullptr;
-      DG_HIP(ctx, hipMalloc((void**)&d_big, sizeof(uint32_t) * big.size()));
-      DG_HIP(ctx, hipMemcpyAsync(d_big, big.data(), sizeof(uint32_t) * big.size(),
-                                 hipMemcpyHostToDevice, ctx->stream));
-      const unsigned g = (unsigned)std::min<size_t>(big.size(), (size_t)ctx->num_cus * 4);
-      cn_big_kernel<<<g, 256, 0, ctx->stream>>>(ctx->d_row_ptr, ctx->d_nbr, d_big,
-                                                (uint32_t)big.size(), ctx->d_cn);
-      DG_HIP(ctx, hipGetLastError());
-      DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      hipFree(d_big);
-    }
-    DG_CHECK(dg_time_end(ctx, tc));
+    DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    DG_CHECK(compute_cn(ctx, v_num, rp, nbr, rev, undirected));
   } else if (m) {
     DG_HIP(ctx, hipMemcpyAsync(ctx->d_cn, cn_data.data(), sizeof(uint32_t) * m,
                                hipMemcpyHostToDevice, ctx->stream));
@@ -329,6 +527,9 @@ extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* e
   std::vector<uint32_t> nbr(m);
   std::vector<uint32_t> cn;
   std::vector<float> weight;
+  std::vector<uint64_t> rev;  // undirected: position of each entry's reverse entry
+  const bool want_rev = undirected && cn_mode != DG_CN_FROM_DATA;
+  if (want_rev) rev.resize(m);
   const bool from_data = cn_mode == DG_CN_FROM_DATA, weighted = cn_mode == DG_CN_COMPUTE_WEIGHTED;
   if (from_data) cn.resize(m);
   if (weighted) weight.resize(m);
@@ -344,10 +545,14 @@ extern "C" int dg_graph_from_edges(dg_ctx* ctx, uint32_t v_num, const dg_edge* e
       nbr[q] = edges[i].src;
       if (from_data) cn[q] = c;
       if (weighted) weight[q] = edges[i].data;
+      if (want_rev) {
+        rev[p] = q;
+        rev[q] = p;
+      }
     }
   }
   dg_walk_free(ctx);  // walker state belongs to the previous graph
-  return finish_graph(ctx, v_num, rp, nbr, cn, weight, cn_mode);
+  return finish_graph(ctx, v_num, rp, nbr, cn, weight, rev, undirected != 0, cn_mode);
 }
 
 extern "C" int dg_graph_load_bin(dg_ctx* ctx, const char* path, uint32_t v_num, int undirected,

@@ -30,6 +30,7 @@
 //     on BF16 MFMA with FP32 accumulation; updates applied in FP32.
 #include <algorithm>
 #include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -986,15 +987,47 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
   s->vocab = (uint32_t)s->vertex_of_rank.size();
   if (s->vocab < 2) return dg_fail(ctx, DG_ERR_ARG, "vocabulary needs at least 2 words");
   const uint32_t vocab = s->vocab, dim = s->dim;
-  // syn0 from the reference LCG, syn1 = 0
+  // syn0 from the reference LCG (word2vec.cu:844-878), syn1 = 0.  Element i
+  // takes the LCG state after i + 1 steps from 1; the threads start from
+  // states reached by jump-ahead (k steps of x -> A x + C compose to
+  // x -> A^k x + C (A^(k-1) + ... + 1) mod 2^64), so the values do not
+  // depend on the thread count.
   std::vector<float> syn0((size_t)vocab * dim);
-  unsigned long long nr = 1;
-  for (size_t i = 0; i < syn0.size(); ++i) {
-    nr = nr * 25214903917ULL + 11;
-    if (p->init_float_ops)
-      syn0[i] = (((nr & 0xFFFF) / 65536.f) - 0.5f) / (float)dim;
-    else
-      syn0[i] = (float)(((double)((float)(nr & 0xFFFF) / (float)65536) - 0.5) / (double)(long long)dim);
+  {
+    const unsigned long long A = 25214903917ULL, Cc = 11ULL;
+    auto jump = [&](unsigned long long k) {  // state after k steps from 1
+      unsigned long long mul = 1, add = 0;     // accumulated map x -> mul x + add
+      unsigned long long pm = A, pa = Cc;      // map of 2^b steps
+      while (k) {
+        if (k & 1) {
+          mul = mul * pm;
+          add = add * pm + pa;
+        }
+        pa = pa * pm + pa;
+        pm = pm * pm;
+        k >>= 1;
+      }
+      return mul * 1ULL + add;
+    };
+    const size_t n = syn0.size();
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+    if (n < ((size_t)1 << 22)) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < nt; ++k) {
+      const size_t lo = n * k / nt, hi = n * (k + 1) / nt;
+      th.emplace_back([&, lo, hi] {
+        unsigned long long nr = jump(lo);
+        for (size_t i = lo; i < hi; ++i) {
+          nr = nr * A + Cc;
+          if (p->init_float_ops)
+            syn0[i] = (((nr & 0xFFFF) / 65536.f) - 0.5f) / (float)dim;
+          else
+            syn0[i] = (float)(((double)((float)(nr & 0xFFFF) / (float)65536) - 0.5) / (double)(long long)dim);
+        }
+      });
+    }
+    for (auto& x : th) x.join();
   }
   // unigram table (double accumulation, word2vec.cu:537-552)
   s->table_size = p->table_size;

@@ -57,6 +57,23 @@ def test_init_matches_oracle(ctx, walked):
     assert np.array_equal(s0, o.syn0) and not s1.any()
 
 
+def test_init_large_vocabulary_matches_oracle():
+    """A vocabulary large enough that the LCG initialisation runs on several
+    host threads from jump-ahead states: the rows equal the single-threaded
+    sequence bit for bit."""
+    V = 50000
+    edges = distger.gen_chung_lu(V, 200000, 2.5, 2, 3)
+    with distger.Context(0) as c:
+        c.graph_from_edges(V, edges, undirected=True, cn_mode=ffi.DG_CN_COMPUTE)
+        rp, _, _ = c.graph_csr()
+        p = c.default_sgns_params(dim=128, table_size=TABLE)
+        c.sgns_init(p)
+        s0, s1 = c.sgns_tables(128)
+    o = dgsgns_oracle.Trainer(2 * np.diff(rp).astype(np.uint64), 128, table_size=TABLE)
+    assert s0.size > (1 << 22)
+    assert np.array_equal(s0, o.syn0) and not s1.any()
+
+
 def test_plain_deterministic_bit_exact(ctx, walked):
     print("[sgns-gpu] plain form, one wavefront", flush=True)
     tok, off, counts = walked

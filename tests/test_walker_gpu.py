@@ -30,8 +30,10 @@ def oracle_graph_from_ctx(ctx):
     return g
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture
 def c1_ctx(ctx, c1_edges):
+    """The shared context with the config-1 graph, loaded for each test (other
+    tests load their own graphs into the same context)."""
     print(f"[walker-gpu] loading config-1 graph: {len(c1_edges)} edges", flush=True)
     ctx.graph_from_edges(10000, c1_edges, undirected=True, cn_mode=distger.ffi.DG_CN_COMPUTE)
     return ctx
@@ -183,6 +185,27 @@ def test_common_neighbours_with_hubs(ctx, hub, v, extra):
     e = _hub_graph(hub, v, extra, hub)
     ctx.graph_from_edges(v, e, cn_mode=distger.ffi.DG_CN_COMPUTE)
     g = dgoracle.OracleGraph(v, e)
+    rp, nbr, cn = ctx.graph_csr()
+    assert np.array_equal(nbr, g.nbr[: g.m])
+    assert np.array_equal(cn, g.cn[: g.m])
+
+
+@pytest.mark.parametrize("undirected", [True, False])
+def test_common_neighbours_multigraph(ctx, undirected):
+    """Duplicate edges and self-loops: every entry gets the distinct-neighbour
+    count of its pair, |N(u) & N(v) \\ {u, v}|, as the oracle's sort-unique
+    merge computes it (also on a 1500-neighbour hub)."""
+    rng = np.random.default_rng(11)
+    v = 3000
+    pairs = [(0, int(w)) for w in rng.integers(1, v, 1500)]       # hub with repeats
+    pairs += [(int(a), int(b)) for a, b in rng.integers(0, v, size=(12000, 2))]
+    pairs += [(int(a), int(a)) for a in rng.integers(0, v, 40)]  # self-loops
+    pairs += pairs[100:400]                                        # duplicated edges
+    e = np.zeros(len(pairs), distger.EDGE_DTYPE)
+    e["src"] = [p[0] for p in pairs]
+    e["dst"] = [p[1] for p in pairs]
+    ctx.graph_from_edges(v, e, undirected=undirected, cn_mode=distger.ffi.DG_CN_COMPUTE)
+    g = dgoracle.OracleGraph(v, e, undirected=undirected)
     rp, nbr, cn = ctx.graph_csr()
     assert np.array_equal(nbr, g.nbr[: g.m])
     assert np.array_equal(cn, g.cn[: g.m])
