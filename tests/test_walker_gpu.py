@@ -307,3 +307,40 @@ def test_weighted_thresholds_definition(ctx, rmat_weighted):
         t = int(thr[e])
         assert pn <= t / M
         assert t == 0 or not (pn <= (t - 1) / M)
+
+
+@pytest.mark.parametrize("scale", [
+    20,
+    pytest.param(25, marks=pytest.mark.skipif(
+        os.environ.get("DG_FULL_SCALE") != "1",
+        reason="R-MAT scale 25 (33.5M vertices, 1.05G entries): about 2 minutes and 70 GB of "
+               "host memory; set DG_FULL_SCALE=1")),
+])
+def test_weighted_rmat_sampled_walks(ctx, scale):
+    """Weighted walks on R-MAT with edge factor 16, (0.57, 0.19, 0.19, 0.05),
+    weights U(0, 1], seed 4: a round of 5 V walks; 300 sampled walks equal the
+    oracle's (built on the device's common-neighbour counts), the visit counts
+    add up to the corpus and every step follows an edge."""
+    V = 1 << scale
+    print(f"[walker-gpu] weighted R-MAT scale {scale}", flush=True)
+    edges = distger.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 4)
+    ctx.graph_from_edges(V, edges, undirected=True, cn_mode=ffi.DG_CN_COMPUTE_WEIGHTED)
+    p = ctx.default_walk_params(seed=4)
+    ctx.walk_reset()
+    try:
+        ctx.walk_round(p, 1, 0, 5 * V)
+    except distger.DgError as e:
+        assert e.code == -7  # isolated vertices: relative entropy undefined
+    tok, off = ctx.corpus()
+    assert int(ctx.vertex_freq().sum(dtype=np.uint64)) == len(tok)
+    rp, nbr, cn = ctx.graph_csr()
+    g = dgoracle.OracleGraph(V, edges, weighted=True, cn=cn)
+    assert np.array_equal(g.row_ptr, rp)
+    rng = np.random.default_rng(scale)
+    cfg = g.cfg(4, 20, 0, 0.999)
+    for w in rng.integers(0, 5 * V, 300):
+        path, _ = g.walk_one(cfg, 1, int(w))
+        dev = tok[off[w]:off[w + 1]]
+        assert np.array_equal(path, dev), f"walk {w} differs"
+        for a, b in zip(dev[:-1], dev[1:]):  # consecutive vertices are adjacent
+            assert b in nbr[rp[a]:rp[a + 1]]
