@@ -54,6 +54,7 @@ extern "C" void dg_close(dg_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  dg_pipe_free(ctx);
   dg_tuples_free(ctx);
   dg_sgns_free(ctx);
   dg_walk_free(ctx);
@@ -63,6 +64,7 @@ extern "C" void dg_close(dg_ctx* ctx) {
     hipEventDestroy(p.stop);
   }
   for (auto e : ctx->event_pool) hipEventDestroy(e);
+  if (ctx->d_stamps) hipFree(ctx->d_stamps);
   if (ctx->d_queue) hipFree(ctx->d_queue);
   if (ctx->h_queue) hipHostFree(ctx->h_queue);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -89,7 +91,66 @@ static int take_event(dg_ctx* ctx, hipEvent_t* ev) {
   return DG_OK;
 }
 
+namespace {
+constexpr uint32_t kStampIntervals = 4096;
+
+// Writes the device's constant-rate wall clock to *slot.  Kernels of one
+// captured stream run one after another, so two stamps bracket the work
+// enqueued between them.
+__global__ void stamp_kernel(unsigned long long* slot) {
+  if (threadIdx.x == 0) *slot = (unsigned long long)wall_clock64();
+}
+}  // namespace
+
+int dg_stamps_begin(dg_ctx* ctx) {
+  if (!ctx->d_stamps) {
+    DG_HIP(ctx, hipMalloc((void**)&ctx->d_stamps, sizeof(unsigned long long) * 2 * kStampIntervals));
+    ctx->stamps_cap = kStampIntervals;
+  }
+  if (ctx->wall_khz <= 0.0) {
+    int khz = 0;
+    DG_HIP(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    if (khz <= 0) return dg_fail(ctx, DG_ERR_HIP, "device reports no wall-clock rate");
+    ctx->wall_khz = (double)khz;
+  }
+  ctx->stamp_names.clear();
+  ctx->stamping = true;
+  return DG_OK;
+}
+
+void dg_stamps_end(dg_ctx* ctx) { ctx->stamping = false; }
+
+int dg_stamps_resolve(dg_ctx* ctx, double* walk_ms, double* sgns_ms) {
+  *walk_ms = *sgns_ms = 0.0;
+  const size_t n = ctx->stamp_names.size();
+  if (n == 0) return DG_OK;
+  std::vector<unsigned long long> h(2 * n);
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  DG_HIP(ctx, hipMemcpy(h.data(), ctx->d_stamps, sizeof(unsigned long long) * 2 * n,
+                        hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    const double ms = (double)(h[2 * i + 1] - h[2 * i]) / ctx->wall_khz;
+    const std::string& name = ctx->stamp_names[i];
+    auto& t = ctx->timing[name];
+    t.launches += 1;
+    t.total_ms += ms;
+    if (name == "walk") *walk_ms += ms;
+    if (name.rfind("sgns", 0) == 0) *sgns_ms += ms;
+  }
+  ctx->stamp_names.clear();
+  return DG_OK;
+}
+
 int dg_time_begin(dg_ctx* ctx, const char* name, size_t* token) {
+  if (ctx->stamping) {
+    const size_t i = ctx->stamp_names.size();
+    if (i >= ctx->stamps_cap) return dg_fail(ctx, DG_ERR_STATE, "too many timed intervals in one capture");
+    ctx->stamp_names.push_back(name);
+    stamp_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_stamps + 2 * i);
+    DG_HIP(ctx, hipGetLastError());
+    *token = i;
+    return DG_OK;
+  }
   dg_pending_event p;
   p.name = name;
   DG_CHECK(take_event(ctx, &p.start));
@@ -101,6 +162,12 @@ int dg_time_begin(dg_ctx* ctx, const char* name, size_t* token) {
 }
 
 int dg_time_end(dg_ctx* ctx, size_t token) {
+  if (ctx->stamping) {
+    if (token >= ctx->stamp_names.size()) return dg_fail(ctx, DG_ERR_STATE, "bad timing token");
+    stamp_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_stamps + 2 * token + 1);
+    DG_HIP(ctx, hipGetLastError());
+    return DG_OK;
+  }
   if (token >= ctx->pending.size()) return dg_fail(ctx, DG_ERR_STATE, "bad timing token");
   DG_HIP(ctx, hipEventRecord(ctx->pending[token].stop, ctx->stream));
   return DG_OK;
@@ -115,8 +182,13 @@ int dg_time_resolve(dg_ctx* ctx) {
     auto& t = ctx->timing[p.name];
     t.launches += 1;
     t.total_ms += ms;
-    ctx->event_pool.push_back(p.start);
-    ctx->event_pool.push_back(p.stop);
+  }
+  // back to the pool in reverse, so the next sequence of dg_time_begin calls
+  // takes the same events in the same order (a recaptured graph then has the
+  // same event nodes and updates in place)
+  for (size_t i = ctx->pending.size(); i-- > 0;) {
+    ctx->event_pool.push_back(ctx->pending[i].stop);
+    ctx->event_pool.push_back(ctx->pending[i].start);
   }
   ctx->pending.clear();
   return DG_OK;

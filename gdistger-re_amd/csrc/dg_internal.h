@@ -46,6 +46,8 @@ struct dg_pending_event {
 };
 
 struct dg_sgns_state;   // defined by the trainer translation unit
+struct dg_walk_plan;    // defined by the walker translation unit
+struct dg_pipe_state;   // defined by the pipeline translation unit
 struct dg_tuple_state;  // defined by the tuple-interface translation unit
 
 struct dg_ctx {
@@ -95,6 +97,7 @@ struct dg_ctx {
   double* d_re_terms = nullptr;      // v_num
   double* d_re_chunks = nullptr;
   size_t re_cap = 0;
+  dg_walk_plan* wplan = nullptr;     // the round planned for the captured step
   uint64_t cur_walks = 0;
   uint64_t cur_tokens = 0;
   double prev_re = 0.0;
@@ -110,9 +113,17 @@ struct dg_ctx {
 
   dg_sgns_state* sgns = nullptr;
   dg_tuple_state* tuples = nullptr;
+  dg_pipe_state* pipe = nullptr;
 
   // timing
   std::map<std::string, dg_timing_entry> timing;
+  // inside a captured graph (events recorded there cannot be timed): device
+  // wall-clock stamps, two per timed interval, named in stamp_names
+  bool stamping = false;
+  unsigned long long* d_stamps = nullptr;
+  uint32_t stamps_cap = 0;  // intervals
+  std::vector<std::string> stamp_names;
+  double wall_khz = 0.0;
   std::vector<dg_pending_event> pending;
   std::vector<hipEvent_t> event_pool;
 };
@@ -181,6 +192,12 @@ int dg_grow_keep(dg_ctx* ctx, T** ptr, size_t* cap, size_t need, size_t keep) {
 int dg_time_begin(dg_ctx* ctx, const char* name, size_t* token);
 int dg_time_end(dg_ctx* ctx, size_t token);
 int dg_time_resolve(dg_ctx* ctx);
+// Device-stamp timing for work that is being captured: begin (allocates; call
+// before the capture), end (after it), resolve (after the graph has run: adds
+// the intervals to the per-family totals and returns the walk and trainer sums).
+int dg_stamps_begin(dg_ctx* ctx);
+void dg_stamps_end(dg_ctx* ctx);
+int dg_stamps_resolve(dg_ctx* ctx, double* walk_ms, double* sgns_ms);
 
 // Graph teardown (graph.hip), trainer teardown (sgns.hip), tuple-interface
 // teardown (tuples.hip).
@@ -205,6 +222,23 @@ struct dg_table_view {
 int dg_sgns_view(dg_ctx* ctx, dg_table_view* v);
 // Record a write to the tables; returns the new version.
 uint64_t dg_sgns_written(dg_ctx* ctx);
+
+// The walk round and the trainer epoch as the captured pipeline step runs
+// them: plan (validation, launch shapes, buffers sized for any outcome; may
+// allocate), enqueue (kernels only, no host waits), collect (after the
+// stream has finished).  A collect that sets *retry found the round too large
+// for its token buffer: the buffer has grown and nothing else changed.
+int dg_walk_graph_plan(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                       uint32_t walker_begin, uint32_t walker_num, uint64_t* tok_bound,
+                       uint64_t* slot_tokens, uint64_t* overflow_tokens);
+int dg_walk_graph_enqueue(dg_ctx* ctx, bool timed);
+int dg_walk_graph_collect(dg_ctx* ctx, double walk_ms, double total_ms, dg_round_stats* out,
+                          bool* retry);
+int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* p, float alpha, uint64_t n_walks,
+                       uint64_t tok_bound, uint64_t slot_tokens, uint64_t overflow_tokens);
+int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, bool timed);
+int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* out);
+void dg_pipe_free(dg_ctx* ctx);
 
 // Exclusive scan of n uint32 lengths into n+1 uint64 offsets (walker.hip).
 int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_off);

@@ -397,6 +397,32 @@ __global__ void zero_u64_kernel(unsigned long long* p) {
   if (threadIdx.x == 0) *p = 0ull;
 }
 
+// Learning rate of every launch of an epoch from the device corpus offsets:
+// the linear schedule of dsgl.hpp:886-897 evaluated at each launch's first
+// sentence, the same double expression the host evaluates for a direct call,
+// so both give the same float.
+__global__ void alpha_kernel(const uint64_t* woff, uint64_t n_walks, uint64_t mb, uint32_t n_mb,
+                             float alpha0, int linear, uint32_t iter, uint32_t ep, float* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mb) return;
+  if (!linear) {
+    out[i] = alpha0;
+    return;
+  }
+  const double n_tok = (double)(woff[n_walks] - woff[0]);
+  const double total_words = (double)iter * n_tok;
+  const double words_before = (double)ep * n_tok;
+  float al = alpha0 * (float)(1.0 - (words_before + (double)(woff[(uint64_t)i * mb] - woff[0])) /
+                                        (total_words + 1.0));
+  if (al < alpha0 * 0.0001f) al = alpha0 * 0.0001f;
+  out[i] = al;
+}
+
+// Adds an epoch's kept-token count (the last subsampled offset) to *acc.
+__global__ void kept_add_kernel(const uint64_t* sen_off, uint64_t n_walks, unsigned long long* acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *acc += (unsigned long long)sen_off[n_walks];
+}
+
 struct SentArgs {
   const int32_t* sen;
   const uint64_t* sen_off;
@@ -408,6 +434,7 @@ struct SentArgs {
   float* syn0;
   float* syn1;
   float alpha;
+  const float* alpha_dev;  // when set, the learning rate is read here (device schedule)
   float wneg;
   int window;
   int kd;
@@ -460,6 +487,7 @@ void sg_sent_kernel(SentArgs t) {
   for (int i = lane; i < KC * LD; i += 64) L.yb[i] = (E)0;
   __syncthreads();
   const int W = t.window, K = t.kd;
+  const float alpha = t.alpha_dev ? *t.alpha_dev : t.alpha;
   const uint64_t pos_base = t.sen_off[t.s_begin];
   uint64_t wave_id = (uint64_t)blockIdx.x * kSentWaves + wv;
   uint64_t nwaves = (uint64_t)gridDim.x * kSentWaves;
@@ -616,7 +644,7 @@ void sg_sent_kernel(SentArgs t) {
           const int orow_n = __shfl(orow, n, 64);
           const bool live = n <= K && (n == 0 || orow_n != row0);
           const float lab = n == 0 ? 1.0f : 0.0f;
-          const float wt = n == 0 ? t.alpha : t.alpha * t.wneg;
+          const float wt = n == 0 ? alpha : alpha * t.wneg;
           float gyr[KC];
 #pragma unroll
           for (int k = 0; k < KC; ++k) gyr[k] = L.Gy[k * (KC + 1) + n];
@@ -846,6 +874,16 @@ struct dg_sgns_state {
   uint8_t* d_ucount = nullptr;  // batched: distinct vertices per sentence of a launch
   size_t ucount_cap = 0;
   unsigned long long* d_pool_rows = nullptr;  // batched: distinct vertices per sentence, summed
+  // captured pipeline step: learning rate per launch, kept tokens, and the plan
+  float* d_alpha = nullptr;
+  size_t alpha_cap = 0;
+  unsigned long long* d_kept = nullptr;
+  struct {
+    dg_sgns_params p;
+    float alpha0;
+    uint32_t kd, epoch0, n_mb;
+    uint64_t n_walks, mb, pos_bound, sentences0;
+  } g{};
   uint32_t epoch = 0;
   uint64_t sentences_done = 0;
   uint64_t version = 0;     // changes on every write of syn0 / syn1
@@ -859,7 +897,7 @@ void dg_sgns_free(dg_ctx* ctx) {
   if (!s) return;
   void* bufs[] = {s->d_rank,  s->d_syn0,    s->d_syn1, s->d_table,   s->d_expt,
                   s->d_keep,  s->d_sen,     s->d_sen_off, s->d_cnt, s->d_shrink,
-                  s->d_negs,  s->d_ucount,  s->d_pool_rows};
+                  s->d_negs,  s->d_ucount,  s->d_pool_rows, s->d_alpha, s->d_kept};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete s;
@@ -1178,6 +1216,65 @@ static void launch_sent_form(dg_ctx* ctx, const SentArgs& t, uint32_t dim) {
   }
 }
 
+// One launch of the batched trainer over sentences [s0, s1) of the subsampled
+// epoch: the draws, the distinct-vertex counts, then the sentence kernels.
+// The draw kernel loops over the launch's positions as the device offsets
+// give them; pos_grid (an upper bound, 0 = none) only sizes its grid.
+static int enqueue_batch(dg_ctx* ctx, dg_sgns_state* s, const dg_sgns_params& p, uint32_t kd,
+                         uint32_t epoch, uint64_t first_sentence, uint64_t s0, uint64_t s1,
+                         uint64_t pos_grid, float alpha, const float* alpha_dev, bool timed) {
+  DrawArgs da;
+  da.sen = s->d_sen;
+  da.sen_off = s->d_sen_off;
+  da.s_begin = s0;
+  da.s_end = s1;
+  da.first_sentence = first_sentence;
+  da.seed = p.seed;
+  da.epoch = epoch;
+  da.table = s->d_table;
+  da.table_size = s->table_size;
+  da.vocab = s->vocab;
+  da.window = p.window;
+  da.kd = (int)kd;
+  da.shrink = s->d_shrink;
+  da.negs = s->d_negs;
+  da.ucount = s->d_ucount;
+  da.pool_rows = s->d_pool_rows;
+  SentArgs t;
+  t.sen = s->d_sen;
+  t.sen_off = s->d_sen_off;
+  t.s_begin = s0;
+  t.s_end = s1;
+  t.shrink = s->d_shrink;
+  t.negs = s->d_negs;
+  t.ucount = s->d_ucount;
+  t.expt = s->d_expt;
+  t.syn0 = s->d_syn0;
+  t.syn1 = s->d_syn1;
+  t.alpha = alpha;
+  t.alpha_dev = alpha_dev;
+  t.wneg = (float)p.negative / (float)kd;
+  t.window = (int)p.window;
+  t.kd = (int)kd;
+  t.exact = (int)p.exact_sigmoid;
+  t.deterministic = (int)p.deterministic;
+  t.all = 0;
+  size_t tk = 0;
+  if (timed) DG_CHECK(dg_time_begin(ctx, "sgns_prep", &tk));
+  if (pos_grid) sent_draw_kernel<<<grid_for(ctx, pos_grid, 256), 256, 0, ctx->stream>>>(da);
+  sent_count_kernel<<<grid_for(ctx, s1 - s0, 256), 256, 0, ctx->stream>>>(da);
+  if (timed) DG_CHECK(dg_time_end(ctx, tk));
+  if (timed)
+    DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_fp32b", &tk));
+  if (p.form == DG_SGNS_BATCHED_BF16)
+    launch_sent_form<uint16_t>(ctx, t, s->dim);
+  else
+    launch_sent_form<float>(ctx, t, s->dim);
+  DG_HIP(ctx, hipGetLastError());
+  if (timed) DG_CHECK(dg_time_end(ctx, tk));
+  return DG_OK;
+}
+
 extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin, float alpha_in,
                              dg_sgns_stats* out) {
   if (!ctx || !pin) return DG_ERR_ARG;
@@ -1279,54 +1376,9 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
         DG_CHECK(dg_reserve(ctx, &s->d_shrink, &s->shrink_cap, (size_t)n_pos + 1));
         DG_CHECK(dg_reserve(ctx, &s->d_negs, &s->negs_cap, (size_t)n_pos * kd + 1));
         DG_CHECK(dg_reserve(ctx, &s->d_ucount, &s->ucount_cap, (size_t)(s1 - s0) + 1));
-        DrawArgs da;
-        da.sen = s->d_sen;
-        da.sen_off = s->d_sen_off;
-        da.s_begin = s0;
-        da.s_end = s1;
-        da.first_sentence = s->sentences_done;
-        da.seed = p.seed;
-        da.epoch = epoch;
-        da.table = s->d_table;
-        da.table_size = s->table_size;
-        da.vocab = s->vocab;
-        da.window = p.window;
-        da.kd = (int)kd;
-        da.shrink = s->d_shrink;
-        da.negs = s->d_negs;
-        da.ucount = s->d_ucount;
-        da.pool_rows = s->d_pool_rows;
-        SentArgs t;
-        t.sen = s->d_sen;
-        t.sen_off = s->d_sen_off;
-        t.s_begin = s0;
-        t.s_end = s1;
-        t.shrink = s->d_shrink;
-        t.negs = s->d_negs;
-        t.ucount = s->d_ucount;
-        t.expt = s->d_expt;
-        t.syn0 = s->d_syn0;
-        t.syn1 = s->d_syn1;
-        t.alpha = alpha = alpha_at(s0);
-        t.wneg = (float)p.negative / (float)kd;
-        t.window = (int)p.window;
-        t.kd = (int)kd;
-        t.exact = (int)p.exact_sigmoid;
-        t.deterministic = (int)p.deterministic;
-        t.all = 0;
-        size_t tk;
-        DG_CHECK(dg_time_begin(ctx, "sgns_prep", &tk));
-        if (n_pos)
-          sent_draw_kernel<<<grid_for(ctx, n_pos, 256), 256, 0, ctx->stream>>>(da);
-        sent_count_kernel<<<grid_for(ctx, s1 - s0, 256), 256, 0, ctx->stream>>>(da);
-        DG_CHECK(dg_time_end(ctx, tk));
-        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_BATCHED_BF16 ? "sgns_bf16" : "sgns_fp32b", &tk));
-        if (p.form == DG_SGNS_BATCHED_BF16)
-          launch_sent_form<uint16_t>(ctx, t, s->dim);
-        else
-          launch_sent_form<float>(ctx, t, s->dim);
-        DG_HIP(ctx, hipGetLastError());
-        DG_CHECK(dg_time_end(ctx, tk));
+        alpha = alpha_at(s0);
+        DG_CHECK(enqueue_batch(ctx, s, p, kd, epoch, s->sentences_done, s0, s1, n_pos, alpha,
+                               nullptr, true));
         st.launches++;
       }
     }
@@ -1350,6 +1402,105 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
   st.ms = ms;
   st.alpha_end = alpha;
   if (out) *out = st;
+  return DG_OK;
+}
+
+// ---- the trainer epoch as the captured pipeline step runs it ---------------
+// The corpus of the round being walked is not known to the host while the
+// step is captured: subsampled buffers are sized by the walker's token bound,
+// each launch's draw buffers by a bound on its positions (slot_tokens per
+// sentence plus the walker's overflow region), and the learning rate of each
+// launch comes from alpha_kernel on the device.
+
+int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, uint64_t n_walks,
+                       uint64_t tok_bound, uint64_t slot_tokens, uint64_t overflow_tokens) {
+  dg_sgns_state* s = ctx->sgns;
+  if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
+  if (!s->has_vocab)
+    return dg_fail(ctx, DG_ERR_STATE, "tables from dg_sgns_tables_create have no vocabulary: run dg_sgns_init to train");
+  DG_CHECK(check_params(ctx, pin));
+  if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
+  if (pin->form == DG_SGNS_PLAIN_FP32)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED,
+                   "the captured pipeline step runs the batched forms (the plain form batches by host offsets)");
+  if (n_walks == 0) return dg_fail(ctx, DG_ERR_STATE, "empty corpus");
+  auto& g = s->g;
+  g.p = *pin;
+  g.alpha0 = alpha_in > 0 ? alpha_in : pin->alpha;
+  g.kd = pin->negative_draws ? pin->negative_draws : pin->negative;
+  g.mb = pin->minibatch ? pin->minibatch : 131072;
+  g.n_walks = n_walks;
+  g.n_mb = (uint32_t)((n_walks + g.mb - 1) / g.mb);
+  const uint64_t per_launch = (n_walks < g.mb ? n_walks : g.mb) * slot_tokens + overflow_tokens;
+  g.pos_bound = per_launch < tok_bound ? per_launch : tok_bound;
+  g.epoch0 = s->epoch;
+  g.sentences0 = s->sentences_done;
+  DG_CHECK(dg_reserve(ctx, &s->d_cnt, &s->cnt_cap, (size_t)n_walks + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_sen_off, &s->sen_off_cap, (size_t)n_walks + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_sen, &s->sen_cap, (size_t)tok_bound + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_shrink, &s->shrink_cap, (size_t)g.pos_bound + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_negs, &s->negs_cap, (size_t)g.pos_bound * g.kd + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_ucount, &s->ucount_cap, (size_t)(n_walks < g.mb ? n_walks : g.mb) + 1));
+  DG_CHECK(dg_reserve(ctx, &s->d_alpha, &s->alpha_cap, (size_t)g.n_mb * pin->iter));
+  if (!s->d_pool_rows) DG_HIP(ctx, hipMalloc((void**)&s->d_pool_rows, sizeof(unsigned long long)));
+  if (!s->d_kept) DG_HIP(ctx, hipMalloc((void**)&s->d_kept, sizeof(unsigned long long)));
+  return DG_OK;
+}
+
+int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, bool timed) {
+  dg_sgns_state* s = ctx->sgns;
+  const auto& g = s->g;
+  const dg_sgns_params& p = g.p;
+  const uint64_t n_walks = g.n_walks;
+  zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(s->d_pool_rows);
+  zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(s->d_kept);
+  for (uint32_t ep = 0; ep < p.iter; ++ep) {
+    const uint32_t epoch = g.epoch0 + ep;
+    subsample_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
+        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_cnt);
+    DG_CHECK(dg_scan_lengths(ctx, s->d_cnt, n_walks, s->d_sen_off));
+    subsample_write_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
+        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_sen_off, s->d_sen);
+    kept_add_kernel<<<1, 64, 0, ctx->stream>>>(s->d_sen_off, n_walks, s->d_kept);
+    float* alphas = s->d_alpha + (size_t)ep * g.n_mb;
+    alpha_kernel<<<(g.n_mb + 255) / 256, 256, 0, ctx->stream>>>(
+        d_off, n_walks, g.mb, g.n_mb, g.alpha0, p.alpha_schedule == DG_ALPHA_LINEAR ? 1 : 0, p.iter,
+        ep, alphas);
+    DG_HIP(ctx, hipGetLastError());
+    uint32_t i = 0;
+    for (uint64_t s0 = 0; s0 < n_walks; s0 += g.mb, ++i) {
+      const uint64_t s1 = n_walks - s0 < g.mb ? n_walks : s0 + g.mb;
+      DG_CHECK(enqueue_batch(ctx, s, p, g.kd, epoch, g.sentences0 + (uint64_t)ep * n_walks, s0, s1,
+                             g.pos_bound, g.alpha0, alphas + i, timed));
+    }
+  }
+  return DG_OK;
+}
+
+int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* out) {
+  dg_sgns_state* s = ctx->sgns;
+  const auto& g = s->g;
+  if (!commit) return DG_OK;
+  unsigned long long kept = 0, pr = 0;
+  float alpha_end = 0.f;
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  DG_HIP(ctx, hipMemcpy(&kept, s->d_kept, sizeof(kept), hipMemcpyDeviceToHost));
+  DG_HIP(ctx, hipMemcpy(&pr, s->d_pool_rows, sizeof(pr), hipMemcpyDeviceToHost));
+  DG_HIP(ctx, hipMemcpy(&alpha_end, s->d_alpha + (size_t)g.p.iter * g.n_mb - 1, sizeof(float),
+                        hipMemcpyDeviceToHost));
+  s->epoch = g.epoch0 + g.p.iter;
+  s->sentences_done = g.sentences0 + (uint64_t)g.p.iter * g.n_walks;
+  dg_sgns_written(ctx);
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->words = (uint64_t)g.p.iter * ctx->cur_tokens;
+    out->kept = kept;
+    out->sentences = (uint64_t)g.p.iter * g.n_walks;
+    out->launches = (uint64_t)g.p.iter * g.n_mb;
+    out->ms = ms;
+    out->alpha_end = alpha_end;
+    out->pool_rows = pr;
+  }
   return DG_OK;
 }
 

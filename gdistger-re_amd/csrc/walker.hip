@@ -333,6 +333,19 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
   }
 }
 
+// Empties a round that overflowed the token buffer or hit the length limit
+// (every walk length 0), so the kernels after it leave the corpus, the visit
+// counts and the tables as they were.
+__global__ void round_guard_kernel(const dg_walk_queue* q, uint32_t* walk_len, uint32_t n) {
+  if (!(q->flags & (DG_QFLAG_TOKEN_OVERFLOW | DG_QFLAG_LENGTH_LIMIT))) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    walk_len[i] = 0;
+}
+
+__global__ void zero_first_kernel(uint64_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0;
+}
+
 __global__ void queue_reset_kernel(dg_walk_queue* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     q->next = 0;
@@ -546,8 +559,12 @@ __global__ __launch_bounds__(64) void hot_reduce_kernel(const uint32_t* hot_part
 // left to right, then the chunk sums added left to right.
 constexpr uint32_t kReChunk = 1024;
 
+// wsum = the visits counted before this round plus the round's tokens (from
+// the queue, so the host need not know the round's token count).
 __global__ void re_terms_kernel(const uint32_t* deg, const uint32_t* freq, uint32_t v_num,
-                                uint64_t dsum, uint64_t wsum, double* terms, dg_walk_queue* q) {
+                                uint64_t dsum, uint64_t wsum_before, double* terms,
+                                dg_walk_queue* q) {
+  const uint64_t wsum = wsum_before + q->tokens;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v_num;
        i += gridDim.x * blockDim.x) {
     const uint32_t d = deg[i], f = freq[i];
@@ -616,6 +633,18 @@ int grid_for(dg_ctx* ctx, uint64_t n, int block) {
 
 }  // namespace
 
+// A planned round (see plan_round below).
+struct dg_walk_plan {
+  WalkArgs a;
+  uint64_t blocks = 0;        // walk_kernel grid
+  uint32_t parts = 0;         // pack_count_kernel grid
+  uint32_t round = 0;
+  uint32_t walker_num = 0;
+  uint32_t length_limit = 0;
+  uint64_t freq_total_before = 0;
+};
+
+
 void dg_walk_free(dg_ctx* ctx) {
   void* bufs[] = {ctx->d_freq,   ctx->d_tok_raw,  ctx->d_walk_off, ctx->d_walk_len,
                   ctx->d_tokens, ctx->d_offsets,  ctx->d_spill,    ctx->d_scan_tmp,
@@ -638,6 +667,8 @@ void dg_walk_free(dg_ctx* ctx) {
   ctx->d_term_table = nullptr;
   ctx->d_freq_round = nullptr;
   ctx->freq_round_cap = 0;
+  delete ctx->wplan;
+  ctx->wplan = nullptr;
   ctx->tok_raw_cap = ctx->walk_cap = ctx->walk_len_cap = ctx->tokens_cap = ctx->offsets_cap = 0;
   ctx->spill_cap = ctx->scan_tmp_cap = ctx->re_cap = 0;
   ctx->train_tokens_cap = ctx->train_offsets_cap = 0;
@@ -657,7 +688,7 @@ int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_
     scan_tile_apply<<<(unsigned)n_tiles, kScanBlock, 0, ctx->stream>>>(d_len, n, ctx->d_scan_tmp,
                                                                        n_tiles, d_off);
   } else {
-    DG_HIP(ctx, hipMemsetAsync(d_off, 0, sizeof(uint64_t), ctx->stream));
+    zero_first_kernel<<<1, 64, 0, ctx->stream>>>(d_off);
   }
   DG_HIP(ctx, hipGetLastError());
   return DG_OK;
@@ -700,8 +731,15 @@ extern "C" int dg_walk_reset(dg_ctx* ctx) {
   return DG_OK;
 }
 
-static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
-                           uint32_t walker_begin, uint32_t walker_num, dg_round_stats* out) {
+// ---- one round, in three parts ---------------------------------------------
+// plan: validation, launch shapes and buffers (may allocate); enqueue: kernels
+// only, no host waits, so a stream capture can record them; collect: the
+// round's statistics once the stream has finished.  dg_walk_round runs them
+// with a host check of the token buffer between the walk and the packing;
+// the captured pipeline step (pipeline.hip) puts round_guard_kernel there.
+
+static int plan_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                      uint32_t walker_begin, uint32_t walker_num, dg_walk_plan* pl) {
   if (!ctx->d_freq) DG_CHECK(dg_walk_reset(ctx));
   const uint32_t length_limit = p->length_limit ? p->length_limit : kDefaultLengthLimit;
   if (p->max_length != 0 && p->max_length > length_limit)
@@ -730,14 +768,30 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   const uint64_t slots_end = (uint64_t)walker_num * kPathLds;
   size_t tok_need = (size_t)slots_end + (size_t)walker_num / 8 + 4096;
   if (ctx->tok_raw_cap < tok_need) DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, tok_need));
-
-  WalkArgs a;
-  a.row_ptr = ctx->d_row_ptr;
-  a.adj = ctx->d_adj;
-  a.term_table = ctx->d_term_table;
   // this round's visits go to a separate count array, added to the cumulative
   // counts once the round is packed
   DG_CHECK(dg_reserve(ctx, &ctx->d_freq_round, &ctx->freq_round_cap, (size_t)ctx->v_num));
+  {
+    // packing: every resident workgroup slot (LDS: n_hot counters each), fewer for small rounds
+    int pcu = 0;
+    DG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&pcu, pack_count_kernel, kPackBlock, 0));
+    if (pcu < 1) pcu = 1;
+    const uint64_t per_wg = (uint64_t)(kPackBlock / 64) * 2 * kPackWalks;  // walks per workgroup pass
+    const uint64_t need = ((uint64_t)walker_num + per_wg - 1) / per_wg;
+    pl->parts = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * pcu, need ? need : 1);
+    DG_CHECK(dg_reserve(ctx, &ctx->d_hot_part, &ctx->hot_part_cap,
+                        (size_t)pl->parts * (ctx->n_hot ? ctx->n_hot : 1)));
+  }
+  const uint32_t n_chunks = (ctx->v_num + kReChunk - 1) / kReChunk;
+  DG_CHECK(dg_reserve(ctx, &ctx->d_re_terms, &ctx->re_cap, (size_t)ctx->v_num + n_chunks));
+  ctx->d_re_chunks = nullptr;  // chunks live after the terms
+  DG_CHECK(dg_reserve(ctx, &ctx->d_scan_tmp, &ctx->scan_tmp_cap,
+                      (size_t)((walker_num + kScanTile - 1) / kScanTile) + 1));
+
+  WalkArgs& a = pl->a;
+  a.row_ptr = ctx->d_row_ptr;
+  a.adj = ctx->d_adj;
+  a.term_table = ctx->d_term_table;
   a.v_num = ctx->v_num;
   a.round = round;
   a.seed = p->seed;
@@ -752,87 +806,132 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   a.walk_len = ctx->d_walk_len;
   a.spill = ctx->d_spill;
   a.q = ctx->d_queue;
-
   a.slots_end = slots_end;
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    a.tokens = ctx->d_tok_raw;
-    a.tok_cap = ctx->tok_raw_cap;
-    queue_reset_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
-    size_t tk;
-    DG_CHECK(dg_time_begin(ctx, "walk", &tk));
-    walk_kernel<<<(unsigned)blocks, kWalkBlock, 0, ctx->stream>>>(a);
-    DG_HIP(ctx, hipGetLastError());
-    DG_CHECK(dg_time_end(ctx, tk));
-    DG_HIP(ctx, hipMemcpyAsync(ctx->h_queue, ctx->d_queue, sizeof(dg_walk_queue),
-                               hipMemcpyDeviceToHost, ctx->stream));
-    DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->h_queue->flags & DG_QFLAG_LENGTH_LIMIT)
-      return dg_fail(ctx, DG_ERR_WALK_TOO_LONG,
-                     "round %u: a walk reached the length limit %u (raise length_limit)", round,
-                     length_limit);
-    if (!(ctx->h_queue->flags & DG_QFLAG_TOKEN_OVERFLOW)) break;
-    const size_t need =
-        (size_t)(slots_end + ctx->h_queue->cursor + ctx->h_queue->cursor / 8 + 1024);
-    DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, need));
-    if (attempt == 2) return dg_fail(ctx, DG_ERR_STATE, "token buffer sizing did not converge");
-  }
-  const uint64_t n_tok = ctx->h_queue->tokens;
-  const uint32_t max_len = ctx->h_queue->max_len;
-  const uint64_t draws = ctx->h_queue->draws;
-  const uint64_t moves = ctx->h_queue->moves;
+  a.tokens = ctx->d_tok_raw;
+  a.tok_cap = ctx->tok_raw_cap;
+  pl->blocks = blocks;
+  pl->round = round;
+  pl->walker_num = walker_num;
+  pl->length_limit = length_limit;
+  pl->freq_total_before = ctx->freq_total;
+  return DG_OK;
+}
 
-  // pack in walk-id order and count the round's visits
-  DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, (size_t)n_tok));
-  size_t tp;
-  DG_CHECK(dg_time_begin(ctx, "pack", &tp));
+// The walk itself: queue reset and the persistent launch.
+static int enqueue_walk(dg_ctx* ctx, const dg_walk_plan& pl, bool timed) {
+  queue_reset_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
+  size_t tk = 0;
+  if (timed) DG_CHECK(dg_time_begin(ctx, "walk", &tk));
+  walk_kernel<<<(unsigned)pl.blocks, kWalkBlock, 0, ctx->stream>>>(pl.a);
+  DG_HIP(ctx, hipGetLastError());
+  if (timed) DG_CHECK(dg_time_end(ctx, tk));
+  return DG_OK;
+}
+
+// Everything after the walk: packing in walk-id order with the round's visit
+// counts, the cumulative counts, and the relative entropy.  Reads only device
+// state, so it needs no host knowledge of the round's token count.
+static int enqueue_tail(dg_ctx* ctx, const dg_walk_plan& pl, bool timed) {
+  const uint32_t walker_num = pl.walker_num;
+  size_t tp = 0;
+  if (timed) DG_CHECK(dg_time_begin(ctx, "pack", &tp));
   DG_CHECK(dg_scan_lengths(ctx, ctx->d_walk_len, walker_num, ctx->d_offsets));
-  {
-    // every resident workgroup slot (LDS: n_hot counters each), fewer for small rounds
-    int per_cu = 0;
-    DG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pack_count_kernel, kPackBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    const uint64_t per_wg = (uint64_t)(kPackBlock / 64) * 2 * kPackWalks;  // walks per workgroup pass
-    const uint64_t need = ((uint64_t)walker_num + per_wg - 1) / per_wg;
-    uint32_t parts = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, need ? need : 1);
-    DG_CHECK(dg_reserve(ctx, &ctx->d_hot_part, &ctx->hot_part_cap, (size_t)parts * (ctx->n_hot ? ctx->n_hot : 1)));
-    fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq_round,
-                                                                             ctx->v_num, 0u);
-    pack_count_kernel<<<parts, kPackBlock, 0, ctx->stream>>>(
-        ctx->d_tok_raw, ctx->d_walk_off, ctx->d_walk_len, ctx->d_offsets, walker_num, ctx->d_tokens,
-        ctx->d_hot_slot, ctx->n_hot, ctx->d_freq_round, ctx->d_hot_part);
-    if (ctx->n_hot) {
-      const dim3 grid((ctx->n_hot + 63) / 64, (parts + kHotRows - 1) / kHotRows);
-      hot_reduce_kernel<<<grid, 64, 0, ctx->stream>>>(ctx->d_hot_part, parts, ctx->n_hot,
-                                                      ctx->d_hot_id, ctx->d_freq_round);
-    }
+  fill_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(ctx->d_freq_round,
+                                                                           ctx->v_num, 0u);
+  pack_count_kernel<<<pl.parts, kPackBlock, 0, ctx->stream>>>(
+      ctx->d_tok_raw, ctx->d_walk_off, ctx->d_walk_len, ctx->d_offsets, walker_num, ctx->d_tokens,
+      ctx->d_hot_slot, ctx->n_hot, ctx->d_freq_round, ctx->d_hot_part);
+  if (ctx->n_hot) {
+    const dim3 grid((ctx->n_hot + 63) / 64, (pl.parts + kHotRows - 1) / kHotRows);
+    hot_reduce_kernel<<<grid, 64, 0, ctx->stream>>>(ctx->d_hot_part, pl.parts, ctx->n_hot,
+                                                    ctx->d_hot_id, ctx->d_freq_round);
   }
   DG_HIP(ctx, hipGetLastError());
-  DG_CHECK(dg_time_end(ctx, tp));
+  if (timed) DG_CHECK(dg_time_end(ctx, tp));
   add_u32_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
       ctx->d_freq, ctx->d_freq_round, ctx->v_num);
   DG_HIP(ctx, hipGetLastError());
-  ctx->freq_total += n_tok;
-  ctx->cur_walks = walker_num;
-  ctx->cur_tokens = n_tok;
 
   // relative entropy against the cumulative visit counts
   const uint32_t n_chunks = (ctx->v_num + kReChunk - 1) / kReChunk;
-  DG_CHECK(dg_reserve(ctx, &ctx->d_re_terms, &ctx->re_cap, (size_t)ctx->v_num + n_chunks));
-  ctx->d_re_chunks = nullptr;  // chunks live after the terms
   double* chunks = ctx->d_re_terms + ctx->v_num;
-  size_t tr;
-  DG_CHECK(dg_time_begin(ctx, "re", &tr));
+  size_t tr = 0;
+  if (timed) DG_CHECK(dg_time_begin(ctx, "re", &tr));
   re_terms_kernel<<<grid_for(ctx, ctx->v_num, 256), 256, 0, ctx->stream>>>(
-      ctx->d_deg, ctx->d_freq, ctx->v_num, ctx->m, ctx->freq_total, ctx->d_re_terms, ctx->d_queue);
+      ctx->d_deg, ctx->d_freq, ctx->v_num, ctx->m, pl.freq_total_before, ctx->d_re_terms,
+      ctx->d_queue);
   re_chunks_kernel<<<(n_chunks + 3) / 4, 256, 0, ctx->stream>>>(ctx->d_re_terms, ctx->d_deg,
                                                                 ctx->d_freq, ctx->v_num, chunks,
                                                                 n_chunks);
   re_final_kernel<<<1, 64, 0, ctx->stream>>>(chunks, n_chunks, ctx->d_queue);
   DG_HIP(ctx, hipGetLastError());
-  DG_CHECK(dg_time_end(ctx, tr));
+  if (timed) DG_CHECK(dg_time_end(ctx, tr));
+  return DG_OK;
+}
+
+// The round's statistics from the queue mirror (the stream has finished and
+// h_queue holds the final queue).  Advances the host-side round state.
+static int collect_round(dg_ctx* ctx, const dg_walk_plan& pl, double walk_ms, double total_ms,
+                         dg_round_stats* out) {
+  const dg_walk_queue& q = *ctx->h_queue;
+  const uint64_t n_tok = q.tokens;
+  ctx->freq_total = pl.freq_total_before + n_tok;
+  ctx->cur_walks = pl.walker_num;
+  ctx->cur_tokens = n_tok;
+  const double re = q.re;
+  const double delta = ctx->rounds_done == 0 ? re : re - ctx->prev_re;
+  ctx->prev_re = re;
+  ctx->rounds_done += 1;
+  if (out) {
+    out->round = pl.round;
+    out->walkers = pl.walker_num;
+    out->tokens = n_tok;
+    out->steps = n_tok - pl.walker_num;
+    out->re = re;
+    out->delta_re = delta;
+    out->walk_ms = walk_ms;
+    out->total_ms = total_ms;
+    out->max_walk_len = q.max_len;
+    out->draws = q.draws;
+    out->moves = q.moves;
+    out->stop = 0;
+  }
+  if (q.zero_terms != 0)
+    return dg_fail(ctx, DG_ERR_RE_UNDEFINED,
+                   "relative entropy undefined: %u vertices with degree 0 or no visits",
+                   q.zero_terms);
+  return DG_OK;
+}
+
+static int read_queue(dg_ctx* ctx) {
   DG_HIP(ctx, hipMemcpyAsync(ctx->h_queue, ctx->d_queue, sizeof(dg_walk_queue),
                              hipMemcpyDeviceToHost, ctx->stream));
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DG_OK;
+}
+
+static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                           uint32_t walker_begin, uint32_t walker_num, dg_round_stats* out) {
+  dg_walk_plan pl;
+  DG_CHECK(plan_round(ctx, p, round, walker_begin, walker_num, &pl));
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    pl.a.tokens = ctx->d_tok_raw;
+    pl.a.tok_cap = ctx->tok_raw_cap;
+    DG_CHECK(enqueue_walk(ctx, pl, true));
+    DG_CHECK(read_queue(ctx));
+    if (ctx->h_queue->flags & DG_QFLAG_LENGTH_LIMIT)
+      return dg_fail(ctx, DG_ERR_WALK_TOO_LONG,
+                     "round %u: a walk reached the length limit %u (raise length_limit)", round,
+                     pl.length_limit);
+    if (!(ctx->h_queue->flags & DG_QFLAG_TOKEN_OVERFLOW)) break;
+    const size_t need =
+        (size_t)(pl.a.slots_end + ctx->h_queue->cursor + ctx->h_queue->cursor / 8 + 1024);
+    DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, need));
+    if (attempt == 2) return dg_fail(ctx, DG_ERR_STATE, "token buffer sizing did not converge");
+  }
+  DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, (size_t)ctx->h_queue->tokens));
+  DG_CHECK(enqueue_tail(ctx, pl, true));
+  DG_CHECK(read_queue(ctx));
 
   double walk_ms = 0.0, total_ms = 0.0;
   for (auto& pe : ctx->pending) {
@@ -842,30 +941,7 @@ static int walk_round_impl(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
     total_ms += ms;
   }
   DG_CHECK(dg_time_resolve(ctx));
-
-  const double re = ctx->h_queue->re;
-  const double delta = ctx->rounds_done == 0 ? re : re - ctx->prev_re;
-  ctx->prev_re = re;
-  ctx->rounds_done += 1;
-  if (out) {
-    out->round = round;
-    out->walkers = walker_num;
-    out->tokens = n_tok;
-    out->steps = n_tok - walker_num;
-    out->re = re;
-    out->delta_re = delta;
-    out->walk_ms = walk_ms;
-    out->total_ms = total_ms;
-    out->max_walk_len = max_len;
-    out->draws = draws;
-    out->moves = moves;
-    out->stop = 0;
-  }
-  if (ctx->h_queue->zero_terms != 0)
-    return dg_fail(ctx, DG_ERR_RE_UNDEFINED,
-                   "relative entropy undefined: %u vertices with degree 0 or no visits",
-                   ctx->h_queue->zero_terms);
-  return DG_OK;
+  return collect_round(ctx, pl, walk_ms, total_ms, out);
 }
 
 extern "C" int dg_walk_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
@@ -875,6 +951,53 @@ extern "C" int dg_walk_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t roun
   if (!(p->mu > 0.0)) return dg_fail(ctx, DG_ERR_ARG, "mu must be positive");
   DG_HIP(ctx, hipSetDevice(ctx->device));
   return walk_round_impl(ctx, p, round, walker_begin, walker_num, out);
+}
+
+// ---- the round as the captured pipeline step runs it -----------------------
+
+int dg_walk_graph_plan(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
+                       uint32_t walker_begin, uint32_t walker_num, uint64_t* tok_bound,
+                       uint64_t* slot_tokens, uint64_t* overflow_tokens) {
+  if (!ctx->d_row_ptr) return dg_fail(ctx, DG_ERR_STATE, "no graph loaded");
+  if (!(p->mu > 0.0)) return dg_fail(ctx, DG_ERR_ARG, "mu must be positive");
+  if (!ctx->wplan) ctx->wplan = new dg_walk_plan();
+  DG_CHECK(plan_round(ctx, p, round, walker_begin, walker_num, ctx->wplan));
+  // the packed corpus can hold every token the raw buffer can
+  DG_CHECK(dg_reserve(ctx, &ctx->d_tokens, &ctx->tokens_cap, ctx->tok_raw_cap));
+  *tok_bound = ctx->tok_raw_cap;
+  *slot_tokens = kPathLds;
+  *overflow_tokens = ctx->tok_raw_cap - ctx->wplan->a.slots_end;
+  return DG_OK;
+}
+
+int dg_walk_graph_enqueue(dg_ctx* ctx, bool timed) {
+  const dg_walk_plan& pl = *ctx->wplan;
+  DG_CHECK(enqueue_walk(ctx, pl, timed));
+  // a round that overflowed the token buffer or hit the length limit becomes
+  // empty here, so the rest of the step changes nothing and can run again
+  round_guard_kernel<<<grid_for(ctx, pl.walker_num, 256), 256, 0, ctx->stream>>>(
+      ctx->d_queue, ctx->d_walk_len, pl.walker_num);
+  DG_HIP(ctx, hipGetLastError());
+  return enqueue_tail(ctx, pl, timed);
+}
+
+int dg_walk_graph_collect(dg_ctx* ctx, double walk_ms, double total_ms, dg_round_stats* out,
+                          bool* retry) {
+  const dg_walk_plan& pl = *ctx->wplan;
+  *retry = false;
+  DG_CHECK(read_queue(ctx));
+  const dg_walk_queue& q = *ctx->h_queue;
+  if (q.flags & DG_QFLAG_LENGTH_LIMIT)
+    return dg_fail(ctx, DG_ERR_WALK_TOO_LONG,
+                   "round %u: a walk reached the length limit %u (raise length_limit)", pl.round,
+                   pl.length_limit);
+  if (q.flags & DG_QFLAG_TOKEN_OVERFLOW) {
+    const size_t need = (size_t)(pl.a.slots_end + q.cursor + q.cursor / 8 + 1024);
+    DG_CHECK(dg_reserve(ctx, &ctx->d_tok_raw, &ctx->tok_raw_cap, need));
+    *retry = true;
+    return DG_OK;
+  }
+  return collect_round(ctx, pl, walk_ms, total_ms, out);
 }
 
 extern "C" int dg_corpus_append(dg_ctx* ctx) {

@@ -176,6 +176,11 @@ def load_library():
         "dg_sgns_init": (c.c_int, [vp, c.POINTER(SgnsParams)]),
         "dg_sgns_train": (c.c_int, [vp, c.c_int, c.POINTER(SgnsParams), c.c_float,
                                     c.POINTER(SgnsStats)]),
+        "dg_pipeline_step": (c.c_int, [vp, c.POINTER(WalkParams), c.c_uint32, c.c_uint32,
+                                       c.c_uint32, c.POINTER(SgnsParams), c.c_float,
+                                       c.POINTER(RoundStats), c.POINTER(SgnsStats)]),
+        "dg_pipeline_info": (c.c_int, [vp, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
+                                       c.POINTER(c.c_uint64)]),
         "dg_sgns_vocab": (c.c_int, [vp, vp, c.POINTER(c.c_uint32)]),
         "dg_sgns_get_tables": (c.c_int, [vp, vp, vp]),
         "dg_sgns_set_tables": (c.c_int, [vp, vp, vp]),
@@ -346,6 +351,22 @@ class Context:
         self._check(self.L.dg_sgns_train(self.h, corpus, ctypes.byref(params), float(alpha),
                                          ctypes.byref(st)))
         return st
+
+    def pipeline_step(self, wparams, round_no, walker_begin, walker_num, sparams, alpha=0.0):
+        """One walk round and one trainer pass over its corpus as one HIP graph
+        (the same results as walk_round followed by sgns_train on the round)."""
+        rs = RoundStats()
+        ss = SgnsStats()
+        self._check(self.L.dg_pipeline_step(self.h, ctypes.byref(wparams), round_no, walker_begin,
+                                            walker_num, ctypes.byref(sparams), float(alpha),
+                                            ctypes.byref(rs), ctypes.byref(ss)))
+        return rs, ss
+
+    def pipeline_info(self):
+        """(instantiations, in-place updates, launches) of the step's graph."""
+        a, b, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.dg_pipeline_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
 
     def sgns_vocab(self):
         n = ctypes.c_uint32()

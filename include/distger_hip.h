@@ -260,6 +260,26 @@ int dg_sgns_score_pairs(dg_ctx* ctx, const uint32_t* src, const uint32_t* dst, u
  * values "%f " (text) or d raw floats (binary), then "\n". */
 int dg_emb_write(dg_ctx* ctx, const char* path, int binary);
 
+/* ---- walk -> train as one HIP graph ------------------------------------
+ * dg_walk_round(ctx, wp, round, walker_begin, walker_num, rs) followed by
+ * dg_sgns_train(ctx, DG_CORPUS_ROUND, sp, alpha, ss), recorded by stream
+ * capture and launched as one HIP graph: no host wait between the walk, the
+ * packing and visit counts, the relative entropy and the trainer launches.
+ * Same results as the two calls (the same bits in deterministic mode).
+ * Batched trainer forms only (DG_ERR_UNSUPPORTED for the plain form).  Every
+ * buffer is sized for any outcome of the round; a round longer than the
+ * walker's token buffer is emptied on the device, the buffer grows and the
+ * step runs again.  rs->total_ms is the whole step; rs->walk_ms and ss->ms the
+ * walk and trainer kernels inside it.  Replaces the walker -> trainer hand-off
+ * through walk files and a SyncQueue (include/type.hpp:29-61,
+ * include/walk.hpp:686, src/word2vec.cu:1285-1300). */
+int dg_pipeline_step(dg_ctx* ctx, const dg_walk_params* wp, uint32_t round, uint32_t walker_begin,
+                     uint32_t walker_num, const dg_sgns_params* sp, float alpha,
+                     dg_round_stats* rs, dg_sgns_stats* ss);
+/* Graph bookkeeping of the context: instantiations, in-place updates, launches. */
+int dg_pipeline_info(dg_ctx* ctx, uint64_t* instantiations, uint64_t* updates,
+                     uint64_t* launches);
+
 /* ---- differentiable tuple interface (forward / hand-written adjoint) ----
  * An autograd-Function-shaped pair over the tables of dg_sgns_init or
  * dg_sgns_tables_create.  A tuple is (center, context, neg_1..neg_K) in table
