@@ -76,19 +76,30 @@ def sgns_sent_bytes(kept, pool_rows, dim, negative):
     return rows * dim * 12 + kept * (5 + 4 * negative)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the BF16 sentence kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy
-    kernel), or None if the summary is absent."""
+def measured_profile(shape):
+    """From the committed rocprofv3 summary of the BF16 sentence kernel: HBM
+    bytes per kept position (FETCH_SIZE and WRITE_SIZE over the timed
+    launches, calibrated against a copy kernel) and the MFMA utilisation, for
+    the launch shape the summary was taken at; None where the summary is
+    absent or was taken at another shape."""
     path = os.path.join(REPO, "profiles", "sg_sent_kernel_bf16_c2_hbm_counters.txt")
+    vals = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
+                if ":" in line and not line.startswith(" "):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    want = f"{shape.get('minibatch')},{shape.get('block_waves')},{shape.get('pool_split')}"
+    if vals.get("shape") != want:
+        return None
+    try:
+        return {"traffic_per_kept": float(vals["traffic_bytes_per_kept_position"]),
+                "mfma_util_pct": float(vals.get("mfma_util_pct", "nan"))}
+    except (KeyError, ValueError):
+        return None
 
 
 def auc_of(ctx, pos, neg):
@@ -257,6 +268,17 @@ def main():
     ap.add_argument("--cpu-sgns-walks", type=int, default=20_000)
     ap.add_argument("--workload", default="chung-lu-2.2m", choices=["chung-lu-2.2m", "rmat25"],
                     help="rmat25: weighted walks and BF16 SGNS on R-MAT scale 25")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape instead of the autotuned one")
+    ap.add_argument("--tune-walks", type=int, default=524288,
+                    help="sentences of the round-1 corpus the autotuner times each shape on")
+    ap.add_argument("--shape", default=None,
+                    help="fixed batched launch shape MINIBATCH,BLOCK_WAVES,POOL_SPLIT instead of "
+                         "the autotuned one (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time direct calls (walk round, then SGNS) instead of the captured step")
+    ap.add_argument("--direct-steps", type=int, default=2,
+                    help="direct-call steps timed after the captured ones, for comparison")
     ap.add_argument("--no-tuples", action="store_true",
                     help="skip the differentiable tuple workload (65,536 Zipf tuples)")
     args = ap.parse_args()
@@ -298,19 +320,43 @@ def main():
         ctx.corpus_append()
         ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
         log(f"round 1: {st1.walkers} walks, {st1.tokens} words, walker {st1.walk_ms:.2f} ms")
+        shape = {}
+        if args.shape:
+            mb, bw, split = (int(x) for x in args.shape.split(","))
+            shape = {"minibatch": mb, "block_waves": bw, "pool_split": split}
+            bf = ctx.default_sgns_params(form=ffi.DG_SGNS_BATCHED_BF16, **sp, **shape)
+            out["launch_shape"] = dict(shape, source="--shape")
+        elif not args.no_tune:
+            t0 = time.time()
+            tuned, d_rate, t_rate = ctx.sgns_autotune(bf, corpus=ffi.DG_CORPUS_ROUND,
+                                                      sample_walks=args.tune_walks)
+            shape = {"minibatch": tuned.minibatch, "block_waves": tuned.block_waves,
+                     "pool_split": tuned.pool_split}
+            out["autotune"] = dict(shape, default_words_per_s=d_rate, tuned_words_per_s=t_rate,
+                                   sample_walks=min(args.tune_walks, st1.walkers),
+                                   seconds=time.time() - t0)
+            bf = tuned
+            log(f"autotune ({time.time() - t0:.1f} s): {shape}, {d_rate:.3e} -> {t_rate:.3e} words/s")
         rnd = 2
+
+        def step(r, graph):
+            if graph:  # walk -> train as one HIP graph
+                return ctx.pipeline_step(wp, r, 0, V, bf)
+            return ctx.walk_round(wp, r, 0, V), ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
+
+        graph = not args.no_graph
         for _ in range(args.warmup):
-            ctx.walk_round(wp, rnd, 0, V)
+            step(rnd, graph)
             ctx.corpus_append()
-            ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
             rnd += 1
         ctx.sync()
         ctx.timing_reset()
         wstats, sstats = [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            wstats.append(ctx.walk_round(wp, rnd, 0, V))
-            sstats.append(ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND))
+            ws, ss = step(rnd, graph)
+            wstats.append(ws)
+            sstats.append(ss)
             rnd += 1
         ctx.sync()
         wall = time.perf_counter() - t0
@@ -324,8 +370,6 @@ def main():
         value = words / wall
         log(f"timed: {args.steps} rounds, {words} words in {wall * 1e3:.1f} ms; walker "
             f"{walk_ms / n_walk:.2f} ms/round, SGNS BF16 {bf_ms / args.steps:.2f} ms/round")
-        for r in range(args.steps):  # keep the timed rounds in the training corpus too
-            pass
         # walker roofline (per launch)
         walk_bytes = walker_bytes(wstats) / n_walk
         walk_gbs = walk_bytes / (walk_ms / n_walk * 1e-3) / 1e9
@@ -351,6 +395,27 @@ def main():
                                      "kept": kept, "pool_rows": pool_rows,
                                      "bytes_per_launch": bf_bytes, "achieved_gbs": bf_gbs}
 
+        if graph:
+            inst, upd, launches = ctx.pipeline_info()
+            cmp = {"captured_ms_per_step": wall * 1e3 / args.steps, "instantiations": inst,
+                   "in_place_updates": upd, "graph_launches": launches,
+                   "kernel_timing": "device wall-clock stamps inside the graph"}
+            if args.direct_steps > 0:
+                ctx.timing_reset()
+                t1 = time.perf_counter()
+                for _ in range(args.direct_steps):
+                    step(rnd, False)
+                    rnd += 1
+                ctx.sync()
+                cmp["direct_ms_per_step"] = (time.perf_counter() - t1) * 1e3 / args.direct_steps
+                n_d, d_ms = ctx.timing("sgns_bf16")
+                cmp["sg_sent_ms_per_launch_hip_events"] = d_ms / n_d
+                cmp["sg_sent_ms_per_launch_stamps"] = bf_ms / n_bf
+            out["walk_train_graph"] = cmp
+            log(f"captured step {cmp['captured_ms_per_step']:.1f} ms vs direct calls "
+                f"{cmp.get('direct_ms_per_step', float('nan')):.1f} ms; graph {inst} instantiation(s), "
+                f"{upd} update(s)")
+
         if not args.no_compare:
             # plain FP32 vs BF16 MFMA on the same accumulated corpus (round 1 + warmup)
             nw, nt = ctx.train_corpus_info()
@@ -358,7 +423,8 @@ def main():
             for name, form in (("bf16_mfma", ffi.DG_SGNS_BATCHED_BF16),
                                ("fp32_batched", ffi.DG_SGNS_BATCHED_FP32),
                                ("fp32_plain", ffi.DG_SGNS_PLAIN_FP32)):
-                p = ctx.default_sgns_params(form=form, **sp)
+                p = ctx.default_sgns_params(form=form, **sp,
+                                            **(shape if form != ffi.DG_SGNS_PLAIN_FP32 else {}))
                 ctx.sgns_init(p)
                 ctx.timing_reset()
                 t1 = time.perf_counter()
@@ -382,6 +448,9 @@ def main():
         if not args.no_tuples:
             out["tuples"] = tuple_workload(ctx)
 
+    shape_used = {"minibatch": bf.minibatch or 131072, "block_waves": bf.block_waves or 2,
+                  "pool_split": bf.pool_split or 16}
+    prof = measured_profile(shape_used)
     line = {
         "metric": "walk steps/s + SGNS words/s at equal AUC; BF16-MFMA vs FP32 plain, 1 GPU",
         "value": value,
@@ -397,11 +466,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": (f"Chung-Lu V={V} E~{args.edges} gamma=2.2 seed=1, 90% train; "
                                 f"per step one walk round of V walks (min_L 20, mu 0.999) + SGNS "
-                                f"BF16-MFMA dim {args.dim} w10 K5 sample 1e-3 on that round"),
+                                f"BF16-MFMA dim {args.dim} w10 K5 sample 1e-3 on that round"
+                                + ("" if args.no_graph else ", captured as one HIP graph")),
                    "global_batch": V, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "sg_sent_kernel<bf16, 128, 16, 8>",
+        "roofline": {"bound": "hbm",
+                     "kernel": (f"sg_sent_kernel<bf16, {args.dim}, 16|32, 8, "
+                                f"{shape_used.get('block_waves')}>, "
+                                f"{shape_used.get('minibatch')} sentences per launch"),
                      "achieved": bf_gbs, "peak": HBM_GBS, "unit": "GB/s",
-                     "frac": bf_gbs / HBM_GBS, "traffic": measured_traffic()},
+                     "frac": bf_gbs / HBM_GBS,
+                     "traffic": (prof["traffic_per_kept"] * kept / n_bf) if prof else None,
+                     "mfma_util_pct": prof["mfma_util_pct"] if prof else None},
         "cpu_baseline": cpu,
     }
     line.update(out)

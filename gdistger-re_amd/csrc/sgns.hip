@@ -242,7 +242,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-constexpr int kSentWaves = 2;   // sentences (wavefronts) per block
 constexpr int kOutCols = 16;    // output columns: the target and up to 15 negatives
 constexpr int kSpan = 64;       // positions of a segment's windows held at once
 constexpr int kMaxSentWindow = 15;
@@ -441,6 +440,8 @@ struct SentArgs {
   int exact;
   int deterministic;
   int all;  // take every sentence (else only those of this pool size)
+  int block_waves;  // launch shape (host side): wavefronts per workgroup, 1, 2 or 4
+  int pool_split;   // launch shape (host side): 16 = small sentences on the 16-row pool; 32 = all on 32 rows
 };
 
 template <typename E, int DIM, int P, int KC>
@@ -464,8 +465,8 @@ struct alignas(16) SentLds {
 // P = 16: sentences of at most 16 distinct vertices (one segment each);
 // P = 32: every other sentence, in segments whose windows hold <= 32.
 // KC output columns: 8 (up to 7 negatives) or 16.
-template <typename E, int DIM, int P, int KC>
-__global__ __launch_bounds__(64 * kSentWaves) __attribute__((amdgpu_waves_per_eu(P == 16 ? 2 : 1)))
+template <typename E, int DIM, int P, int KC, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(P == 16 ? 2 : 1)))
 void sg_sent_kernel(SentArgs t) {
   typedef Chunk<E> C;
   typedef typename C::v_t V;
@@ -475,7 +476,7 @@ void sg_sent_kernel(SentArgs t) {
   constexpr int NPT = P / 16;    // pool tiles
   constexpr int YCH = KC * DIM / 4 / 64;  // float4 chunks per lane for KC output rows
   static_assert(KC % 4 == 0 && KC <= 16 && YCH >= 1, "shape");
-  __shared__ Lds lds_all[kSentWaves];
+  __shared__ Lds lds_all[NW];
   __shared__ float s_expt[kExpTable + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, g4 = lane >> 4;
@@ -489,8 +490,8 @@ void sg_sent_kernel(SentArgs t) {
   const int W = t.window, K = t.kd;
   const float alpha = t.alpha_dev ? *t.alpha_dev : t.alpha;
   const uint64_t pos_base = t.sen_off[t.s_begin];
-  uint64_t wave_id = (uint64_t)blockIdx.x * kSentWaves + wv;
-  uint64_t nwaves = (uint64_t)gridDim.x * kSentWaves;
+  uint64_t wave_id = (uint64_t)blockIdx.x * NW + wv;
+  uint64_t nwaves = (uint64_t)gridDim.x * NW;
   if (t.deterministic) {
     nwaves = 1;
     if (wave_id != 0) wave_id = t.s_end;  // one wavefront, sentences in order
@@ -944,6 +945,8 @@ extern "C" void dg_sgns_params_default(dg_sgns_params* p) {
   p->table_size = 100000000u;
   p->init_float_ops = 0;
   p->negative_draws = 0;
+  p->block_waves = 0;
+  p->pool_split = 0;
   p->seed = 1;
 }
 
@@ -973,6 +976,10 @@ static int check_params(dg_ctx* ctx, const dg_sgns_params* p) {
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched FP32 form supports dim <= 128");
   if (p->form == DG_SGNS_BATCHED_BF16 && p->dim > 256)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched BF16 form supports dim <= 256");
+  if (p->block_waves != 0 && p->block_waves != 1 && p->block_waves != 2 && p->block_waves != 4)
+    return dg_fail(ctx, DG_ERR_ARG, "block_waves %u: 0 (default), 1, 2 or 4", p->block_waves);
+  if (p->pool_split != 0 && p->pool_split != 16 && p->pool_split != 32)
+    return dg_fail(ctx, DG_ERR_ARG, "pool_split %u: 0 (default), 16 or 32", p->pool_split);
   return DG_OK;
 }
 
@@ -1172,27 +1179,49 @@ static unsigned resident_grid(dg_ctx* ctx, Kern k, int block) {
   return (unsigned)per_cu * (unsigned)ctx->num_cus;
 }
 
-template <typename E, int DIM, int P, int KC>
+template <typename E, int DIM, int P, int KC, int NW>
 static void launch_sent(dg_ctx* ctx, const SentArgs& t) {
-  auto k = sg_sent_kernel<E, DIM, P, KC>;
-  uint64_t g = t.deterministic ? 1 : resident_grid(ctx, k, 64 * kSentWaves);
-  const uint64_t need = (t.s_end - t.s_begin + kSentWaves - 1) / kSentWaves;
+  auto k = sg_sent_kernel<E, DIM, P, KC, NW>;
+  uint64_t g = t.deterministic ? 1 : resident_grid(ctx, k, 64 * NW);
+  const uint64_t need = (t.s_end - t.s_begin + NW - 1) / NW;
   if (g > need) g = need;
-  k<<<(unsigned)(g ? g : 1), 64 * kSentWaves, 0, ctx->stream>>>(t);
+  k<<<(unsigned)(g ? g : 1), 64 * NW, 0, ctx->stream>>>(t);
+}
+
+// Four wavefronts per workgroup where their LDS fits in one CU's 160 KB,
+// else two.
+template <typename E, int DIM, int P, int KC>
+static void launch_sent_nw(dg_ctx* ctx, const SentArgs& t) {
+  constexpr bool four = 4 * sizeof(SentLds<E, DIM, P, KC>) + sizeof(float) * (kExpTable + 1) <= 163840;
+  switch (t.block_waves) {
+    case 1: launch_sent<E, DIM, P, KC, 1>(ctx, t); break;
+    case 4:
+      if constexpr (four) {
+        launch_sent<E, DIM, P, KC, 4>(ctx, t);
+        break;
+      }
+      [[fallthrough]];
+    default: launch_sent<E, DIM, P, KC, 2>(ctx, t); break;
+  }
 }
 
 // Hogwild: sentences of <= 16 distinct vertices on the 16-row pool, the
-// others on the 32-row pool.  Deterministic: one wavefront, every sentence
-// in order, on the 32-row pool.
+// others on the 32-row pool (pool_split 16), or every sentence on the 32-row
+// pool (pool_split 32).  Deterministic: one wavefront, every sentence in
+// order, on the 32-row pool.
 template <typename E, int DIM, int KC>
 static void launch_sent_kc(dg_ctx* ctx, SentArgs t) {
   if (t.deterministic) {
     t.all = 1;
-    launch_sent<E, DIM, 32, KC>(ctx, t);
+    t.block_waves = 1;
+    launch_sent_nw<E, DIM, 32, KC>(ctx, t);
+  } else if (t.pool_split == 32) {
+    t.all = 1;
+    launch_sent_nw<E, DIM, 32, KC>(ctx, t);
   } else {
     t.all = 0;
-    launch_sent<E, DIM, 16, KC>(ctx, t);
-    launch_sent<E, DIM, 32, KC>(ctx, t);
+    launch_sent_nw<E, DIM, 16, KC>(ctx, t);
+    launch_sent_nw<E, DIM, 32, KC>(ctx, t);
   }
 }
 
@@ -1259,6 +1288,8 @@ static int enqueue_batch(dg_ctx* ctx, dg_sgns_state* s, const dg_sgns_params& p,
   t.exact = (int)p.exact_sigmoid;
   t.deterministic = (int)p.deterministic;
   t.all = 0;
+  t.block_waves = p.block_waves ? (int)p.block_waves : 2;
+  t.pool_split = p.pool_split ? (int)p.pool_split : 16;
   size_t tk = 0;
   if (timed) DG_CHECK(dg_time_begin(ctx, "sgns_prep", &tk));
   if (pos_grid) sent_draw_kernel<<<grid_for(ctx, pos_grid, 256), 256, 0, ctx->stream>>>(da);
@@ -1275,6 +1306,10 @@ static int enqueue_batch(dg_ctx* ctx, dg_sgns_state* s, const dg_sgns_params& p,
   return DG_OK;
 }
 
+static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, uint64_t n_walks,
+                      uint64_t n_tok, const dg_sgns_params& p, float alpha_in,
+                      dg_sgns_stats* out);
+
 extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin, float alpha_in,
                              dg_sgns_stats* out) {
   if (!ctx || !pin) return DG_ERR_ARG;
@@ -1285,12 +1320,19 @@ extern "C" int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
   DG_CHECK(check_params(ctx, pin));
   if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
   DG_HIP(ctx, hipSetDevice(ctx->device));
-  const dg_sgns_params p = *pin;
   const uint32_t* d_tok = corpus == DG_CORPUS_TRAIN ? ctx->d_train_tokens : ctx->d_tokens;
   const uint64_t* d_off = corpus == DG_CORPUS_TRAIN ? ctx->d_train_offsets : ctx->d_offsets;
   const uint64_t n_walks = corpus == DG_CORPUS_TRAIN ? ctx->train_walks : ctx->cur_walks;
   const uint64_t n_tok = corpus == DG_CORPUS_TRAIN ? ctx->train_tokens : ctx->cur_tokens;
   if (!d_tok || n_walks == 0) return dg_fail(ctx, DG_ERR_STATE, "empty corpus");
+  return train_impl(ctx, d_tok, d_off, n_walks, n_tok, *pin, alpha_in, out);
+}
+
+// p->iter epochs over sentences [0, n_walks) of a device corpus (n_tok tokens).
+static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, uint64_t n_walks,
+                      uint64_t n_tok, const dg_sgns_params& p, float alpha_in,
+                      dg_sgns_stats* out) {
+  dg_sgns_state* s = ctx->sgns;
   std::vector<uint64_t> h_woff(n_walks + 1);
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_HIP(ctx, hipMemcpy(h_woff.data(), d_off, sizeof(uint64_t) * (n_walks + 1), hipMemcpyDeviceToHost));
@@ -1500,6 +1542,132 @@ int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* ou
     out->ms = ms;
     out->alpha_end = alpha_end;
     out->pool_rows = pr;
+  }
+  return DG_OK;
+}
+
+// ---- launch-shape autotuner ------------------------------------------------
+// Coordinate search on the caller's corpus: wavefronts per workgroup x pool
+// split at the default launch size, then sentences per launch with the best
+// shape.  Each candidate trains the sample twice (Hogwild) and keeps its faster
+// device time; the tables and the epoch/sentence counters are copied before
+// and restored after, so tuning leaves no trace in the training state.
+
+extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
+                                uint64_t sample_walks, dg_sgns_params* tuned,
+                                double* words_per_s) {
+  if (!ctx || !pin || !tuned) return DG_ERR_ARG;
+  dg_sgns_state* s = ctx->sgns;
+  if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
+  if (!s->has_vocab)
+    return dg_fail(ctx, DG_ERR_STATE, "tables from dg_sgns_tables_create have no vocabulary: run dg_sgns_init to train");
+  DG_CHECK(check_params(ctx, pin));
+  if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
+  if (pin->form == DG_SGNS_PLAIN_FP32)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "the launch shapes tuned here are those of the batched forms");
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t* d_tok = corpus == DG_CORPUS_TRAIN ? ctx->d_train_tokens : ctx->d_tokens;
+  const uint64_t* d_off = corpus == DG_CORPUS_TRAIN ? ctx->d_train_offsets : ctx->d_offsets;
+  const uint64_t n_walks = corpus == DG_CORPUS_TRAIN ? ctx->train_walks : ctx->cur_walks;
+  if (!d_tok || n_walks == 0) return dg_fail(ctx, DG_ERR_STATE, "empty corpus");
+  uint64_t n = sample_walks ? sample_walks : 524288;
+  if (n > n_walks) n = n_walks;
+  uint64_t off[2] = {0, 0};
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  DG_HIP(ctx, hipMemcpy(&off[0], d_off, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  DG_HIP(ctx, hipMemcpy(&off[1], d_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  const uint64_t n_tok = off[1] - off[0];
+
+  // the training state, copied aside
+  const size_t bytes = sizeof(float) * (size_t)s->vocab * s->dim;
+  float* save0 = nullptr;
+  float* save1 = nullptr;
+  DG_HIP(ctx, hipMalloc((void**)&save0, bytes));
+  if (hipMalloc((void**)&save1, bytes) != hipSuccess) {
+    hipFree(save0);
+    return dg_fail(ctx, DG_ERR_NOMEM, "autotune: no room to save the tables");
+  }
+  const uint32_t epoch0 = s->epoch;
+  const uint64_t sent0 = s->sentences_done;
+  int rc = DG_OK;
+  auto restore = [&]() {
+    hipMemcpyAsync(s->d_syn0, save0, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(s->d_syn1, save1, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(save0);
+    hipFree(save1);
+    s->epoch = epoch0;
+    s->sentences_done = sent0;
+    dg_sgns_written(ctx);
+  };
+  if (hipMemcpyAsync(save0, s->d_syn0, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(save1, s->d_syn1, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+    restore();
+    return dg_fail(ctx, DG_ERR_HIP, "autotune: saving the tables failed");
+  }
+
+  dg_sgns_params base = *pin;
+  base.iter = 1;
+  base.deterministic = 0;
+  // device time of one candidate: the faster of two runs
+  auto rate = [&](const dg_sgns_params& c, double* wps) -> int {
+    double best = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {
+      dg_sgns_stats st;
+      const int r = train_impl(ctx, d_tok, d_off, n, n_tok, c, 0.f, &st);
+      if (r != DG_OK) return r;
+      const double w = st.ms > 0 ? (double)st.words / (st.ms * 1e-3) : 0.0;
+      if (w > best) best = w;
+    }
+    *wps = best;
+    return DG_OK;
+  };
+  dg_sgns_params best = base;
+  best.minibatch = base.minibatch ? base.minibatch : 131072;
+  best.block_waves = base.block_waves ? base.block_waves : 2;
+  best.pool_split = base.pool_split ? base.pool_split : 16;
+  double best_rate = 0.0, default_rate = 0.0, warm = 0.0;
+  rc = rate(best, &warm);  // code and caches warm
+  if (rc == DG_OK) rc = rate(best, &default_rate);
+  best_rate = default_rate;
+  const uint32_t waves[3] = {1, 2, 4};
+  const uint32_t splits[2] = {16, 32};
+  for (int i = 0; i < 3 && rc == DG_OK; ++i)
+    for (int j = 0; j < 2 && rc == DG_OK; ++j) {
+      dg_sgns_params c = best;
+      c.block_waves = waves[i];
+      c.pool_split = splits[j];
+      if (c.block_waves == best.block_waves && c.pool_split == best.pool_split) continue;
+      double r = 0.0;
+      rc = rate(c, &r);
+      if (rc == DG_OK && r > best_rate) {
+        best_rate = r;
+        best.block_waves = c.block_waves;
+        best.pool_split = c.pool_split;
+      }
+    }
+  const uint32_t sizes[5] = {32768, 65536, 131072, 262144, 524288};
+  const uint32_t mb0 = best.minibatch;
+  for (int i = 0; i < 5 && rc == DG_OK; ++i) {
+    if (sizes[i] == mb0 || sizes[i] > n) continue;
+    dg_sgns_params c = best;
+    c.minibatch = sizes[i];
+    double r = 0.0;
+    rc = rate(c, &r);
+    if (rc == DG_OK && r > best_rate) {
+      best_rate = r;
+      best.minibatch = c.minibatch;
+    }
+  }
+  restore();
+  if (rc != DG_OK) return rc;
+  *tuned = *pin;
+  tuned->minibatch = best.minibatch;
+  tuned->block_waves = best.block_waves;
+  tuned->pool_split = best.pool_split;
+  if (words_per_s) {
+    words_per_s[0] = default_rate;
+    words_per_s[1] = best_rate;
   }
   return DG_OK;
 }

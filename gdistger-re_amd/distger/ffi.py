@@ -99,6 +99,8 @@ class SgnsParams(ctypes.Structure):
         ("table_size", ctypes.c_uint32),
         ("init_float_ops", ctypes.c_uint32),
         ("negative_draws", ctypes.c_uint32),
+        ("block_waves", ctypes.c_uint32),
+        ("pool_split", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
     ]
 
@@ -176,6 +178,8 @@ def load_library():
         "dg_sgns_init": (c.c_int, [vp, c.POINTER(SgnsParams)]),
         "dg_sgns_train": (c.c_int, [vp, c.c_int, c.POINTER(SgnsParams), c.c_float,
                                     c.POINTER(SgnsStats)]),
+        "dg_sgns_autotune": (c.c_int, [vp, c.c_int, c.POINTER(SgnsParams), c.c_uint64,
+                                       c.POINTER(SgnsParams), vp]),
         "dg_pipeline_step": (c.c_int, [vp, c.POINTER(WalkParams), c.c_uint32, c.c_uint32,
                                        c.c_uint32, c.POINTER(SgnsParams), c.c_float,
                                        c.POINTER(RoundStats), c.POINTER(SgnsStats)]),
@@ -351,6 +355,15 @@ class Context:
         self._check(self.L.dg_sgns_train(self.h, corpus, ctypes.byref(params), float(alpha),
                                          ctypes.byref(st)))
         return st
+
+    def sgns_autotune(self, params, corpus=DG_CORPUS_TRAIN, sample_walks=0):
+        """Fastest launch shape for `params` on the corpus: (tuned params,
+        default words/s, tuned words/s).  Leaves the tables as they were."""
+        tuned = SgnsParams()
+        rates = np.zeros(2, np.float64)
+        self._check(self.L.dg_sgns_autotune(self.h, corpus, ctypes.byref(params), sample_walks,
+                                            ctypes.byref(tuned), _vp(rates)))
+        return tuned, float(rates[0]), float(rates[1])
 
     def pipeline_step(self, wparams, round_no, walker_begin, walker_num, sparams, alpha=0.0):
         """One walk round and one trainer pass over its corpus as one HIP graph

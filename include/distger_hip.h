@@ -207,12 +207,19 @@ typedef struct {
                               order of every update); 0: concurrent sentences with atomic
                               row updates (Hogwild, as the reference kernels) */
   uint32_t exact_sigmoid;  /* 0: the reference's 1000-entry sigmoid table; 1: 1/(1+e^-f) */
-  uint32_t minibatch;      /* batched: sentences per launch (0 = tuned default) */
+  uint32_t minibatch;      /* batched: sentences per launch (0 = 131072); sets how often the
+                              linear learning rate is updated */
   uint32_t table_size;     /* unigram table entries (1e8 in the reference) */
   uint32_t init_float_ops; /* 0: word2vec.cu init rounding (double), 1: dsgl.hpp (float) */
   uint32_t negative_draws; /* batched: negatives drawn per position, >= negative (0 =
                               negative); each one's gradient is scaled by negative /
                               negative_draws, so the expected update is that of `negative` */
+  uint32_t block_waves;    /* batched launch shape: sentences (wavefronts) per workgroup,
+                              1, 2 or 4 (0 = 2; 4 runs as 2 where four pools exceed a CU's
+                              LDS); does not change results */
+  uint32_t pool_split;     /* batched launch shape: 16 = sentences of <= 16 distinct
+                              vertices on a 16-row pool, the rest on 32 rows; 32 = every
+                              sentence on the 32-row pool (0 = 16); does not change results */
   uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
 } dg_sgns_params;
 
@@ -259,6 +266,17 @@ int dg_sgns_score_pairs(dg_ctx* ctx, const uint32_t* src, const uint32_t* dst, u
 /* Embedding file (dsgl.hpp:1443-1461): "V d\n", then per rank "id " and d
  * values "%f " (text) or d raw floats (binary), then "\n". */
 int dg_emb_write(dg_ctx* ctx, const char* path, int binary);
+
+/* Launch-shape autotuner for the batched forms: times the trainer on the
+ * first sample_walks sentences of a device corpus (0 = 524288) for candidate
+ * shapes (block_waves x pool_split, then minibatch) and writes the fastest
+ * into *tuned (a copy of *p with those three fields set).  The tables and the
+ * draw counters are saved and restored: tuning changes no training state.
+ * words_per_s[0] is the default shape's rate, [1] the chosen one's (may be
+ * NULL).  No reference counterpart: the reference's launch size is the
+ * compile-time MAX_SENTENCE (src/word2vec.cu:35,1002-1040). */
+int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* p, uint64_t sample_walks,
+                     dg_sgns_params* tuned, double* words_per_s);
 
 /* ---- walk -> train as one HIP graph ------------------------------------
  * dg_walk_round(ctx, wp, round, walker_begin, walker_num, rs) followed by

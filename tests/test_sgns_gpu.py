@@ -165,3 +165,57 @@ def test_score_pairs_and_embedding_file(ctx, walked, tmp_path):
     assert head == [str(len(vocab)), "128"]
     assert int(first[0]) == vocab[0] and len(first) == 129
     assert abs(float(first[1]) - s0[0, 0]) < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(1, 16), (1, 32), (2, 32), (4, 16), (4, 32)])
+def test_launch_shapes_train_alike(ctx, walked, shape):
+    """Wavefronts per workgroup and the pool split change only which
+    wavefront trains which sentence: the co-occurrence separation and the
+    table norms match the default shape's (Hogwild noise aside)."""
+    tok, off, counts = walked
+    rng = np.random.default_rng(0)
+    rnd = rng.choice(np.unique(tok), (5000, 2))
+    res = []
+    for bw, split in ((2, 16), shape):
+        p = _params(ctx, form=ffi.DG_SGNS_BATCHED_BF16, deterministic=0, iter=3,
+                    block_waves=bw, pool_split=split)
+        ctx.sgns_init(p)
+        ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
+        s0, s1 = ctx.sgns_tables(128)
+        assert np.isfinite(s0).all() and np.isfinite(s1).all()
+        a = distger.auc(ctx.sgns_score_pairs(tok[:5000], tok[1:5001]),
+                        ctx.sgns_score_pairs(rnd[:, 0], rnd[:, 1]))
+        res.append((a, np.linalg.norm(s0), np.linalg.norm(s1)))
+    (a0, n00, n10), (a1, n01, n11) = res
+    print(f"[sgns-gpu] shape {shape}: auc {a1:.4f} vs {a0:.4f}, norms {n01:.4f}/{n00:.4f} "
+          f"{n11:.4f}/{n10:.4f}", flush=True)
+    assert abs(a1 - a0) < 0.02
+    assert abs(n01 / n00 - 1) < 0.02 and abs(n11 / n10 - 1) < 0.02
+
+
+def test_autotune_leaves_training_state_unchanged(ctx, walked):
+    """Tuning trains the sample many times on saved tables: afterwards a
+    deterministic pass gives the same bits as one on an untouched context
+    state, and the chosen shape is one of the candidates."""
+    p = _params(ctx, form=ffi.DG_SGNS_BATCHED_BF16, deterministic=1,
+                alpha_schedule=ffi.DG_ALPHA_LINEAR)
+    ctx.sgns_init(p)
+    ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
+    ref0, ref1 = ctx.sgns_tables(128)
+    ctx.sgns_init(p)
+    before = ctx.sgns_tables(128)
+    tuned, d_rate, t_rate = ctx.sgns_autotune(p, corpus=ffi.DG_CORPUS_ROUND, sample_walks=600)
+    print(f"[sgns-gpu] autotune: minibatch {tuned.minibatch} block_waves {tuned.block_waves} "
+          f"pool_split {tuned.pool_split}; {d_rate:.3e} -> {t_rate:.3e} words/s", flush=True)
+    after = ctx.sgns_tables(128)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert tuned.block_waves in (1, 2, 4) and tuned.pool_split in (16, 32)
+    assert tuned.minibatch in (32768, 65536, 131072, 262144, 524288)
+    assert d_rate > 0 and t_rate >= d_rate
+    assert tuned.deterministic == 1 and tuned.dim == 128  # the rest is the caller's
+    ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
+    a0, a1 = ctx.sgns_tables(128)
+    assert np.array_equal(a0, ref0) and np.array_equal(a1, ref1)
+    with pytest.raises(distger.DgError) as e:
+        ctx.sgns_autotune(_params(ctx, form=ffi.DG_SGNS_PLAIN_FP32), corpus=ffi.DG_CORPUS_ROUND)
+    assert e.value.code == -8
