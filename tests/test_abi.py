@@ -51,3 +51,34 @@ def test_missing_library_is_an_error(monkeypatch, tmp_path):
     monkeypatch.setattr(ffi, "lib_path", lambda: str(tmp_path / "libdistger_hip.so"))
     with pytest.raises(distger.DgError):
         ffi.load_library()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct the binding passes by pointer has the C compiler's size
+    and field offsets (a field added on one side only would shift the rest)."""
+    import subprocess
+
+    pairs = [("dg_walk_params", ffi.WalkParams), ("dg_round_stats", ffi.RoundStats),
+             ("dg_sgns_params", ffi.SgnsParams), ("dg_sgns_stats", ffi.SgnsStats)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "distger_hip.h"',
+             "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append('  printf("dg_edge size %zu\\n", sizeof(dg_edge));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, cls in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+    assert got[("dg_edge", "size")] == ffi.EDGE_DTYPE.itemsize
