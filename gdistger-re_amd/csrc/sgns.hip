@@ -392,6 +392,13 @@ __global__ void sent_count_kernel(DrawArgs a) {
   }
 }
 
+// A wave-uniform 64-bit value moved to scalar registers.
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p) {
   if (threadIdx.x == 0) *p = 0ull;
 }
@@ -517,10 +524,11 @@ void sg_sent_kernel(SentArgs t) {
     if (wave_id != 0) wave_id = t.s_end;  // one wavefront, sentences in order
   }
   for (uint64_t s = t.s_begin + wave_id; s < t.s_end; s += nwaves) {
-    const int uc = t.ucount[s - t.s_begin];
+    // sentence-wide values are wave-uniform: kept in scalar registers
+    const int uc = __builtin_amdgcn_readfirstlane(t.ucount[s - t.s_begin]);
     if (!t.all && (P == 16) != (uc <= 16)) continue;
-    const uint64_t sb = t.sen_off[s];
-    const int Ls = (int)(t.sen_off[s + 1] - sb);
+    const uint64_t sb = uniform_u64(t.sen_off[s]);
+    const int Ls = __builtin_amdgcn_readfirstlane((int)(t.sen_off[s + 1] - sb));
     if (Ls < 2) continue;
     const int32_t* tok = t.sen + sb;
     const uint8_t* shr = t.shrink + (sb - pos_base);
@@ -536,7 +544,7 @@ void sg_sent_kernel(SentArgs t) {
       // segment is every position whose whole window has joined
       int my_id = -1, np = 0, c_last = lo - 1;
       for (int c = lo; c < Ls && c - lo < kSpan; ++c) {
-        const int v = __shfl(tk, c - lo, 64);
+        const int v = __builtin_amdgcn_readlane(tk, c - lo);
         const uint64_t m = __ballot(lane < np && my_id == v);
         int idx;
         if (m) {
@@ -571,12 +579,12 @@ void sg_sent_kernel(SentArgs t) {
       for (int q = q0; q < q1; ++q) {
         if (t.deterministic) __threadfence();
         // output rows: lane n <= K holds row n (target, then negatives)
-        const int target = __shfl(tk, q - lo, 64);
+        const int target = __builtin_amdgcn_readlane(tk, q - lo);
         int orow = -1;
         if (lane == 0) orow = target;
         else if (lane <= K) orow = ngs[(uint64_t)q * K + lane - 1];
-        const int row0 = __shfl(orow, 0, 64);
-        const int b = shr[q];
+        const int row0 = target;
+        const int b = __builtin_amdgcn_readfirstlane(shr[q]);
         // stage the output rows (rows past K stay zero): all loads issued first
         {
           float4 yv[YCH];
@@ -696,7 +704,7 @@ void sg_sent_kernel(SentArgs t) {
           for (int j = 0; j < J; ++j) gxu[j] = L.Gx[u * GXL + grp + G * j];  // Gx symmetric
           for (int i = 0; i < nctx; ++i) {
             // next context's operands, in flight during this one
-            const int un = i + 1 < nctx ? __builtin_amdgcn_readlane(uctx, i + 1) : u;
+            const int un = __builtin_amdgcn_readlane(uctx, i + 1 < nctx ? i + 1 : i);
             const float s_next = L.S[un * (KC + 1) + n];
             float gx_next[J];
 #pragma unroll
@@ -713,8 +721,9 @@ void sg_sent_kernel(SentArgs t) {
             float gpk[KC];  // g of the earlier occurrences of this vertex, per column
 #pragma unroll
             for (int k = 0; k < KC; ++k) {
-              gpk[k] = k <= K ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_own), k + KC * ou))
-                              : 0.0f;
+              // columns past K (and skipped ones) never receive a g: their
+              // sums read 0, so every column is read without a branch
+              gpk[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_own), k + KC * ou));
               sv += gpk[k] * gyr[k];
             }
             float g = 0.0f;
