@@ -576,15 +576,20 @@ void sg_sent_kernel(SentArgs t) {
         }
       }
       // ---- positions of the segment
+      // the draws of each position are loaded one position ahead
+      int neg_nx = lane >= 1 && lane <= K ? ngs[(uint64_t)q0 * K + lane - 1] : -1;
+      int b_nx = shr[q0];
       for (int q = q0; q < q1; ++q) {
         if (t.deterministic) __threadfence();
         // output rows: lane n <= K holds row n (target, then negatives)
         const int target = __builtin_amdgcn_readlane(tk, q - lo);
-        int orow = -1;
-        if (lane == 0) orow = target;
-        else if (lane <= K) orow = ngs[(uint64_t)q * K + lane - 1];
+        const int orow = lane == 0 ? target : neg_nx;
         const int row0 = target;
-        const int b = __builtin_amdgcn_readfirstlane(shr[q]);
+        const int b = __builtin_amdgcn_readfirstlane(b_nx);
+        if (q + 1 < q1) {
+          neg_nx = lane >= 1 && lane <= K ? ngs[(uint64_t)(q + 1) * K + lane - 1] : -1;
+          b_nx = shr[q + 1];
+        }
         // stage the output rows (rows past K stay zero): all loads issued first
         {
           float4 yv[YCH];
@@ -803,6 +808,14 @@ void sg_sent_kernel(SentArgs t) {
           }
         }
         // ---- dY = Gp^T X + M Y -> output rows
+        int drow[4];
+        bool dok[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int n = 4 * g4 + e;
+          drow[e] = __shfl(orow, n, 64);
+          dok[e] = n <= K && (n == 0 || drow[e] != row0);
+        }
         V am = vzero;
         if (krow && kchk) {
           const float4 mq = *reinterpret_cast<const float4*>(L.Mm + l16 * GPL + 4 * g4);
@@ -831,12 +844,8 @@ void sg_sent_kernel(SentArgs t) {
             dy = C::mma(am, by, dy);
           }
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int n = 4 * g4 + e;
-            const int row = __shfl(orow, n, 64);
-            if (n <= K && (n == 0 || row != row0))
-              atomicAdd(&t.syn1[(uint64_t)row * DIM + 16 * tt + l16], dy[e]);
-          }
+          for (int e = 0; e < 4; ++e)
+            if (dok[e]) atomicAdd(&t.syn1[(uint64_t)drow[e] * DIM + 16 * tt + l16], dy[e]);
         }
         __builtin_amdgcn_wave_barrier();
       }
