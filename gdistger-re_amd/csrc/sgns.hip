@@ -590,25 +590,18 @@ void sg_sent_kernel(SentArgs t) {
           neg_nx = lane >= 1 && lane <= K ? ngs[(uint64_t)(q + 1) * K + lane - 1] : -1;
           b_nx = shr[q + 1];
         }
-        // stage the output rows (rows past K stay zero): all loads issued first
-        {
-          float4 yv[YCH];
+        // the output rows' loads go out first (rows past K stay zero); the
+        // pool rows as they stand and Gx = X X^T, which need only the pool,
+        // are formed while they are in flight
+        float4 yv[YCH];
 #pragma unroll
-          for (int r = 0; r < YCH; ++r) {
-            const int ch = r * 64 + lane;
-            const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
-            const int row = __shfl(orow, n, 64);
-            yv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (n <= K) yv[r] = *reinterpret_cast<const float4*>(t.syn1 + (uint64_t)row * DIM + 4 * c4);
-          }
-#pragma unroll
-          for (int r = 0; r < YCH; ++r) {
-            const int ch = r * 64 + lane;
-            const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
-            if (n <= K) C::put4(L.yb + n * LD + 4 * c4, yv[r].x, yv[r].y, yv[r].z, yv[r].w);
-          }
+        for (int r = 0; r < YCH; ++r) {
+          const int ch = r * 64 + lane;
+          const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
+          const int row = __shfl(orow, n, 64);
+          yv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (n <= K) yv[r] = *reinterpret_cast<const float4*>(t.syn1 + (uint64_t)row * DIM + 4 * c4);
         }
-        // pool rows as they stand
 #pragma unroll
         for (int p = 0; p < NPT; ++p)
 #pragma unroll
@@ -626,13 +619,33 @@ void sg_sent_kernel(SentArgs t) {
           }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        // S^T = Y X^T, Gx = X X^T, Gy = Y Y^T (chunk operands from LDS)
+#pragma unroll
+        for (int p = 0; p < NPT; ++p)
+#pragma unroll
+          for (int p2 = 0; p2 < NPT; ++p2) {
+            f32x4_t gx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt)
+              gx = C::mma(C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4),
+                          C::load4(L.xc + (p2 * 16 + l16) * LD + 16 * tt + 4 * g4), gx);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) L.Gx[(p * 16 + 4 * g4 + e) * GXL + p2 * 16 + l16] = gx[e];
+          }
+#pragma unroll
+        for (int r = 0; r < YCH; ++r) {
+          const int ch = r * 64 + lane;
+          const int n = ch / (DIM / 4), c4 = ch % (DIM / 4);
+          if (n <= K) C::put4(L.yb + n * LD + 4 * c4, yv[r].x, yv[r].y, yv[r].z, yv[r].w);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // S^T = Y X^T, Gy = Y Y^T (chunk operands from LDS)
         {
           f32x4_t gy = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) {
-            const V yv = krow ? C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4) : vzero;
-            gy = C::mma(yv, yv, gy);
+            const V yq = krow ? C::load4(L.yb + l16 * LD + 16 * tt + 4 * g4) : vzero;
+            gy = C::mma(yq, yq, gy);
           }
           if (kchk && krow) {
 #pragma unroll
@@ -650,16 +663,6 @@ void sg_sent_kernel(SentArgs t) {
           if (kchk) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) L.S[(p * 16 + l16) * (KC + 1) + 4 * g4 + e] = st[e];
-          }
-#pragma unroll
-          for (int p2 = 0; p2 < NPT; ++p2) {
-            f32x4_t gx = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int tt = 0; tt < NT; ++tt)
-              gx = C::mma(C::load4(L.xc + (p * 16 + l16) * LD + 16 * tt + 4 * g4),
-                          C::load4(L.xc + (p2 * 16 + l16) * LD + 16 * tt + 4 * g4), gx);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) L.Gx[(p * 16 + 4 * g4 + e) * GXL + p2 * 16 + l16] = gx[e];
           }
         }
         __builtin_amdgcn_wave_barrier();
