@@ -95,27 +95,31 @@ def _rel_err(a, b, ref):
     return d / max(np.abs(ref).max(), 1e-30)
 
 
-@pytest.mark.parametrize("form,tol", [(ffi.DG_SGNS_BATCHED_FP32, 1e-3), (ffi.DG_SGNS_BATCHED_BF16, 3e-2)])
-def test_batched_deterministic_vs_fp64_oracle(ctx, walked, form, tol):
+@pytest.mark.parametrize("form,dim,tol", [(ffi.DG_SGNS_BATCHED_FP32, 128, 1e-3),
+                                          (ffi.DG_SGNS_BATCHED_BF16, 128, 3e-2),
+                                          (ffi.DG_SGNS_BATCHED_FP32, 64, 1e-3),
+                                          (ffi.DG_SGNS_BATCHED_BF16, 64, 3e-2),
+                                          (ffi.DG_SGNS_BATCHED_BF16, 256, 3e-2)])
+def test_batched_deterministic_vs_fp64_oracle(ctx, walked, form, dim, tol):
     """One epoch in one wavefront (sentences in order), exact sigmoid: the
     updated rows within `tol` of the FP64 restatement of the batched forms'
     evaluation (dgo_sg_shared_first_order), relative to the largest update.
     FP32: MFMA accumulation order only; BF16: operands rounded to BF16.
     That restatement is pinned to the sequential order by the CPU tests."""
-    print(f"[sgns-gpu] batched form {form}", flush=True)
+    print(f"[sgns-gpu] batched form {form} dim {dim}", flush=True)
     tok, off, counts = walked
-    p = _params(ctx, form=form, deterministic=1, exact_sigmoid=1)
+    p = _params(ctx, form=form, deterministic=1, exact_sigmoid=1, dim=dim)
     ctx.sgns_init(p)
-    init0, _ = ctx.sgns_tables(128)
+    init0, _ = ctx.sgns_tables(dim)
     ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
-    s0, s1 = ctx.sgns_tables(128)
-    o = dgsgns_oracle.Trainer(counts, 128, table_size=TABLE)
+    s0, s1 = ctx.sgns_tables(dim)
+    o = dgsgns_oracle.Trainer(counts, dim, table_size=TABLE)
     sen, soff = o.subsample(tok, off, 11, 0)
     o.shared_first_order(sen, soff, 10, 5, 0.025, True, 11, 0)
     d0 = o.syn0.astype(np.float64) - init0
     e0 = _rel_err(s0 - init0, o.syn0 - init0, d0)
     e1 = _rel_err(s1, o.syn1, o.syn1)
-    print(f"[sgns-gpu] max rel err syn0 {e0:.3e} syn1 {e1:.3e}", flush=True)
+    print(f"[sgns-gpu] form {form} dim {dim}: max rel err syn0 {e0:.3e} syn1 {e1:.3e}", flush=True)
     assert np.abs(o.syn1).max() > 0
     assert e0 < tol and e1 < tol
 
