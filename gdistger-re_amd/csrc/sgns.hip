@@ -710,6 +710,9 @@ void sg_sent_kernel(SentArgs t) {
           float gxu[J];
 #pragma unroll
           for (int j = 0; j < J; ++j) gxu[j] = L.Gx[u * GXL + grp + G * j];  // Gx symmetric
+          // pool vertices already seen among this position's contexts: only
+          // their g sums can be non-zero (a first occurrence adds nothing)
+          uint64_t seen = 0;
           for (int i = 0; i < nctx; ++i) {
             // next context's operands, in flight during this one
             const int un = __builtin_amdgcn_readlane(uctx, i + 1 < nctx ? i + 1 : i);
@@ -726,18 +729,24 @@ void sg_sent_kernel(SentArgs t) {
             }
             const float cu = __shfl(c_own, n + KC * ou, 64);
             float sv = s_nu + cu;
+            const bool rep = (seen >> u) & 1ull;
+            seen |= 1ull << u;
             float gpk[KC];  // g of the earlier occurrences of this vertex, per column
+            if (rep) {
 #pragma unroll
-            for (int k = 0; k < KC; ++k) {
-              // columns past K (and skipped ones) never receive a g: their
-              // sums read 0, so every column is read without a branch
-              gpk[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_own), k + KC * ou));
-              sv += gpk[k] * gyr[k];
+              for (int k = 0; k < KC; ++k) {
+                // columns past K (and skipped ones) never receive a g: their
+                // sums read 0, so every column is read without a branch
+                gpk[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_own), k + KC * ou));
+                sv += gpk[k] * gyr[k];
+              }
             }
             float g = 0.0f;
             if (live) g = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
+            if (rep) {
 #pragma unroll
-            for (int k = 0; k < KC; ++k) mrow[k] += g * gpk[k];
+              for (int k = 0; k < KC; ++k) mrow[k] += g * gpk[k];
+            }
             if (lane < KC) L.Gs[slot * KC + n] = g;
             if (lane == 0) L.su[slot] = u;
             ++slot;
