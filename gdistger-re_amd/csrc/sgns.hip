@@ -82,11 +82,13 @@ __device__ __forceinline__ float lane_partial(const float (&x)[D], const float (
   return p[0];
 }
 
+// The table lookup reads at a clamped index and selects afterwards, so it
+// has no branch: inside [-kMaxExp, kMaxExp] the index is the reference's.
 __device__ __forceinline__ float sigmoid_of(float f, const float* expt, int exact) {
   if (exact) return (float)(1.0 / (1.0 + exp(-(double)f)));
-  if (f > kMaxExp) return 1.0f;
-  if (f < -kMaxExp) return 0.0f;
-  return expt[(int)((f + kMaxExp) * (kExpTable / kMaxExp / 2))];
+  const float fc = fminf(fmaxf(f, -(float)kMaxExp), (float)kMaxExp);
+  const float v = expt[(int)((fc + kMaxExp) * (kExpTable / kMaxExp / 2))];
+  return f > kMaxExp ? 1.0f : (f < -kMaxExp ? 0.0f : v);
 }
 
 // ---- subsampling (word2vec.cu:1011-1020 with a Philox draw per token) ------
@@ -741,14 +743,16 @@ void sg_sent_kernel(SentArgs t) {
                 sv += gpk[k] * gyr[k];
               }
             }
-            float g = 0.0f;
-            if (live) g = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
+            const float gl = (lab - sigmoid_of(sv, s_expt, t.exact)) * wt;
+            const float g = live ? gl : 0.0f;
             if (rep) {
 #pragma unroll
               for (int k = 0; k < KC; ++k) mrow[k] += g * gpk[k];
             }
-            if (lane < KC) L.Gs[slot * KC + n] = g;
-            if (lane == 0) L.su[slot] = u;
+            // every lane of a column holds the same g, and u is uniform: the
+            // stores need no lane mask
+            L.Gs[slot * KC + n] = g;
+            L.su[slot] = u;
             ++slot;
 #pragma unroll
             for (int j = 0; j < J; ++j) {
