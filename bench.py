@@ -183,13 +183,24 @@ def rmat25(args):
         t0 = time.time()
         ctx.sgns_init(bf)
         log(f"SGNS tables for {V} vertices ({time.time() - t0:.1f} s)")
+        shape = {}
+        if not args.no_tune:
+            t0 = time.time()
+            bf, d_rate, t_rate = ctx.sgns_autotune(bf, corpus=ffi.DG_CORPUS_ROUND,
+                                                   sample_walks=args.tune_walks)
+            shape = {"minibatch": bf.minibatch, "block_waves": bf.block_waves,
+                     "pool_split": bf.pool_split}
+            log(f"autotune ({time.time() - t0:.1f} s): {shape}, {d_rate:.3e} -> {t_rate:.3e} words/s")
+
+        def step(rnd):  # one round of V walks and its training, as one HIP graph
+            return ctx.pipeline_step(wp, rnd, 0, V, bf, allow_undefined_re=True)
+        step(2)  # warmup
         ctx.sync()
         ctx.timing_reset()
-        words = steps = 0
+        words = 0
         t0 = time.perf_counter()
         for r in range(args.steps):
-            walk(2 + r, V)
-            st = ctx.sgns_train(bf, corpus=ffi.DG_CORPUS_ROUND)
+            _, st = step(3 + r)
             words += st.words
         ctx.sync()
         wall = time.perf_counter() - t0
@@ -206,6 +217,7 @@ def rmat25(args):
                         "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS},
                "sgns_bf16_words_per_s": words / (bf_ms * 1e-3),
                "graph": {"vertices": V, "entries": m, "cn_ms": cn_ms, "threshold_ms": th_ms},
+               "launch_shape": shape,
                "hbm_resident_bytes": resident, "hbm_resident_frac": resident / 288e9}
     return {
         "metric": "walk steps/s + SGNS words/s at equal AUC; BF16-MFMA vs FP32 plain, 1 GPU",
@@ -215,7 +227,8 @@ def rmat25(args):
         "data": "synthetic",
         "config": {"workload": "R-MAT scale 25 (33,554,432 vertices), edge factor 16, "
                                "(0.57,0.19,0.19,0.05), weights U(0,1], seed 4; per step one round "
-                               "of V weighted walks + BF16-MFMA SGNS dim 128 w10 K5 on it",
+                               "of V weighted walks + BF16-MFMA SGNS dim 128 w10 K5 on it, "
+                               "captured as one HIP graph",
                    "global_batch": V, "parallelism": "single GPU"},
         **out}
 

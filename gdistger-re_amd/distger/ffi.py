@@ -14,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libdistger_hip.so"
 
 DG_OK = 0
+DG_ERR_ARG, DG_ERR_HIP, DG_ERR_NOMEM, DG_ERR_IO, DG_ERR_STATE = -1, -2, -3, -4, -5
+DG_ERR_WALK_TOO_LONG, DG_ERR_RE_UNDEFINED, DG_ERR_UNSUPPORTED = -6, -7, -8
 DG_CN_FROM_DATA = 0
 DG_CN_COMPUTE = 1
 DG_CN_COMPUTE_WEIGHTED = 2
@@ -365,14 +367,20 @@ class Context:
                                             ctypes.byref(tuned), _vp(rates)))
         return tuned, float(rates[0]), float(rates[1])
 
-    def pipeline_step(self, wparams, round_no, walker_begin, walker_num, sparams, alpha=0.0):
+    def pipeline_step(self, wparams, round_no, walker_begin, walker_num, sparams, alpha=0.0,
+                      allow_undefined_re=False):
         """One walk round and one trainer pass over its corpus as one HIP graph
-        (the same results as walk_round followed by sgns_train on the round)."""
+        (the same results as walk_round followed by sgns_train on the round).
+        allow_undefined_re: a round whose relative entropy is undefined (a
+        vertex never visited, e.g. an isolated one) returns its statistics
+        instead of raising; the round and its training are complete either way."""
         rs = RoundStats()
         ss = SgnsStats()
-        self._check(self.L.dg_pipeline_step(self.h, ctypes.byref(wparams), round_no, walker_begin,
-                                            walker_num, ctypes.byref(sparams), float(alpha),
-                                            ctypes.byref(rs), ctypes.byref(ss)))
+        rc = self.L.dg_pipeline_step(self.h, ctypes.byref(wparams), round_no, walker_begin,
+                                     walker_num, ctypes.byref(sparams), float(alpha),
+                                     ctypes.byref(rs), ctypes.byref(ss))
+        if not (allow_undefined_re and rc == DG_ERR_RE_UNDEFINED):
+            self._check(rc)
         return rs, ss
 
     def pipeline_info(self):
