@@ -296,8 +296,8 @@ def main():
                     help="skip the differentiable tuple workload (65,536 Zipf tuples)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(DESIGN.md describes how the path would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: the walk -> train path runs on a single device, "
+              "so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "rmat25":

@@ -1,7 +1,7 @@
 /*
  * dg_sgns_oracle.h - CPU oracle for the SGNS trainer.  TEST INFRASTRUCTURE
  * ONLY: loaded by the tests, the smoke check and the benchmark's CPU-baseline
- * leg as the checker, never by the product library.
+ * leg, to check results, never by the product library.
  */
 #ifndef DG_SGNS_ORACLE_H
 #define DG_SGNS_ORACLE_H

@@ -15,7 +15,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def c1_edges():
-    """Config-1 graph: Chung-Lu, 10,000 vertices, ~50k edges, gamma 2.5, min degree 2, seed 0."""
+    """Chung-Lu, 10,000 vertices, ~50k edges, gamma 2.5, min degree 2, seed 0."""
     import distger
     return distger.gen_chung_lu(10000, 50000, 2.5, 2, 0)
 

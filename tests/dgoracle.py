@@ -1,7 +1,7 @@
 """ctypes bindings for the CPU oracle (oracle/_build/libdgoracle*.so).
 
 Test infrastructure: used only by the tests, the smoke check and the
-benchmark's CPU-baseline leg, as the checker.  Never on a product path.
+benchmark's CPU-baseline leg, to check results.  Never on a product path.
 """
 
 import ctypes

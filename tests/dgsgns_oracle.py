@@ -1,6 +1,6 @@
 """ctypes bindings for the SGNS half of the CPU oracle (oracle/sgns_oracle.c).
 
-Test infrastructure: the checker for the device trainer, never a product path.
+Test infrastructure: the reference results for the device trainer, never a product path.
 """
 
 import ctypes

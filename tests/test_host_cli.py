@@ -2,7 +2,7 @@
 scan (src/examples/huge_walk.cpp, src/word2vec.cu:1418-1428,1548-1566).
 
 CPU: flag parsing through --print-params (no GPU touched) and its errors.
-GPU: the command line end to end on the config-1 graph, its walk files equal
+GPU: the command line end to end on the 10,000-vertex Chung-Lu graph, its walk files equal
 to the library's own, its embedding file well formed.
 """
 
@@ -71,14 +71,14 @@ def test_graph_and_vertex_count_required():
                          "not found by reading (the walk and training calls it makes pass in "
                          "the other GPU tests), so it stays off the GPU until one is shown")
 def test_cli_end_to_end(tmp_path, c1_edges):
-    """Three rounds on the config-1 graph: the walk files equal those of the
+    """Three rounds on the 10,000-vertex Chung-Lu graph: the walk files equal those of the
     library called directly with the same parameters; the embedding file has
     one row of dim values per vocabulary word."""
     g = str(tmp_path / "c1.bin")
     distger.write_edges_bin(g, c1_edges)
     out = str(tmp_path / "w")
     emb = str(tmp_path / "emb.txt")
-    print("[cli-gpu] huge_walk on the config-1 graph", flush=True)
+    print("[cli-gpu] huge_walk on the 10,000-vertex Chung-Lu graph", flush=True)
     r = run("-g", g, "-v", "10000", "-o", out, "--make-undirected", "-cn", "1", "-max-rounds", "3",
             "-max-length", "80", "-size", "64", "-form", "2", "-emb_output", emb, "-seed", "4",
             timeout=300)

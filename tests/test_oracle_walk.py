@@ -305,7 +305,7 @@ def test_walk_file_bytes(tmp_path):
 
 def test_libm_and_dg_math_give_the_same_corpus(c1_edges):
     """Decisions taken with glibc's pow/log2/sqrt/tanh (as the reference
-    computes them) equal those taken with dg_math.h on config 1."""
+    computes them) equal those taken with dg_math.h on the 10,000-vertex Chung-Lu graph."""
     g = dgoracle.OracleGraph(10000, c1_edges, variant="dg")
     gl = dgoracle.OracleGraph(10000, c1_edges, variant="libm")
     for minl, mu, cap in [(20, 0.999, 80), (3, 0.8, 200)]:
@@ -316,7 +316,7 @@ def test_libm_and_dg_math_give_the_same_corpus(c1_edges):
 
 def test_reference_parameters_fix_walk_length(c1_edges):
     """With mu = 0.999 the first termination check (step minLength+1) always
-    fires on config 1, so every walk holds minLength + 2 vertices."""
+    fires on the 10,000-vertex Chung-Lu graph, so every walk holds minLength + 2 vertices."""
     g = dgoracle.OracleGraph(10000, c1_edges)
     tok, off = g.walk_round(g.cfg(1, 20, 80), 1, 0, 50000, threads=4)
     assert set(np.diff(off).tolist()) == {22}

@@ -25,8 +25,8 @@ TABLE = 1_000_000
 
 @pytest.fixture(scope="module")
 def walked(ctx, c1_edges):
-    """Config-1 graph in HBM and one round of 1000 walks (reference parameters)."""
-    print("[sgns-gpu] walking 1000 walks on the config-1 graph", flush=True)
+    """The 10,000-vertex Chung-Lu graph in HBM and one round of 1000 walks (reference parameters)."""
+    print("[sgns-gpu] walking 1000 walks on the 10,000-vertex Chung-Lu graph", flush=True)
     ctx.graph_from_edges(10000, c1_edges, cn_mode=ffi.DG_CN_COMPUTE)
     p = ctx.default_walk_params(seed=5, max_length=80)
     ctx.walk_reset()

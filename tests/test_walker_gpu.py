@@ -32,9 +32,9 @@ def oracle_graph_from_ctx(ctx):
 
 @pytest.fixture
 def c1_ctx(ctx, c1_edges):
-    """The shared context with the config-1 graph, loaded for each test (other
+    """The shared context with the 10,000-vertex Chung-Lu graph, loaded for each test (other
     tests load their own graphs into the same context)."""
-    print(f"[walker-gpu] loading config-1 graph: {len(c1_edges)} edges", flush=True)
+    print(f"[walker-gpu] loading the 10,000-vertex Chung-Lu graph: {len(c1_edges)} edges", flush=True)
     ctx.graph_from_edges(10000, c1_edges, undirected=True, cn_mode=distger.ffi.DG_CN_COMPUTE)
     return ctx
 
@@ -48,7 +48,7 @@ def test_device_csr_matches_oracle(c1_ctx, c1_edges):
 
 
 @pytest.mark.parametrize("minl,mu,cap,seed", [
-    (20, 0.999, 80, 7),    # config 1: reference termination, max length 80
+    (20, 0.999, 80, 7),    # reference termination, max length 80
     (20, 0.999, 0, 8),     # no cap (reference)
     (3, 0.8, 200, 9),      # uneven lengths, LDS spill beyond 47 vertices
     (4, 0.7, 120, 10),     # many walks reach the cap
