@@ -43,7 +43,10 @@ namespace {
 constexpr int kExpTable = 1000;
 constexpr int kMaxExp = 6;
 constexpr int kMaxNeg = 16;     // -negative
-constexpr int kMaxWindow = 24;  // tile positions + 2 * window <= 64 staged context rows
+// -window: the plain form takes any window up to the reference's sentence
+// length cap (MAX_SENTENCE_LENGTH, src/word2vec.cu:32); the batched forms
+// at most kMaxSentWindow
+constexpr int kMaxWindow = 1000;
 constexpr uint32_t kNoRank = 0xffffffffu;
 
 struct SgnsDev {
