@@ -1594,8 +1594,10 @@ int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* ou
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_HIP(ctx, hipMemcpy(&kept, s->d_kept, sizeof(kept), hipMemcpyDeviceToHost));
   DG_HIP(ctx, hipMemcpy(&pr, s->d_pool_rows, sizeof(pr), hipMemcpyDeviceToHost));
-  DG_HIP(ctx, hipMemcpy(&alpha_end, s->d_alpha + (size_t)g.p.iter * g.n_mb - 1, sizeof(float),
-                        hipMemcpyDeviceToHost));
+  alpha_end = g.alpha0;  // no epoch (iter 0): nothing trained
+  if (g.p.iter > 0)
+    DG_HIP(ctx, hipMemcpy(&alpha_end, s->d_alpha + (size_t)g.p.iter * g.n_mb - 1, sizeof(float),
+                          hipMemcpyDeviceToHost));
   s->epoch = g.epoch0 + g.p.iter;
   s->sentences_done = g.sentences0 + (uint64_t)g.p.iter * g.n_walks;
   dg_sgns_written(ctx);

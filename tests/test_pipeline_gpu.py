@@ -134,3 +134,20 @@ def test_plain_form_is_rejected(small_graph):
         assert e.value.code == -8
     finally:
         c.close()
+
+
+def test_zero_epochs_walk_only(small_graph):
+    """iter 0: the step walks the round and trains nothing."""
+    c = _open(small_graph, 2000)
+    try:
+        sp = _sgns(c, ffi.DG_SGNS_BATCHED_BF16, 0, iter=0)
+        c.sgns_init(sp)
+        c.walk_reset()
+        before = c.sgns_tables(128)
+        rs, ss = c.pipeline_step(c.default_walk_params(seed=7), 1, 0, 2000, sp)
+        after = c.sgns_tables(128)
+        assert rs.walkers == 2000 and rs.tokens > 2000
+        assert ss.words == 0 and ss.kept == 0 and ss.launches == 0
+        assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    finally:
+        c.close()
