@@ -343,6 +343,9 @@ struct DrawArgs {
 __global__ void sent_draw_kernel(DrawArgs a) {
   const uint64_t base = a.sen_off[a.s_begin];
   const uint64_t n = a.sen_off[a.s_end] - base;
+  const int32_t* __restrict__ table = a.table;
+  int32_t* __restrict__ negs = a.negs;
+  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     // sentence of kept position base + i
@@ -353,18 +356,32 @@ __global__ void sent_draw_kernel(DrawArgs a) {
     }
     const uint32_t q = (uint32_t)(base + i - a.sen_off[lo]);
     const uint32_t c1 = (uint32_t)(a.first_sentence + lo), c2 = q | (a.epoch << 20);
-    a.shrink[i] = (uint8_t)(stream_word(a.seed, c1, c2, DG_TAG_SGNS, 0) % a.window);
-    for (int k = 0; k < a.kd; ++k) {
-      const uint32_t w = stream_word(a.seed, c1, c2, DG_TAG_SGNS, (uint32_t)(k + 1));
-      int v = a.table[w % a.table_size];
-      if (v == 0) v = (int)(w % (a.vocab - 1) + 1);
-      a.negs[i * a.kd + k] = v;
+    // word j of the stream is word j % 4 of Philox block j / 4 (stream_word);
+    // each block is evaluated once, and every table read is issued before
+    // any result is stored
+    uint32_t wd[kOutCols];
+    dg_u32x4 blk = dg_philox4x32_10(0, c1, c2, DG_TAG_SGNS, k0, k1);
+    a.shrink[i] = (uint8_t)(blk.x0 % a.window);
+    int v[kOutCols];
+#pragma unroll
+    for (int k = 0; k < kOutCols - 1; ++k) {
+      if (k >= a.kd) break;
+      const uint32_t j = (uint32_t)(k + 1), r = j & 3u;
+      if (r == 0) blk = dg_philox4x32_10(j >> 2, c1, c2, DG_TAG_SGNS, k0, k1);
+      wd[k] = r == 0 ? blk.x0 : (r == 1 ? blk.x1 : (r == 2 ? blk.x2 : blk.x3));
+      v[k] = table[wd[k] % a.table_size];
+    }
+#pragma unroll
+    for (int k = 0; k < kOutCols - 1; ++k) {
+      if (k >= a.kd) break;
+      negs[i * a.kd + k] = v[k] == 0 ? (int)(wd[k] % (a.vocab - 1) + 1) : v[k];
     }
   }
 }
 
 // Distinct vertices per sentence (255 when the sentence is longer than 64).
 __global__ void sent_count_kernel(DrawArgs a) {
+  unsigned long long acc = 0;  // pool rows of this thread's sentences
   for (uint64_t s = a.s_begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.s_end;
        s += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t sb = a.sen_off[s];
@@ -392,9 +409,12 @@ __global__ void sent_count_kernel(DrawArgs a) {
           d += !seen;
         }
       }
-      atomicAdd(a.pool_rows, (unsigned long long)d);
+      acc += (unsigned long long)d;
     }
   }
+  // one atomic per wavefront on the shared counter, not one per sentence
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(a.pool_rows, acc);
 }
 
 // A wave-uniform 64-bit value moved to scalar registers.
