@@ -387,7 +387,20 @@ __global__ void sent_count_kernel(DrawArgs a) {
     const uint64_t sb = a.sen_off[s];
     const int L = (int)(a.sen_off[s + 1] - sb);
     int u = 255;
-    if (L <= 64) {
+    if (L <= 32) {
+      // the common case: the tokens once into registers, the pairs compared there
+      int tk[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) tk[i] = i < L ? a.sen[sb + i] : -1 - i;
+      u = 0;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        bool seen = false;
+#pragma unroll
+        for (int j = 0; j < i; ++j) seen |= tk[j] == tk[i];
+        u += (i < L && !seen) ? 1 : 0;
+      }
+    } else if (L <= 64) {
       u = 0;
       for (int i = 0; i < L; ++i) {
         const int v = a.sen[sb + i];
