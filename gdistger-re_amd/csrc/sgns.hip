@@ -240,10 +240,15 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
 // written per element) for each 2 B it saves on the gather.
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// Two floats rounded to BF16 (nearest, ties to even) and packed, a in the low
+// half: one v_cvt_pk_bf16_f32.  For every finite float it gives the bits of
+// (u + 0x7fff + ((u >> 16) & 1)) >> 16 on the float's bits u.
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+  const f32x2_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -291,18 +296,12 @@ struct Chunk<uint16_t> {
   }
   __device__ static v_t from(float a, float b, float c, float d) {
     U u;
-    u.h[0] = f2bf(a);
-    u.h[1] = f2bf(b);
-    u.h[2] = f2bf(c);
-    u.h[3] = f2bf(d);
+    u.q = make_uint2(f2bf2(a, b), f2bf2(c, d));
     return u.v;
   }
   __device__ static float elem(const uint16_t* p) { return bf2f(*p); }
   __device__ static void put4(uint16_t* dst, float a, float b, float c, float d) {
-    uint2 h;
-    h.x = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
-    h.y = (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16);
-    *reinterpret_cast<uint2*>(dst) = h;
+    *reinterpret_cast<uint2*>(dst) = make_uint2(f2bf2(a, b), f2bf2(c, d));
   }
   __device__ static v_t load4(const uint16_t* src) {
     U u;
@@ -873,16 +872,22 @@ void sg_sent_kernel(SentArgs t) {
           const float4 mq = *reinterpret_cast<const float4*>(L.Mm + l16 * GPL + 4 * g4);
           am = C::from(mq.x, mq.y, mq.z, mq.w);
         }
+        // Gp^T operands, the same for every dim tile
+        V agp[NPT];
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+          agp[p] = vzero;
+          if (krow) {
+            const float* gc = L.Gp + (p * 16 + 4 * g4) * GPL + l16;
+            agp[p] = C::from(gc[0], gc[GPL], gc[2 * GPL], gc[3 * GPL]);
+          }
+        }
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
           f32x4_t dy = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int p = 0; p < NPT; ++p) {
-            V ag = vzero;
-            if (krow) {
-              const float* gc = L.Gp + (p * 16 + 4 * g4) * GPL + l16;
-              ag = C::from(gc[0], gc[GPL], gc[2 * GPL], gc[3 * GPL]);
-            }
+            const V ag = agp[p];
             const E* xcol = L.xc + (p * 16 + 4 * g4) * LD + 16 * tt + l16;
             const V bx = C::from(C::elem(xcol), C::elem(xcol + LD), C::elem(xcol + 2 * LD), C::elem(xcol + 3 * LD));
             dy = C::mma(ag, bx, dy);
