@@ -275,7 +275,11 @@ struct Chunk<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
   }
   __device__ static v_t from(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
-  __device__ static float elem(const float* p) { return *p; }
+  // column l16 of the four rows from blk (row stride LDN): k value e from row e
+  template <int LDN>
+  __device__ static v_t col4(const float* blk, int l16) {
+    return make_float4(blk[l16], blk[l16 + LDN], blk[l16 + 2 * LDN], blk[l16 + 3 * LDN]);
+  }
   __device__ static void put4(float* dst, float a, float b, float c, float d) {
     *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
   }
@@ -299,7 +303,17 @@ struct Chunk<uint16_t> {
     u.q = make_uint2(f2bf2(a, b), f2bf2(c, d));
     return u.v;
   }
-  __device__ static float elem(const uint16_t* p) { return bf2f(*p); }
+  // column l16 of the four rows from blk (row stride LDN, 8-byte aligned) by
+  // one transposing LDS read: in each group of 16 lanes, lane j supplies the
+  // 8 bytes of row j / 4, columns 4 (j % 4) .. 4 (j % 4) + 3, and lane i
+  // receives column i of the four rows
+  template <int LDN>
+  __device__ static v_t col4(const uint16_t* blk, int l16) {
+    typedef short s16x4_t __attribute__((ext_vector_type(4)));
+    const uint16_t* src = blk + (l16 >> 2) * LDN + 4 * (l16 & 3);
+    const s16x4_t r = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4_t*)src);
+    return __builtin_bit_cast(v_t, r);
+  }
   __device__ static void put4(uint16_t* dst, float a, float b, float c, float d) {
     *reinterpret_cast<uint2*>(dst) = make_uint2(f2bf2(a, b), f2bf2(c, d));
   }
@@ -839,12 +853,9 @@ void sg_sent_kernel(SentArgs t) {
           }
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) {
-            V ay = vzero;
-            if (kchk) {
-              const E* yc = L.yb + (4 * g4) * LD + 16 * tt + l16;
-              ay = C::from(C::elem(yc), C::elem(yc + LD), C::elem(yc + 2 * LD), C::elem(yc + 3 * LD));
-            }
-            dx[p][tt] = C::mma(ay, bgp, dx[p][tt]);
+            // groups past the KC output rows read rows 0..3 and take zeros
+            const V yk = C::template col4<LD>(L.yb + (kchk ? 4 * g4 : 0) * LD + 16 * tt, l16);
+            dx[p][tt] = C::mma(kchk ? yk : vzero, bgp, dx[p][tt]);
           }
 #pragma unroll
           for (int p2 = 0; p2 < NPT; ++p2) {
@@ -852,8 +863,7 @@ void sg_sent_kernel(SentArgs t) {
             const V bh = C::from(hq.x, hq.y, hq.z, hq.w);
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt) {
-              const E* xq = L.xc + (p2 * 16 + 4 * g4) * LD + 16 * tt + l16;
-              const V ax = C::from(C::elem(xq), C::elem(xq + LD), C::elem(xq + 2 * LD), C::elem(xq + 3 * LD));
+              const V ax = C::template col4<LD>(L.xc + (p2 * 16 + 4 * g4) * LD + 16 * tt, l16);
               dx[p][tt] = C::mma(ax, bh, dx[p][tt]);
             }
           }
@@ -887,18 +897,12 @@ void sg_sent_kernel(SentArgs t) {
           f32x4_t dy = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int p = 0; p < NPT; ++p) {
-            const V ag = agp[p];
-            const E* xcol = L.xc + (p * 16 + 4 * g4) * LD + 16 * tt + l16;
-            const V bx = C::from(C::elem(xcol), C::elem(xcol + LD), C::elem(xcol + 2 * LD), C::elem(xcol + 3 * LD));
-            dy = C::mma(ag, bx, dy);
+            const V bx = C::template col4<LD>(L.xc + (p * 16 + 4 * g4) * LD + 16 * tt, l16);
+            dy = C::mma(agp[p], bx, dy);
           }
           {
-            V by = vzero;
-            if (kchk) {
-              const E* yq = L.yb + (4 * g4) * LD + 16 * tt + l16;
-              by = C::from(C::elem(yq), C::elem(yq + LD), C::elem(yq + 2 * LD), C::elem(yq + 3 * LD));
-            }
-            dy = C::mma(am, by, dy);
+            const V yk = C::template col4<LD>(L.yb + (kchk ? 4 * g4 : 0) * LD + 16 * tt, l16);
+            dy = C::mma(am, kchk ? yk : vzero, dy);
           }
 #pragma unroll
           for (int e = 0; e < 4; ++e)
