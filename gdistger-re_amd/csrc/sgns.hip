@@ -638,7 +638,13 @@ void sg_sent_kernel(SentArgs t) {
         const int row0 = target;
         const int b = __builtin_amdgcn_readfirstlane(b_nx);
         if (q + 1 < q1) {
-          neg_nx = lane >= 1 && lane <= K ? ngs[(uint64_t)(q + 1) * K + lane - 1] : -1;
+          // the row of draws as a wave-uniform address each position: a
+          // per-lane pointer stepped across positions would be held (and
+          // spilled) for the whole loop
+          typedef const int32_t __attribute__((address_space(1))) * gptr_t;
+          const gptr_t nq = reinterpret_cast<gptr_t>(
+              uniform_u64(reinterpret_cast<uint64_t>(ngs + (uint64_t)(q + 1) * K)));
+          neg_nx = lane >= 1 && lane <= K ? nq[lane - 1] : -1;
           b_nx = shr[q + 1];
         }
         // the output rows' loads go out first (rows past K stay zero); the
