@@ -76,12 +76,12 @@ def sgns_sent_bytes(kept, pool_rows, dim, negative):
     return rows * dim * 12 + kept * (5 + 4 * negative)
 
 
-def measured_profile(shape):
+def measured_profile(shape, dim):
     """From the committed rocprofv3 summary of the BF16 sentence kernel: HBM
     bytes per kept position (FETCH_SIZE and WRITE_SIZE over the timed
     launches, calibrated against a copy kernel) and the MFMA utilisation, for
-    the launch shape the summary was taken at; None where the summary is
-    absent or was taken at another shape."""
+    the launch shape and dim the summary was taken at; None where the summary
+    is absent or was taken at another shape or dim."""
     path = os.path.join(REPO, "profiles", "sg_sent_kernel_bf16_c2_hbm_counters.txt")
     vals = {}
     try:
@@ -93,7 +93,7 @@ def measured_profile(shape):
     except OSError:
         return None
     want = f"{shape.get('minibatch')},{shape.get('block_waves')},{shape.get('pool_split')}"
-    if vals.get("shape") != want:
+    if vals.get("shape") != want or vals.get("dim") != str(dim):
         return None
     try:
         return {"traffic_per_kept": float(vals["traffic_bytes_per_kept_position"]),
@@ -465,7 +465,7 @@ def main():
 
     shape_used = {"minibatch": bf.minibatch or 131072, "block_waves": bf.block_waves or 2,
                   "pool_split": bf.pool_split or 16}
-    prof = measured_profile(shape_used)
+    prof = measured_profile(shape_used, args.dim)
     line = {
         "metric": "walk steps/s + SGNS words/s at equal AUC; BF16-MFMA vs FP32 plain, 1 GPU",
         "value": value,

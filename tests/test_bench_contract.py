@@ -30,14 +30,16 @@ def test_more_than_one_gpu_is_refused():
 
 def test_profile_parses_at_the_tuned_shape():
     bench = load_bench()
-    prof = bench.measured_profile({"minibatch": 524288, "block_waves": 4, "pool_split": 16})
+    tuned = {"minibatch": 524288, "block_waves": 4, "pool_split": 16}
+    prof = bench.measured_profile(tuned, 128)
     assert prof is not None
     # HBM bytes per kept position: below the algorithmic ~10 KB (hub rows stay
     # in L2), far above the ~2 KB of the output rows' reads alone
     assert 2000 < prof["traffic_per_kept"] < 20000
     assert 0 < prof["mfma_util_pct"] < 100
-    # another launch shape has no measured traffic: the bench reports null
-    assert bench.measured_profile({"minibatch": 131072, "block_waves": 2, "pool_split": 16}) is None
+    # another launch shape or dim has no measured traffic: the bench reports null
+    assert bench.measured_profile({"minibatch": 131072, "block_waves": 2, "pool_split": 16}, 128) is None
+    assert bench.measured_profile(tuned, 256) is None
 
 
 def test_profile_commands_run_committed_files():
