@@ -80,26 +80,34 @@ def measured_profile(shape, dim):
     """From the committed rocprofv3 summary of the BF16 sentence kernel: HBM
     bytes per kept position (FETCH_SIZE and WRITE_SIZE over the timed
     launches, calibrated against a copy kernel) and the MFMA utilisation, for
-    the launch shape and dim the summary was taken at; None where the summary
-    is absent or was taken at another shape or dim."""
+    a launch shape and dim the summary was taken at; None where the summary
+    is absent or holds no measurement at this shape and dim.  The summary
+    ends in one block per measured shape, each opened by its `shape:` line."""
     path = os.path.join(REPO, "profiles", "sg_sent_kernel_bf16_c2_hbm_counters.txt")
-    vals = {}
+    keys = ("shape", "dim", "traffic_bytes_per_kept_position", "mfma_util_pct")
+    blocks = []
     try:
         with open(path) as f:
             for line in f:
-                if ":" in line and not line.startswith(" "):
-                    k, v = line.split(":", 1)
-                    vals[k.strip()] = v.strip()
+                k, sep, v = line.partition(":")
+                if not sep or k not in keys:
+                    continue
+                if k == "shape":
+                    blocks.append({})
+                if blocks:
+                    blocks[-1][k] = v.strip()
     except OSError:
         return None
     want = f"{shape.get('minibatch')},{shape.get('block_waves')},{shape.get('pool_split')}"
-    if vals.get("shape") != want or vals.get("dim") != str(dim):
-        return None
-    try:
-        return {"traffic_per_kept": float(vals["traffic_bytes_per_kept_position"]),
-                "mfma_util_pct": float(vals.get("mfma_util_pct", "nan"))}
-    except (KeyError, ValueError):
-        return None
+    for b in blocks:
+        if b.get("shape") != want or b.get("dim") != str(dim):
+            continue
+        try:
+            return {"traffic_per_kept": float(b["traffic_bytes_per_kept_position"]),
+                    "mfma_util_pct": float(b.get("mfma_util_pct", "nan"))}
+        except (KeyError, ValueError):
+            return None
+    return None
 
 
 def auc_of(ctx, pos, neg):

@@ -37,6 +37,9 @@ def test_profile_parses_at_the_tuned_shape():
     # in L2), far above the ~2 KB of the output rows' reads alone
     assert 2000 < prof["traffic_per_kept"] < 20000
     assert 0 < prof["mfma_util_pct"] < 100
+    # the autotuner's other pick on this corpus (two wavefronts per workgroup)
+    other = bench.measured_profile({"minibatch": 524288, "block_waves": 2, "pool_split": 16}, 128)
+    assert other is not None and abs(other["traffic_per_kept"] - prof["traffic_per_kept"]) < 0.05 * prof["traffic_per_kept"]
     # another launch shape or dim has no measured traffic: the bench reports null
     assert bench.measured_profile({"minibatch": 131072, "block_waves": 2, "pool_split": 16}, 128) is None
     assert bench.measured_profile(tuned, 256) is None
