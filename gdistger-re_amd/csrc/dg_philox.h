@@ -109,6 +109,13 @@ DG_RNG_HD uint32_t dg_stream_next(dg_stream* s) {
   return r;
 }
 
+/* Key of a stream that belongs to one training epoch: the epoch number goes
+ * into the high key word, so the counters of different epochs never need to
+ * share bits (each epoch is its own family of streams under the user seed). */
+DG_RNG_HD uint64_t dg_epoch_key(uint64_t seed, uint32_t epoch) {
+  return seed ^ ((uint64_t)epoch << 32);
+}
+
 /* Uniform integer in [0, n), n >= 1 (Lemire 2019, with rejection). */
 DG_RNG_HD uint32_t dg_stream_uniform(dg_stream* s, uint32_t n) {
   uint32_t x = dg_stream_next(s);

@@ -95,18 +95,21 @@ __device__ __forceinline__ float sigmoid_of(float f, const float* expt, int exac
 }
 
 // ---- subsampling (word2vec.cu:1011-1020 with a Philox draw per token) ------
+// The draw of corpus token i in epoch e is word 0 of Philox(counter (0, i lo,
+// i hi, SAMPLE), key dg_epoch_key(seed, e)).
 __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* offsets,
                                        uint64_t n_walks, const uint32_t* rank,
                                        const uint32_t* keep, uint64_t seed, uint32_t epoch,
                                        uint32_t* kept) {
+  const uint64_t key = dg_epoch_key(seed, epoch);
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = 0;
     for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
       const uint32_t r = rank[tokens[i]];
       if (r == kNoRank) continue;
-      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32) ^ (epoch << 24),
-                                          DG_TAG_SAMPLE, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
+                                          (uint32_t)key, (uint32_t)(key >> 32));
       c += ((o.x0 & 0xFFFFu) <= keep[r]);
     }
     kept[w] = c;
@@ -117,14 +120,15 @@ __global__ void subsample_write_kernel(const uint32_t* tokens, const uint64_t* o
                                        uint64_t n_walks, const uint32_t* rank,
                                        const uint32_t* keep, uint64_t seed, uint32_t epoch,
                                        const uint64_t* sen_off, int32_t* sen) {
+  const uint64_t key = dg_epoch_key(seed, epoch);
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t o_pos = sen_off[w];
     for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
       const uint32_t r = rank[tokens[i]];
       if (r == kNoRank) continue;
-      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32) ^ (epoch << 24),
-                                          DG_TAG_SAMPLE, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
+                                          (uint32_t)key, (uint32_t)(key >> 32));
       if ((o.x0 & 0xFFFFu) <= keep[r]) sen[o_pos++] = (int32_t)r;
     }
   }
@@ -335,8 +339,7 @@ __device__ __forceinline__ uint32_t stream_word(uint64_t seed, uint32_t c1, uint
 struct DrawArgs {
   const int32_t* sen;
   const uint64_t* sen_off;  // of the epoch's sentences
-  uint64_t s_begin, s_end;  // sentences of the launch
-  uint64_t first_sentence;  // Philox counter of sentence 0 of the epoch
+  uint64_t s_begin, s_end;  // sentences of the launch (indices in the epoch's corpus)
   uint64_t seed;
   uint32_t epoch;
   const int32_t* table;
@@ -350,15 +353,17 @@ struct DrawArgs {
   unsigned long long* pool_rows;  // sum of the distinct vertices (sentences of L >= 2)
 };
 
-// Draws of every position: the stream of position q of sentence s (counter
-// (blk, first_sentence + s, q | epoch << 20, SGNS)): word 0 gives b = word %
-// window, word k negative k: table[word % table_size], 0 -> word % (vocab-1) + 1.
+// Draws of every position: the stream of position q of sentence s of the
+// epoch's corpus (counter (blk, s, q, SGNS), key dg_epoch_key(seed, epoch)):
+// word 0 gives b = word % window, word k negative k: table[word % table_size],
+// 0 -> word % (vocab-1) + 1.
 __global__ void sent_draw_kernel(DrawArgs a) {
   const uint64_t base = a.sen_off[a.s_begin];
   const uint64_t n = a.sen_off[a.s_end] - base;
   const int32_t* __restrict__ table = a.table;
   int32_t* __restrict__ negs = a.negs;
-  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  const uint64_t key = dg_epoch_key(a.seed, a.epoch);
+  const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     // sentence of kept position base + i
@@ -368,7 +373,7 @@ __global__ void sent_draw_kernel(DrawArgs a) {
       if (a.sen_off[mid] <= base + i) lo = mid; else hi = mid;
     }
     const uint32_t q = (uint32_t)(base + i - a.sen_off[lo]);
-    const uint32_t c1 = (uint32_t)(a.first_sentence + lo), c2 = q | (a.epoch << 20);
+    const uint32_t c1 = (uint32_t)lo, c2 = q;
     // word j of the stream is word j % 4 of Philox block j / 4 (stream_word);
     // each block is evaluated once, and every table read is issued before
     // any result is stored
@@ -997,10 +1002,9 @@ struct dg_sgns_state {
     dg_sgns_params p;
     float alpha0;
     uint32_t kd, epoch0, n_mb;
-    uint64_t n_walks, mb, pos_bound, sentences0;
+    uint64_t n_walks, mb, pos_bound;
   } g{};
-  uint32_t epoch = 0;
-  uint64_t sentences_done = 0;
+  uint32_t epoch = 0;       // epochs trained so far: part of every draw's Philox key
   uint64_t version = 0;     // changes on every write of syn0 / syn1
   bool has_vocab = true;    // false for dg_sgns_tables_create tables
   uint32_t rank_size = 0;   // entries of d_rank (vertex ids accepted by score_pairs)
@@ -1364,14 +1368,13 @@ static void launch_sent_form(dg_ctx* ctx, const SentArgs& t, uint32_t dim) {
 // The draw kernel loops over the launch's positions as the device offsets
 // give them; pos_grid (an upper bound, 0 = none) only sizes its grid.
 static int enqueue_batch(dg_ctx* ctx, dg_sgns_state* s, const dg_sgns_params& p, uint32_t kd,
-                         uint32_t epoch, uint64_t first_sentence, uint64_t s0, uint64_t s1,
+                         uint32_t epoch, uint64_t s0, uint64_t s1,
                          uint64_t pos_grid, float alpha, const float* alpha_dev, bool timed) {
   DrawArgs da;
   da.sen = s->d_sen;
   da.sen_off = s->d_sen_off;
   da.s_begin = s0;
   da.s_end = s1;
-  da.first_sentence = first_sentence;
   da.seed = p.seed;
   da.epoch = epoch;
   da.table = s->d_table;
@@ -1533,12 +1536,11 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
         DG_CHECK(dg_reserve(ctx, &s->d_negs, &s->negs_cap, (size_t)n_pos * kd + 1));
         DG_CHECK(dg_reserve(ctx, &s->d_ucount, &s->ucount_cap, (size_t)(s1 - s0) + 1));
         alpha = alpha_at(s0);
-        DG_CHECK(enqueue_batch(ctx, s, p, kd, epoch, s->sentences_done, s0, s1, n_pos, alpha,
+        DG_CHECK(enqueue_batch(ctx, s, p, kd, epoch, s0, s1, n_pos, alpha,
                                nullptr, true));
         st.launches++;
       }
     }
-    s->sentences_done += n_walks;
   }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   dg_sgns_written(ctx);
@@ -1590,7 +1592,6 @@ int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, u
   const uint64_t per_launch = (n_walks < g.mb ? n_walks : g.mb) * slot_tokens + overflow_tokens;
   g.pos_bound = per_launch < tok_bound ? per_launch : tok_bound;
   g.epoch0 = s->epoch;
-  g.sentences0 = s->sentences_done;
   DG_CHECK(dg_reserve(ctx, &s->d_cnt, &s->cnt_cap, (size_t)n_walks + 1));
   DG_CHECK(dg_reserve(ctx, &s->d_sen_off, &s->sen_off_cap, (size_t)n_walks + 1));
   DG_CHECK(dg_reserve(ctx, &s->d_sen, &s->sen_cap, (size_t)tok_bound + 1));
@@ -1626,7 +1627,7 @@ int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_
     uint32_t i = 0;
     for (uint64_t s0 = 0; s0 < n_walks; s0 += g.mb, ++i) {
       const uint64_t s1 = n_walks - s0 < g.mb ? n_walks : s0 + g.mb;
-      DG_CHECK(enqueue_batch(ctx, s, p, g.kd, epoch, g.sentences0 + (uint64_t)ep * n_walks, s0, s1,
+      DG_CHECK(enqueue_batch(ctx, s, p, g.kd, epoch, s0, s1,
                              g.pos_bound, g.alpha0, alphas + i, timed));
     }
   }
@@ -1647,7 +1648,6 @@ int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* ou
     DG_HIP(ctx, hipMemcpy(&alpha_end, s->d_alpha + (size_t)g.p.iter * g.n_mb - 1, sizeof(float),
                           hipMemcpyDeviceToHost));
   s->epoch = g.epoch0 + g.p.iter;
-  s->sentences_done = g.sentences0 + (uint64_t)g.p.iter * g.n_walks;
   dg_sgns_written(ctx);
   if (out) {
     memset(out, 0, sizeof(*out));
@@ -1666,7 +1666,7 @@ int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* ou
 // Coordinate search on the caller's corpus: wavefronts per workgroup x pool
 // split at the default launch size, then sentences per launch with the best
 // shape.  Each candidate trains the sample twice (Hogwild) and keeps its faster
-// device time; the tables and the epoch/sentence counters are copied before
+// device time; the tables and the epoch counter are copied before
 // and restored after, so tuning leaves no trace in the training state.
 
 extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* pin,
@@ -1704,7 +1704,6 @@ extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* p
     return dg_fail(ctx, DG_ERR_NOMEM, "autotune: no room to save the tables");
   }
   const uint32_t epoch0 = s->epoch;
-  const uint64_t sent0 = s->sentences_done;
   int rc = DG_OK;
   auto restore = [&]() {
     hipMemcpyAsync(s->d_syn0, save0, bytes, hipMemcpyDeviceToDevice, ctx->stream);
@@ -1713,7 +1712,6 @@ extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* p
     hipFree(save0);
     hipFree(save1);
     s->epoch = epoch0;
-    s->sentences_done = sent0;
     dg_sgns_written(ctx);
   };
   if (hipMemcpyAsync(save0, s->d_syn0, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||

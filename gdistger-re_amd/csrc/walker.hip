@@ -38,7 +38,7 @@ constexpr int kPathLds = 23;    // vertices per lane kept in LDS (odd); referenc
 constexpr int kTermTable = 32;  // entropy terms tabulated for walks shorter than 32
 constexpr int kTermStride = kTermTable + 1;
 constexpr uint32_t kPool = 64;
-constexpr uint32_t kDefaultLengthLimit = 1024;
+constexpr uint32_t kDefaultLengthLimit = DG_WALK_LENGTH_LIMIT_DEFAULT;
 
 struct WalkArgs {
   const uint64_t* row_ptr;

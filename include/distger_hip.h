@@ -101,6 +101,12 @@ int dg_graph_get_thresholds(dg_ctx* ctx, uint32_t* thr);
 
 /* ---- walker (W4-W9) ---------------------------------------------------- */
 
+/* Safety limit on vertices per walk when dg_walk_params.length_limit is 0.
+ * The reference has no limit (walk.hpp:979-985); a walk that reaches it ends
+ * the round with DG_ERR_WALK_TOO_LONG.  The walker keeps (limit - 23)
+ * vertices of spill space per resident lane. */
+#define DG_WALK_LENGTH_LIMIT_DEFAULT 1024
+
 #define DG_ROUNDS_REFERENCE 0 /* stop after max_rounds rounds (walk.hpp:740-744) */
 #define DG_ROUNDS_CONVERGE 1  /* also stop once |delta RE| <= delta (paper; walk.hpp:745-749) */
 
@@ -114,7 +120,7 @@ typedef struct {
   double delta;              /* RE convergence threshold, 0.001 (walk.hpp:26) */
   uint32_t max_rounds;       /* 30 (walk.hpp:740) */
   uint32_t rounds_mode;      /* DG_ROUNDS_* */
-  uint32_t length_limit;     /* safety limit on vertices per walk; 0 = 4096 */
+  uint32_t length_limit;     /* safety limit on vertices per walk; 0 = DG_WALK_LENGTH_LIMIT_DEFAULT */
   uint32_t reserved;
 } dg_walk_params;
 
@@ -271,7 +277,8 @@ int dg_emb_write(dg_ctx* ctx, const char* path, int binary);
  * first sample_walks sentences of a device corpus (0 = 524288) for candidate
  * shapes (block_waves x pool_split, then minibatch) and writes the fastest
  * into *tuned (a copy of *p with those three fields set).  The tables and the
- * draw counters are saved and restored: tuning changes no training state.
+ * epoch counter (part of every draw's Philox key) are saved and restored:
+ * tuning changes no training state.
  * words_per_s[0] is the default shape's rate, [1] the chosen one's (may be
  * NULL).  No reference counterpart: the reference's launch size is the
  * compile-time MAX_SENTENCE (src/word2vec.cu:35,1002-1040). */

@@ -48,7 +48,9 @@ typedef struct {
 } dgo_walk_cfg;
 
 /* Graph build (W1/W2).  Returns 0, or -1 on an out-of-range vertex id.
- * row_ptr has v_num+1 entries; nbr/data have e_num*(undirected?2:1) entries. */
+ * row_ptr has v_num+1 entries; nbr/data have e_num*(undirected?2:1) entries
+ * (either may be NULL: the edge data alone, in CSR order, is then the
+ * per-entry weight array of a CSR built elsewhere in the same order). */
 int dgo_build_csr(uint32_t v_num, const dgo_edge* edges, uint64_t e_num, int undirected,
                   uint64_t* row_ptr, uint32_t* nbr, float* data);
 

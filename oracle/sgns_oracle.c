@@ -120,18 +120,20 @@ void dgo_keep_limits(const uint64_t* cn_of_rank, uint32_t vocab, float sample, u
 /* Subsampled sentences: each walk of the corpus becomes one sentence (tokens
  * mapped to vocabulary ranks; vertices outside the vocabulary skipped, as
  * SearchVocab == -1 at src/word2vec.cu:1012).  The draw for corpus token i
- * in epoch e is word 0 of Philox(ctr = (0, i lo, i hi ^ e<<24, SAMPLE)). */
+ * in epoch e is word 0 of Philox(ctr = (0, i lo, i hi, SAMPLE)) under the key
+ * dg_epoch_key(seed, e). */
 uint64_t dgo_subsample(const uint32_t* tokens, const uint64_t* offsets, uint64_t n_walks,
                        const uint32_t* rank_of_vertex, const uint32_t* keep, uint64_t seed,
                        uint32_t epoch, int32_t* sen, uint64_t* sen_off) {
   uint64_t n = 0;
+  const uint64_t key = dg_epoch_key(seed, epoch);
   sen_off[0] = 0;
   for (uint64_t w = 0; w < n_walks; ++w) {
     for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
       uint32_t r = rank_of_vertex[tokens[i]];
       if (r == 0xffffffffu) continue;
-      dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32) ^ (epoch << 24),
-                                    DG_TAG_SAMPLE, (uint32_t)seed, (uint32_t)(seed >> 32));
+      dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
+                                    (uint32_t)key, (uint32_t)(key >> 32));
       if ((o.x0 & 0xFFFFu) > keep[r]) continue;
       sen[n++] = (int32_t)r;
     }
@@ -251,9 +253,10 @@ uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* ba
  * 1 when K = negative).  Tables are held in double for the whole call and
  * rounded to float at the end: the FP64 reference the device forms follow.
  *
- * Draws: Philox stream (ctr = (blk, first_sen + s, q | epoch << 20, SGNS))
- * of position q of sentence s: word 0 gives b = word % window, word k
- * (1..K) negative k: table[word % table_size], 0 -> word % (vocab-1) + 1.
+ * Draws: Philox stream (ctr = (blk, first_sen + s, q, SGNS), key
+ * dg_epoch_key(seed, epoch)) of position q of sentence s, first_sen + s being
+ * the sentence's index in the epoch's corpus: word 0 gives b = word % window,
+ * word k (1..K) negative k: table[word % table_size], 0 -> word % (vocab-1) + 1.
  * ---------------------------------------------------------------------- */
 static double sig_d(double f, const float* expt, int exact) {
   if (exact) return 1.0 / (1.0 + exp(-f));
@@ -266,7 +269,7 @@ void dgo_position_draws(uint64_t seed, uint64_t gs, uint32_t q, uint32_t epoch, 
                         uint32_t draws, const int32_t* table, uint32_t table_size, uint32_t vocab,
                         uint32_t* b, int32_t* negs) {
   dg_stream st;
-  dg_stream_init(&st, seed, (uint32_t)gs, q | (epoch << 20), DG_TAG_SGNS);
+  dg_stream_init(&st, dg_epoch_key(seed, epoch), (uint32_t)gs, q, DG_TAG_SGNS);
   *b = dg_stream_next(&st) % window;
   for (uint32_t k = 0; k < draws; ++k) {
     uint32_t x = dg_stream_next(&st);

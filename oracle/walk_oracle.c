@@ -81,12 +81,12 @@ int dgo_build_csr(uint32_t v_num, const dgo_edge* edges, uint64_t e_num, int und
   memcpy(fill, row_ptr, sizeof(uint64_t) * (size_t)v_num);
   for (uint64_t i = 0; i < e_num; ++i) {
     uint64_t p = fill[edges[i].src]++;
-    nbr[p] = edges[i].dst;
-    data[p] = edges[i].data;
+    if (nbr) nbr[p] = edges[i].dst;
+    if (data) data[p] = edges[i].data;
     if (undirected) {
       uint64_t q = fill[edges[i].dst]++;
-      nbr[q] = edges[i].src;
-      data[q] = edges[i].data;
+      if (nbr) nbr[q] = edges[i].src;
+      if (data) data[q] = edges[i].data;
     }
   }
   (void)mult;

@@ -106,6 +106,24 @@ class OracleGraph:
         self._g = _Graph(v_num, m, self.row_ptr.ctypes.data, self.nbr.ctypes.data,
                          self.cn.ctypes.data, self.w.ctypes.data if weighted else None)
 
+    @classmethod
+    def from_csr(cls, row_ptr, nbr, cn, w=None, variant="dg"):
+        """Oracle view of a CSR built elsewhere (the device's, as dg_graph_get_csr
+        returns it): the same neighbour order and common-neighbour counts;
+        `w` the per-entry weights for the weighted acceptance."""
+        g = cls.__new__(cls)
+        g.L = load(variant)
+        g.v_num = len(row_ptr) - 1
+        g.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        g.nbr = np.ascontiguousarray(nbr, dtype=np.uint32)
+        g.cn = np.ascontiguousarray(cn, dtype=np.uint32)
+        g.m = len(g.nbr)
+        g.w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+        g.data = g.w
+        g._g = _Graph(g.v_num, g.m, g.row_ptr.ctypes.data, g.nbr.ctypes.data, g.cn.ctypes.data,
+                      g.w.ctypes.data if g.w is not None else None)
+        return g
+
     @property
     def degrees(self):
         return np.diff(self.row_ptr).astype(np.uint32)
@@ -165,6 +183,20 @@ def walk_run(g, seed, walkers_per_round, init_round=5, min_length=20, max_length
         if stop:
             break
     return out
+
+
+def csr_edge_data(v_num, edges, undirected=True):
+    """The edges' data fields in the reference CSR order (doubled list, stable
+    by source): the per-entry weights of the CSR built from `edges`."""
+    L = load()
+    edges = np.ascontiguousarray(edges, dtype=EDGE_DTYPE)
+    m = len(edges) * (2 if undirected else 1)
+    row_ptr = np.zeros(v_num + 1, np.uint64)
+    data = np.zeros(max(m, 1), np.float32)
+    if L.dgo_build_csr(v_num, _vp(edges), len(edges), int(undirected), _vp(row_ptr), None,
+                       _vp(data)) != 0:
+        raise ValueError("edge out of range")
+    return row_ptr, data[:m]
 
 
 def write_walks(path, tokens, offsets, threads=1, variant="dg"):

@@ -82,3 +82,17 @@ def test_struct_layouts_match_the_header(tmp_path):
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
     assert got[("dg_edge", "size")] == ffi.EDGE_DTYPE.itemsize
+
+
+def test_walk_length_limit_default():
+    """The header names the length limit a zero length_limit stands for, the
+    walker takes it from there, and the reference defaults leave the field 0."""
+    text = open(HEADER).read()
+    m = re.search(r"#define DG_WALK_LENGTH_LIMIT_DEFAULT (\d+)", text)
+    assert m and int(m.group(1)) == 1024
+    assert "0 = DG_WALK_LENGTH_LIMIT_DEFAULT" in text
+    src = open(os.path.join(REPO, "gdistger-re_amd", "csrc", "walker.hip")).read()
+    assert "kDefaultLengthLimit = DG_WALK_LENGTH_LIMIT_DEFAULT;" in src
+    p = ffi.WalkParams()
+    distger.load_library().dg_walk_params_default(ctypes.byref(p), 1000)
+    assert p.length_limit == 0 and p.max_length == 0 and p.min_length == 20

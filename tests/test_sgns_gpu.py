@@ -3,9 +3,11 @@
 * set-up (vocabulary, LCG rows): bit-exact;
 * plain FP32 form in its deterministic order (one wavefront, sentences in
   order): bit-exact against the oracle's restatement of skip_gram_kernel;
-* batched forms in deterministic mode (snapshot per launch, sorted scatter-add):
-  against the FP64 oracle of the same semantics, with the tolerance written in
-  each test; non-deterministic (atomic) modes for sanity and AUC.
+* batched forms in deterministic mode (one wavefront, sentences in order, the
+  sequential order to first order in the updates, per-position shared
+  negatives): against the FP64 restatement of that evaluation
+  (dgo_sg_shared_first_order), with the tolerance written in each test;
+  non-deterministic (Hogwild, atomic row updates) modes for sanity and AUC.
 """
 
 import os
@@ -79,7 +81,6 @@ def test_plain_deterministic_bit_exact(ctx, walked):
     tok, off, counts = walked
     p = _params(ctx, form=ffi.DG_SGNS_PLAIN_FP32, deterministic=1)
     ctx.sgns_init(p)
-    ctx.corpus_append() if False else None
     st = ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
     s0, s1 = ctx.sgns_tables(128)
     o = dgsgns_oracle.Trainer(counts, 128, table_size=TABLE)

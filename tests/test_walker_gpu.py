@@ -309,22 +309,25 @@ def test_weighted_thresholds_definition(ctx, rmat_weighted):
         assert t == 0 or not (pn <= (t - 1) / M)
 
 
-@pytest.mark.parametrize("scale", [
-    20,
-    pytest.param(25, marks=pytest.mark.skipif(
-        os.environ.get("DG_FULL_SCALE") != "1",
-        reason="R-MAT scale 25 (33.5M vertices, 1.05G entries): about 2 minutes and 70 GB of "
-               "host memory; set DG_FULL_SCALE=1")),
-])
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("scale", [20, 25])
 def test_weighted_rmat_sampled_walks(ctx, scale):
     """Weighted walks on R-MAT with edge factor 16, (0.57, 0.19, 0.19, 0.05),
-    weights U(0, 1], seed 4: a round of 5 V walks; 300 sampled walks equal the
-    oracle's (built on the device's common-neighbour counts), the visit counts
+    weights U(0, 1], seed 4 (scale 25: 33,554,432 vertices, 1.05G entries): a
+    round of 5 V walks; 300 sampled walks equal the oracle's, which walks the
+    device's CSR (same neighbour order, the device's common-neighbour counts)
+    with the weights placed in CSR order from the edge list; the visit counts
     add up to the corpus and every step follows an edge."""
+    import time
     V = 1 << scale
-    print(f"[walker-gpu] weighted R-MAT scale {scale}", flush=True)
+    t0 = time.time()
     edges = distger.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 4)
+    print(f"[walker-gpu] weighted R-MAT scale {scale}: {len(edges)} edges ({time.time() - t0:.0f} s)",
+          flush=True)
     ctx.graph_from_edges(V, edges, undirected=True, cn_mode=ffi.DG_CN_COMPUTE_WEIGHTED)
+    rp_w, w = dgoracle.csr_edge_data(V, edges)
+    del edges
+    print(f"[walker-gpu] scale {scale}: graph resident ({time.time() - t0:.0f} s)", flush=True)
     p = ctx.default_walk_params(seed=4)
     ctx.walk_reset()
     try:
@@ -334,13 +337,14 @@ def test_weighted_rmat_sampled_walks(ctx, scale):
     tok, off = ctx.corpus()
     assert int(ctx.vertex_freq().sum(dtype=np.uint64)) == len(tok)
     rp, nbr, cn = ctx.graph_csr()
-    g = dgoracle.OracleGraph(V, edges, weighted=True, cn=cn)
-    assert np.array_equal(g.row_ptr, rp)
+    assert np.array_equal(rp, rp_w)
+    g = dgoracle.OracleGraph.from_csr(rp, nbr, cn, w)
     rng = np.random.default_rng(scale)
     cfg = g.cfg(4, 20, 0, 0.999)
-    for w in rng.integers(0, 5 * V, 300):
-        path, _ = g.walk_one(cfg, 1, int(w))
-        dev = tok[off[w]:off[w + 1]]
-        assert np.array_equal(path, dev), f"walk {w} differs"
+    for wk in rng.integers(0, 5 * V, 300):
+        path, _ = g.walk_one(cfg, 1, int(wk))
+        dev = tok[off[wk]:off[wk + 1]]
+        assert np.array_equal(path, dev), f"walk {wk} differs"
         for a, b in zip(dev[:-1], dev[1:]):  # consecutive vertices are adjacent
             assert b in nbr[rp[a]:rp[a + 1]]
+    print(f"[walker-gpu] scale {scale}: 300 sampled walks equal ({time.time() - t0:.0f} s)", flush=True)
