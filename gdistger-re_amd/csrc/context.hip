@@ -55,6 +55,7 @@ extern "C" void dg_close(dg_ctx* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   dg_pipe_free(ctx);
+  dg_dsgl_free(ctx);
   dg_tuples_free(ctx);
   dg_sgns_free(ctx);
   dg_walk_free(ctx);

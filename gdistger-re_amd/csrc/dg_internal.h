@@ -49,6 +49,7 @@ struct dg_sgns_state;   // defined by the trainer translation unit
 struct dg_walk_plan;    // defined by the walker translation unit
 struct dg_pipe_state;   // defined by the pipeline translation unit
 struct dg_tuple_state;  // defined by the tuple-interface translation unit
+struct dg_dsgl_state;   // defined by the DSGL translation unit
 
 struct dg_ctx {
   int device = 0;
@@ -114,6 +115,7 @@ struct dg_ctx {
   dg_sgns_state* sgns = nullptr;
   dg_tuple_state* tuples = nullptr;
   dg_pipe_state* pipe = nullptr;
+  dg_dsgl_state* dsgl = nullptr;
 
   // timing
   std::map<std::string, dg_timing_entry> timing;
@@ -239,6 +241,29 @@ int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* p, float alpha, uint64
 int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, bool timed);
 int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* out);
 void dg_pipe_free(dg_ctx* ctx);
+
+// One call of the DSGL trainer (dsgl.hip) over a device corpus, on the
+// trainer's tables.
+struct dg_dsgl_call {
+  const uint32_t* tok;
+  const uint64_t* off;
+  uint64_t n_walks, n_tok;
+  uint64_t max_walk;        // longest walk of the corpus
+  const uint32_t* rank;
+  const uint32_t* keep;     // DSGL keep limits per rank
+  const int32_t* table;
+  uint32_t table_size, vocab;
+  const float* expt;
+  float* syn0;
+  float* syn1;
+  uint32_t dim, window, K, iter, workers;
+  float alpha0;
+  uint64_t train_words, seed;
+  uint32_t epoch0;
+  int exact, deterministic;
+};
+int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms, float* alpha_end);
+void dg_dsgl_free(dg_ctx* ctx);
 
 // Exclusive scan of n uint32 lengths into n+1 uint64 offsets (walker.hip).
 int dg_scan_lengths(dg_ctx* ctx, const uint32_t* d_len, uint64_t n, uint64_t* d_off);

@@ -43,6 +43,9 @@
 #define DG_TAG_GEN 0x47454E52u    /* synthetic graph generators */
 #define DG_TAG_SPLIT 0x53504C54u  /* link-prediction split */
 #define DG_TAG_TUPLE 0x54555045u  /* synthetic tuples and embedding tables */
+#define DG_TAG_DSGL_WIN 0x44535749u /* DSGL trainer: window shrink per (worker, step) */
+#define DG_TAG_DSGL_NEG 0x4453474Eu /* DSGL trainer: negative batch per (worker, batch) */
+#define DG_TAG_SGNS_REUSE 0x52555345u /* negative-reuse trainer: negatives per sentence */
 
 typedef struct {
   uint32_t x0, x1, x2, x3;

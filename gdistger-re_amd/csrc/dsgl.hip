@@ -1,0 +1,738 @@
+// dsgl.hip - DSGL, the reference's CPU trainer (include/dsgl.hpp:783-1427), on
+// gfx950: one wavefront is one DSGL compute thread.
+//
+// Semantics (the same as the oracle's restatement, oracle/dsgl_oracle.c):
+//   * the corpus, with one separator after every walk (local_corpus,
+//     include/path.hpp:96,110), is split into `workers` equal word ranges
+//     (dsgl.hpp:814-825); a worker reads sentences from its range: every
+//     position read counts as a word, a separator or the range's last
+//     position closes a sentence, subsampling drops tokens by
+//     sqrt(r) + r (r = sample * train_words / cn) with a Philox draw per
+//     corpus token, sentences of one token are skipped;
+//   * two sentences advance in lockstep, one window shrink per step for both
+//     (dsgl.hpp:1102-1134); their distinct input rows are copied into
+//     sentence buffers (here in LDS) and go back to syn0 when the sentence
+//     ends;
+//   * a batch of max(L1, L2) * K negatives is drawn whenever two sentences
+//     have finished (dsgl.hpp:1052-1083); its distinct output rows are
+//     copied into a batch buffer (here per-wavefront scratch in HBM) and go
+//     back to syn1 with the batch; each step reads K of them from position
+//     neg_count % batch;
+//   * per step the outputs are [target, distinct negatives with their
+//     multiplicity, target2] (a negative equal to the target dropped,
+//     dsgl.hpp:1138-1159), the contexts those of both windows; every
+//     gradient comes from one snapshot: G = Y X^T, g = (label - sig) * alpha
+//     * multiplicity with label 1 for (target, first-sentence contexts) and
+//     (target2, second-sentence contexts), dY = G X, dX = G^T Y
+//     (dsgl.hpp:1241-1310, the MKL branch); dX goes into the sentence
+//     buffers, dY into the batch buffer and, for the two targets, into syn1
+//     directly;
+//   * the learning rate steps every 10000 counted words (dsgl.hpp:882-894),
+//     the worker's own count times the number of workers standing in for
+//     the shared counter.
+// Chosen here: write-backs add live - snapshot (atomics) instead of
+// overwriting the rows, so concurrent wavefronts' updates compose; Philox
+// draws; a context that occurs several times in one window is one pool entry
+// weighted by its count (the same sum in another order).
+//
+// Per step on MFMA (v_mfma_f32_16x16x4f32): the outputs are the 16 rows of
+// the M dimension (target, at most 14 distinct negatives, target2), the distinct contexts are the N (or
+// K) dimension in tiles of 16, the embedding dims the K (or N) dimension.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dg_internal.h"
+#include "dg_philox.h"
+
+namespace {
+
+constexpr int kDsglMaxSentence = 1000;  // MAX_SENTENCE_LENGTH (dsgl.hpp:23)
+constexpr int kDsglMaxExp = 6;
+constexpr int kDsglExpTable = 1000;
+constexpr int kEntCap = 64;        // distinct contexts of one step (2 windows x 2W <= 60)
+constexpr int kGL = kEntCap + 4;   // G row stride
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+
+__device__ __forceinline__ f32x4_t mma4(float4 a, float4 b, f32x4_t c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A load that reads the level-2 cache (values this wavefront wrote earlier
+// in the launch with other lanes, or added by atomics).
+__device__ __forceinline__ float load_l2(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// word j of the Philox stream (counter (j / 4, worker, c2, tag), key)
+__device__ __forceinline__ uint32_t draw_word(uint64_t key, uint32_t worker, uint32_t c2,
+                                              uint32_t tag, uint32_t j) {
+  const dg_u32x4 o = dg_philox4x32_10(j >> 2, worker, c2, tag, (uint32_t)key, (uint32_t)(key >> 32));
+  const uint32_t r = j & 3u;
+  return r == 0 ? o.x0 : (r == 1 ? o.x1 : (r == 2 ? o.x2 : o.x3));
+}
+
+struct DsglArgs {
+  const uint32_t* tok;
+  const uint64_t* off;
+  uint64_t n_walks;
+  uint64_t words_num;      // tokens + one separator per walk
+  const uint32_t* rank;
+  const uint32_t* keep;
+  const int32_t* table;
+  uint32_t table_size;
+  uint32_t vocab;
+  const float* expt;
+  float* syn0;
+  float* syn1;
+  uint32_t window;
+  uint32_t K;
+  uint32_t workers;
+  uint32_t lcap;           // sentence tokens (and distinct words) one buffer holds
+  uint32_t ncap;           // negative draws of one batch at most (lcap * K)
+  float alpha0;
+  float total;             // (float)(iter * train_words + 1)
+  uint64_t key;            // Philox key of the epoch
+  int exact;
+  int deterministic;
+  float* scratch;          // per wavefront: sentence snapshots, batch snapshot and live rows
+  uint64_t scratch_stride; // floats per wavefront
+  unsigned long long* actual;  // per worker: counted words, carried across epochs
+  float* alpha;            // per worker: learning rate, carried across epochs
+  unsigned long long* kept;    // tokens kept by subsampling
+};
+
+// LDS of one wavefront (dynamic; sizes follow lcap, ncap and DIM).
+struct DsglLds {
+  float* V;       // [2][lcap][LDV] live sentence rows
+  float* Y;       // [16][LDV] the step's output rows
+  float* G;       // [16][kGL]
+  float* Z;       // [LDV] zeros
+  int* sen;       // [2][lcap]
+  int* slot;      // [2][lcap] word index of each position
+  int* word;      // [2][lcap] distinct words
+  int* nword;     // [ncap] negative draws of the batch
+  int* nslot;     // [ncap] batch-buffer index of each draw
+  int* nid;       // [ncap] distinct negatives
+  int* ent_row;   // [kEntCap] row of each distinct context (s * lcap + word index)
+  int* ent_cnt;   // [kEntCap]
+  int* out_id;    // [16]
+  int* out_meta;  // [16]
+  int* out_slot;  // [16]
+  int* tmp;       // [64]
+};
+
+template <int DIM>
+__host__ __device__ constexpr int dsgl_ldv() { return DIM + 4; }
+
+template <int DIM>
+__host__ __device__ inline size_t dsgl_lds_bytes(uint32_t lcap, uint32_t ncap) {
+  constexpr int LDV = dsgl_ldv<DIM>();
+  size_t f = (size_t)2 * lcap * LDV + 16 * LDV + 16 * kGL + LDV;
+  size_t i = (size_t)3 * 2 * lcap + 3 * (size_t)ncap + 2 * kEntCap + 3 * 16 + 64;
+  return ((f * 4 + i * 4) + 15) & ~(size_t)15;
+}
+
+template <int DIM>
+__device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t ncap) {
+  constexpr int LDV = dsgl_ldv<DIM>();
+  DsglLds L;
+  float* f = reinterpret_cast<float*>(base);
+  L.V = f;
+  f += (size_t)2 * lcap * LDV;
+  L.Y = f;
+  f += 16 * LDV;
+  L.G = f;
+  f += 16 * kGL;
+  L.Z = f;
+  f += LDV;
+  int* p = reinterpret_cast<int*>(f);
+  L.sen = p;
+  p += 2 * lcap;
+  L.slot = p;
+  p += 2 * lcap;
+  L.word = p;
+  p += 2 * lcap;
+  L.nword = p;
+  p += ncap;
+  L.nslot = p;
+  p += ncap;
+  L.nid = p;
+  p += ncap;
+  L.ent_row = p;
+  p += kEntCap;
+  L.ent_cnt = p;
+  p += kEntCap;
+  L.out_id = p;
+  p += 16;
+  L.out_meta = p;
+  p += 16;
+  L.out_slot = p;
+  p += 16;
+  L.tmp = p;
+  return L;
+}
+
+__device__ __forceinline__ float dsgl_sig(float f, const float* expt, int exact) {
+  if (exact) return (float)(1.0 / (1.0 + exp(-(double)f)));
+  if (f > (float)kDsglMaxExp) return 1.0f;
+  if (f < -(float)kDsglMaxExp) return 0.0f;
+  return expt[(int)((f + (float)kDsglMaxExp) * (float)(kDsglExpTable / kDsglMaxExp / 2))];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
+  extern __shared__ __attribute__((aligned(16))) char dsgl_smem[];
+  constexpr int LDV = dsgl_ldv<DIM>();
+  constexpr int NT = DIM / 16;  // dim tiles / k chunks
+  constexpr int PER = DIM / 64; // dims per lane in row copies
+  const int lane = threadIdx.x & 63;
+  const int l16 = lane & 15, g4 = lane >> 4;
+  DsglLds L = dsgl_carve<DIM>(dsgl_smem, a.lcap, a.ncap);
+  for (int i = lane; i < LDV; i += 64) L.Z[i] = 0.0f;
+  const uint64_t gw = blockIdx.x;
+  uint64_t nw = gridDim.x;
+  if (a.deterministic) {
+    nw = 1;
+    if (gw != 0) return;
+  }
+  float* snap = a.scratch + gw * a.scratch_stride;        // [2][lcap][DIM]
+  float* nsnap = snap + (size_t)2 * a.lcap * DIM;          // [ncap][DIM]
+  float* nlive = nsnap + (size_t)a.ncap * DIM;             // [ncap][DIM]
+  const int W = (int)a.window, K = (int)a.K;
+  const uint32_t lcap = a.lcap;
+  unsigned long long kept_total = 0;
+
+  for (uint64_t t = gw; t < a.workers; t += nw) {
+    // ---- the worker's word range (dsgl.hpp:814-825)
+    const uint64_t q = a.words_num / a.workers;
+    const uint64_t start = q * t;
+    const uint64_t end = start + (t == a.workers - 1 ? a.words_num - (uint64_t)(a.workers - 1) * q : q);
+    uint64_t cur = start;
+    uint64_t walk;
+    {
+      uint64_t lo = 0, hi = a.n_walks;  // last walk w with off[w] + w <= start
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (a.off[mid] + mid <= start) lo = mid; else hi = mid;
+      }
+      walk = uni64(lo);
+    }
+    uint64_t wc = 0, last_wc = 0;
+    unsigned long long actual = a.actual[t];
+    float alpha = a.alpha[t];
+    int len[2] = {0, 0}, pos[2] = {0, 0}, nwords[2] = {0, 0};
+    int tsn = 0, change_neg = 1, nbn = 0, nneg = 0;
+    uint32_t neg_count = 0, step = 0, batch = 0;
+
+    while (true) {
+      if (a.deterministic) __threadfence();
+      // ---- the learning rate (dsgl.hpp:882-894)
+      if (wc - last_wc > 10000) {
+        actual += wc - last_wc;
+        last_wc = wc;
+        float al = a.alpha0 * (1.0f - (float)(actual * (unsigned long long)a.workers) / a.total);
+        if (al < a.alpha0 * 0.0001f) al = a.alpha0 * 0.0001f;
+        alpha = al;
+      }
+      // ---- sentences (dsgl.hpp:899-1030)
+      for (int s = 0; s < 2; ++s) {
+        while (len[s] == 0) {
+          if (cur >= end) break;
+          int n = 0;
+          while (true) {
+            uint64_t sep = uni64(a.off[walk + 1] + walk);
+            while (sep < cur) {
+              ++walk;
+              sep = uni64(a.off[walk + 1] + walk);
+            }
+            // the sentence closes at the first separator or at the range's
+            // last position, whichever comes first (never kept)
+            const uint64_t stop = sep < end - 1 ? sep : end - 1;
+            const uint64_t span = stop - cur;  // positions before the stop
+            const uint64_t p = cur + (uint64_t)lane;
+            bool kp = false;
+            int r = -1;
+            if ((uint64_t)lane < span) {
+              const uint64_t ti = p - walk;
+              const uint32_t rr = a.rank[a.tok[ti]];
+              if (rr != 0xffffffffu) {
+                const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)ti, (uint32_t)(ti >> 32), DG_TAG_SAMPLE,
+                                                    (uint32_t)a.key, (uint32_t)(a.key >> 32));
+                kp = (o.x0 & 0xFFFFu) <= a.keep[rr];
+                r = (int)rr;
+              }
+            }
+            const uint64_t km = __ballot(kp);
+            const int nk = __popcll(km);
+            const int room = kDsglMaxSentence - n;
+            if (nk >= room) {
+              // the sentence reaches MAX_SENTENCE_LENGTH at the room-th kept token
+              uint64_t m = km;
+              for (int i = 1; i < room; ++i) m &= m - 1;
+              const int last = __builtin_ctzll(m);
+              if (kp && lane <= last) L.sen[s * lcap + n + (int)lanes_below(km)] = r;
+              n += room;
+              wc += (uint64_t)last + 1;
+              cur += (uint64_t)last + 1;
+              break;
+            }
+            if (kp) L.sen[s * lcap + n + (int)lanes_below(km)] = r;
+            n += nk;
+            if (span < 64) {
+              wc += span + 1;
+              cur += span + 1;
+              break;
+            }
+            wc += 64;
+            cur += 64;
+          }
+          kept_total += (unsigned long long)n;
+          wave_sync();
+          if (n <= 1) continue;
+          len[s] = n;
+          pos[s] = 0;
+          // distinct words in order of first appearance, and each position's word
+          int nwd = 0;
+          for (int i = 0; i < n; ++i) {
+            const int w = L.sen[s * lcap + i];
+            int found = -1;
+            for (int c0 = 0; c0 < nwd && found < 0; c0 += 64) {
+              const uint64_t m = __ballot(c0 + lane < nwd && L.word[s * lcap + c0 + lane] == w);
+              if (m) found = c0 + __builtin_ctzll(m);
+            }
+            if (found < 0) {
+              found = nwd;
+              if (lane == 0) L.word[s * lcap + nwd] = w;
+              ++nwd;
+            }
+            if (lane == 0) L.slot[s * lcap + i] = found;
+            wave_sync();
+          }
+          nwords[s] = nwd;
+          // live rows in LDS, snapshots in scratch (dsgl.hpp:942-945)
+          for (int u = 0; u < nwd; ++u) {
+            const float* src = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              const float v = src[lane + 64 * j];
+              L.V[(s * lcap + u) * LDV + lane + 64 * j] = v;
+              snap[((size_t)s * lcap + u) * DIM + lane + 64 * j] = v;
+            }
+          }
+          wave_sync();
+        }
+      }
+      if (cur >= end) {
+        // the range is exhausted: sentences in hand and the batch are dropped
+        actual += wc - last_wc;
+        break;
+      }
+      // ---- a new negative batch (dsgl.hpp:1052-1083)
+      if (change_neg) {
+        nbn = len[0] > len[1] ? len[0] : len[1];
+        const int nd = nbn * K;
+        for (int i0 = 0; i0 < nd; i0 += 64) {
+          const int i = i0 + lane;
+          if (i < nd) {
+            const uint32_t x = draw_word(a.key, (uint32_t)t, batch, DG_TAG_DSGL_NEG, (uint32_t)i);
+            int smp = a.table[x % a.table_size];
+            if (!smp) smp = (int)(x % (a.vocab - 1) + 1);
+            L.nword[i] = smp;
+          }
+        }
+        wave_sync();
+        // distinct draws in order of first appearance
+        int nd_first = 0;
+        for (int i0 = 0; i0 < nd; i0 += 64) {
+          const int i = i0 + lane;
+          int first_of = -1;
+          if (i < nd) {
+            const int w = L.nword[i];
+            for (int j = 0; j < i; ++j)
+              if (L.nword[j] == w) {
+                first_of = j;
+                break;
+              }
+          }
+          const uint64_t fm = __ballot(i < nd && first_of < 0);
+          if (i < nd && first_of < 0) {
+            const int k = nd_first + (int)lanes_below(fm);
+            L.nid[k] = L.nword[i];
+            L.nslot[i] = k;
+          }
+          nd_first += __popcll(fm);
+          wave_sync();
+          if (i < nd && first_of >= 0) L.tmp[lane] = first_of;
+          wave_sync();
+          if (i < nd && first_of >= 0) L.nslot[i] = L.nslot[L.tmp[lane]];
+          wave_sync();
+        }
+        nneg = nd_first;
+        for (int k = 0; k < nneg; ++k) {
+          const float* src = a.syn1 + (uint64_t)L.nid[k] * DIM;
+#pragma unroll
+          for (int j = 0; j < PER; ++j) {
+            const float v = src[lane + 64 * j];
+            nsnap[(size_t)k * DIM + lane + 64 * j] = v;
+            nlive[(size_t)k * DIM + lane + 64 * j] = v;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        ++batch;
+        change_neg = 0;
+      }
+      // ---- one step: both positions (dsgl.hpp:1085-1356)
+      const int target = uni(L.sen[pos[0]]);
+      const int target2 = uni(L.sen[lcap + pos[1]]);
+      const int b = (int)(draw_word(a.key, (uint32_t)t, step, DG_TAG_DSGL_WIN, 0) % (uint32_t)W);
+      ++step;
+      // contexts: lanes 0..31 the first window, 32..63 the second
+      const int hs = lane >> 5, li = lane & 31;
+      const int span = 2 * (W - b);
+      int my = -1;
+      {
+        const int i = b + li + (b + li >= W ? 1 : 0);
+        const int c = pos[hs] - W + i;
+        if (li < span && c >= 0 && c < len[hs]) my = L.slot[hs * lcap + c];
+      }
+      L.tmp[lane] = my;
+      wave_sync();
+      bool first = my >= 0;
+      int cnt = 0;
+      for (int j = 0; j < 32; ++j) {
+        const int o = L.tmp[hs * 32 + j];
+        cnt += (my >= 0 && o == my) ? 1 : 0;
+        if (j < li && o == my) first = false;
+      }
+      const uint64_t em = __ballot(first);
+      const int E = __popcll(em);
+      const int E1 = __popcll(em & 0xffffffffull);
+      if (first) {
+        const int e = (int)lanes_below(em);
+        L.ent_row[e] = hs * (int)lcap + my;
+        L.ent_cnt[e] = cnt;
+      }
+      // outputs: [target, distinct negatives with multiplicity, target2]
+      ++neg_count;
+      const int nst = (int)(neg_count % (uint32_t)nbn);
+      int smp = -1, meta = 0, sl = 0;
+      bool nfirst = false;
+      if (lane < K) {
+        smp = L.nword[nst * K + lane];
+        sl = L.nslot[nst * K + lane];
+        nfirst = smp != target;
+        for (int k = 0; k < K; ++k) {
+          const int o = L.nword[nst * K + k];
+          if (o == smp) {
+            ++meta;
+            if (k < lane) nfirst = false;
+          }
+        }
+      }
+      const uint64_t om = __ballot(nfirst);
+      const int O = 2 + __popcll(om);
+      if (nfirst) {
+        const int o = 1 + (int)lanes_below(om);
+        L.out_id[o] = smp;
+        L.out_meta[o] = meta;
+        L.out_slot[o] = sl;
+      }
+      if (lane == 0) {
+        L.out_id[0] = target;
+        L.out_meta[0] = 1;
+        L.out_id[O - 1] = target2;
+        L.out_meta[O - 1] = 1;
+      }
+      wave_sync();
+      // output rows: the targets from syn1, the negatives from the batch buffer
+      for (int r = 0; r < 16; ++r) {
+        float* dst = L.Y + r * LDV;
+        if (r < O) {
+          const bool tab = (r == 0 || r == O - 1);
+          const float* src = tab ? a.syn1 + (uint64_t)L.out_id[r] * DIM
+                                 : nlive + (size_t)L.out_slot[r] * DIM;
+          // from the level-2 cache: the rows this wavefront's atomics moved
+#pragma unroll
+          for (int j = 0; j < PER; ++j) dst[lane + 64 * j] = load_l2(src + lane + 64 * j);
+        } else {
+#pragma unroll
+          for (int j = 0; j < PER; ++j) dst[lane + 64 * j] = 0.0f;
+        }
+      }
+      wave_sync();
+      // G = Y X^T and g (dsgl.hpp:1241-1273)
+      const int ET = (E + 15) >> 4;
+      for (int et = 0; et < ET; ++et) {
+        const int e = 16 * et + l16;
+        const float* xr = e < E ? L.V + L.ent_row[e] * LDV : L.Z;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NT; ++kc) {
+          const float4 ya = *reinterpret_cast<const float4*>(L.Y + l16 * LDV + 16 * kc + 4 * g4);
+          const float4 xb = *reinterpret_cast<const float4*>(xr + 16 * kc + 4 * g4);
+          acc = mma4(ya, xb, acc);
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int i = 4 * g4 + x;
+          float g = 0.0f;
+          if (i < O && e < E) {
+            const bool s1 = e < E1;
+            const float label = ((i == 0 && s1) || (i == O - 1 && !s1)) ? 1.0f : 0.0f;
+            g = (label - dsgl_sig(acc[x], a.expt, a.exact)) * alpha * (float)L.out_meta[i] *
+                (float)L.ent_cnt[e];
+          }
+          L.G[i * kGL + e] = g;
+        }
+      }
+      wave_sync();
+      // dY = G X: lane holds dY[4 g4 + x][16 tt + l16]
+      for (int tt = 0; tt < NT; ++tt) {
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ec = 0; ec < ET; ++ec) {
+          const float4 ga = *reinterpret_cast<const float4*>(L.G + l16 * kGL + 16 * ec + 4 * g4);
+          float xv[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const int e = 16 * ec + 4 * g4 + x;
+            xv[x] = (e < E ? L.V + L.ent_row[e] * LDV : L.Z)[16 * tt + l16];
+          }
+          acc = mma4(ga, make_float4(xv[0], xv[1], xv[2], xv[3]), acc);
+        }
+        const int d = 16 * tt + l16;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int i = 4 * g4 + x;
+          if (i >= O) continue;
+          if (i == 0)
+            atomicAdd(a.syn1 + (uint64_t)target * DIM + d, acc[x]);
+          else if (i == O - 1)
+            atomicAdd(a.syn1 + (uint64_t)target2 * DIM + d, acc[x]);
+          else
+            atomicAdd(nlive + (size_t)L.out_slot[i] * DIM + d, acc[x]);
+        }
+      }
+      // dX = G^T Y into the sentence buffers: lane holds dX[16 et + 4 g4 + x][16 tt + l16]
+      for (int et = 0; et < ET; ++et) {
+        float ga[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) ga[x] = L.G[(4 * g4 + x) * kGL + 16 * et + l16];
+        const float4 a4 = make_float4(ga[0], ga[1], ga[2], ga[3]);
+        f32x4_t acc[NT];
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+          float yv[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) yv[x] = L.Y[(4 * g4 + x) * LDV + 16 * tt + l16];
+          acc[tt] = mma4(a4, make_float4(yv[0], yv[1], yv[2], yv[3]), f32x4_t{0.f, 0.f, 0.f, 0.f});
+        }
+        wave_sync();
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const int e = 16 * et + 4 * g4 + x;
+            if (e < E) L.V[L.ent_row[e] * LDV + 16 * tt + l16] += acc[tt][x];
+          }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      wave_sync();
+      // ---- finished sentences and batch back to the tables (dsgl.hpp:1358-1404)
+      for (int s = 0; s < 2; ++s) {
+        if (++pos[s] < len[s]) continue;
+        for (int u = 0; u < nwords[s]; ++u) {
+          float* dst = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+#pragma unroll
+          for (int j = 0; j < PER; ++j) {
+            const int d = lane + 64 * j;
+            atomicAdd(dst + d, L.V[(s * lcap + u) * LDV + d] - load_l2(snap + ((size_t)s * lcap + u) * DIM + d));
+          }
+        }
+        len[s] = 0;
+        ++tsn;
+      }
+      if (tsn >= 2) {
+        for (int k = 0; k < nneg; ++k) {
+          float* dst = a.syn1 + (uint64_t)L.nid[k] * DIM;
+#pragma unroll
+          for (int j = 0; j < PER; ++j) {
+            const int d = lane + 64 * j;
+            atomicAdd(dst + d, load_l2(nlive + (size_t)k * DIM + d) - load_l2(nsnap + (size_t)k * DIM + d));
+          }
+        }
+        change_neg = 1;
+        tsn = 0;
+      }
+      wave_sync();
+    }
+    if (lane == 0) {
+      a.actual[t] = actual;
+      a.alpha[t] = alpha;
+    }
+  }
+  if (lane == 0 && kept_total) atomicAdd(a.kept, kept_total);
+}
+
+__global__ void dsgl_state_init_kernel(unsigned long long* actual, float* alpha, uint32_t n,
+                                       float alpha0, unsigned long long* kept) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    actual[i] = 0ull;
+    alpha[i] = alpha0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *kept = 0ull;
+}
+
+template <int DIM>
+int launch_dims(dg_ctx* ctx, const DsglArgs& a, size_t lds, unsigned grid) {
+  auto k = dsgl_kernel<DIM>;
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "DSGL kernel LDS attribute: %s", hipGetErrorString(e));
+  k<<<grid, 64, lds, ctx->stream>>>(a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "DSGL kernel launch: %s", hipGetErrorString(e));
+  return DG_OK;
+}
+
+template <int DIM>
+int plan_dims(dg_ctx* ctx, uint32_t lcap, uint32_t ncap, size_t* lds, unsigned* per_cu) {
+  *lds = dsgl_lds_bytes<DIM>(lcap, ncap);
+  if (*lds > 160 * 1024)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED,
+                   "DSGL form: sentences of %u tokens at dim %d need %zu bytes of LDS per wavefront "
+                   "(160 KB available)", lcap, DIM, *lds);
+  int n = 0;
+  hipError_t e = hipFuncSetAttribute((const void*)dsgl_kernel<DIM>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dsgl_kernel<DIM>, 64, *lds);
+  if (e != hipSuccess) return dg_fail(ctx, DG_ERR_HIP, "DSGL occupancy: %s", hipGetErrorString(e));
+  *per_cu = n < 1 ? 1u : (unsigned)n;
+  return DG_OK;
+}
+
+}  // namespace
+
+struct dg_dsgl_state {
+  float* scratch = nullptr;
+  size_t scratch_cap = 0;
+  unsigned long long* actual = nullptr;
+  size_t actual_cap = 0;
+  float* alpha = nullptr;
+  size_t alpha_cap = 0;
+  unsigned long long* kept = nullptr;
+};
+
+void dg_dsgl_free(dg_ctx* ctx) {
+  dg_dsgl_state* s = ctx->dsgl;
+  if (!s) return;
+  if (s->scratch) hipFree(s->scratch);
+  if (s->actual) hipFree(s->actual);
+  if (s->alpha) hipFree(s->alpha);
+  if (s->kept) hipFree(s->kept);
+  delete s;
+  ctx->dsgl = nullptr;
+}
+
+int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms, float* alpha_end) {
+  if (c.dim != 64 && c.dim != 128 && c.dim != 256)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "DSGL form: dim %u (64, 128 or 256)", c.dim);
+  if (!ctx->dsgl) ctx->dsgl = new dg_dsgl_state();
+  dg_dsgl_state* st = ctx->dsgl;
+  const uint32_t lcap = c.max_walk < (uint64_t)kDsglMaxSentence ? (uint32_t)(c.max_walk ? c.max_walk : 1)
+                                                                 : (uint32_t)kDsglMaxSentence;
+  const uint32_t ncap = lcap * (c.K ? c.K : 1);
+  size_t lds = 0;
+  unsigned per_cu = 1;
+  switch (c.dim) {
+    case 64: DG_CHECK(plan_dims<64>(ctx, lcap, ncap, &lds, &per_cu)); break;
+    case 128: DG_CHECK(plan_dims<128>(ctx, lcap, ncap, &lds, &per_cu)); break;
+    default: DG_CHECK(plan_dims<256>(ctx, lcap, ncap, &lds, &per_cu)); break;
+  }
+  const uint64_t resident = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
+  uint32_t workers = c.workers ? c.workers : (uint32_t)std::min<uint64_t>(resident, 1u << 20);
+  if ((uint64_t)workers > c.n_walks + c.n_tok) workers = (uint32_t)std::max<uint64_t>(1, (c.n_walks + c.n_tok) / 64);
+  const unsigned grid = c.deterministic ? 1u : (unsigned)std::min<uint64_t>(workers, resident);
+  const uint64_t stride = ((uint64_t)2 * lcap + 2 * (uint64_t)ncap) * c.dim;
+  DG_CHECK(dg_reserve(ctx, &st->scratch, &st->scratch_cap, (size_t)(grid * stride)));
+  DG_CHECK(dg_reserve(ctx, &st->actual, &st->actual_cap, (size_t)workers));
+  DG_CHECK(dg_reserve(ctx, &st->alpha, &st->alpha_cap, (size_t)workers));
+  if (!st->kept) DG_HIP(ctx, hipMalloc((void**)&st->kept, sizeof(unsigned long long)));
+  dsgl_state_init_kernel<<<(workers + 255) / 256, 256, 0, ctx->stream>>>(st->actual, st->alpha, workers,
+                                                                          c.alpha0, st->kept);
+  DG_HIP(ctx, hipGetLastError());
+  DsglArgs a;
+  a.tok = c.tok;
+  a.off = c.off;
+  a.n_walks = c.n_walks;
+  a.words_num = c.n_tok + c.n_walks;
+  a.rank = c.rank;
+  a.keep = c.keep;
+  a.table = c.table;
+  a.table_size = c.table_size;
+  a.vocab = c.vocab;
+  a.expt = c.expt;
+  a.syn0 = c.syn0;
+  a.syn1 = c.syn1;
+  a.window = c.window;
+  a.K = c.K;
+  a.workers = workers;
+  a.lcap = lcap;
+  a.ncap = ncap;
+  a.alpha0 = c.alpha0;
+  a.total = (float)((uint64_t)c.iter * c.train_words + 1);
+  a.exact = c.exact;
+  a.deterministic = c.deterministic;
+  a.scratch = st->scratch;
+  a.scratch_stride = stride;
+  a.actual = st->actual;
+  a.alpha = st->alpha;
+  a.kept = st->kept;
+  double total_ms = 0.0;
+  for (uint32_t ep = 0; ep < c.iter; ++ep) {
+    a.key = dg_epoch_key(c.seed, c.epoch0 + ep);
+    size_t tk;
+    DG_CHECK(dg_time_begin(ctx, "sgns_dsgl", &tk));
+    switch (c.dim) {
+      case 64: DG_CHECK(launch_dims<64>(ctx, a, lds, grid)); break;
+      case 128: DG_CHECK(launch_dims<128>(ctx, a, lds, grid)); break;
+      default: DG_CHECK(launch_dims<256>(ctx, a, lds, grid)); break;
+    }
+    DG_CHECK(dg_time_end(ctx, tk));
+  }
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& pe : ctx->pending) {
+    float m = 0.f;
+    if (pe.name == "sgns_dsgl") {
+      DG_HIP(ctx, hipEventElapsedTime(&m, pe.start, pe.stop));
+      total_ms += m;
+    }
+  }
+  unsigned long long k = 0;
+  DG_HIP(ctx, hipMemcpy(&k, st->kept, sizeof(k), hipMemcpyDeviceToHost));
+  *kept = k;
+  *ms = total_ms;
+  DG_HIP(ctx, hipMemcpy(alpha_end, st->alpha + (workers - 1), sizeof(float), hipMemcpyDeviceToHost));
+  return DG_OK;
+}

@@ -389,3 +389,105 @@ extern "C" int dg_read_edges_bin(const char* path, dg_edge* edges, uint64_t e_ca
   fclose(f);
   return got == n ? DG_OK : DG_ERR_IO;
 }
+
+extern "C" int dg_convert_text_edges(const char* in_path, const char* out_path, int weighted,
+                                     uint64_t* e_num) {
+  if (!in_path || !out_path) return DG_ERR_ARG;
+  FILE* in = fopen(in_path, "r");
+  if (!in) return DG_ERR_IO;
+  FILE* out = fopen(out_path, "wb");
+  if (!out) {
+    fclose(in);
+    return DG_ERR_IO;
+  }
+  // records are buffered and written in blocks; the reference collects them
+  // in a vector and writes once (gconverter.cpp:70-76, storage.hpp:108-115)
+  std::vector<char> buf;
+  buf.reserve(1 << 22);
+  uint64_t n = 0;
+  bool ok = true;
+  uint32_t rec[3];
+  const size_t rsize = weighted ? 12 : 8;
+  while (ok) {
+    unsigned src = 0, dst = 0;
+    float w = 0.0f;
+    if (weighted) {
+      if (fscanf(in, "%u %u %f", &src, &dst, &w) != 3) break;
+    } else if (fscanf(in, "%u %u", &src, &dst) != 2) {
+      break;
+    }
+    rec[0] = src;
+    rec[1] = dst;
+    memcpy(&rec[2], &w, sizeof(float));
+    const char* p = reinterpret_cast<const char*>(rec);
+    buf.insert(buf.end(), p, p + rsize);
+    ++n;
+    if (buf.size() >= (1u << 22)) {
+      ok = fwrite(buf.data(), 1, buf.size(), out) == buf.size();
+      buf.clear();
+    }
+  }
+  if (ok && !buf.empty()) ok = fwrite(buf.data(), 1, buf.size(), out) == buf.size();
+  fclose(in);
+  if (fclose(out) != 0) ok = false;
+  if (e_num) *e_num = n;
+  return ok ? DG_OK : DG_ERR_IO;
+}
+
+extern "C" int dg_lp_write_files(const char* train_path, const char* test_edges_path,
+                                 const char* test_labels_path, const dg_edge* train,
+                                 uint64_t n_train, const dg_edge* pos, uint64_t n_pos,
+                                 const dg_edge* neg, uint64_t n_neg) {
+  if ((n_train && !train) || (n_pos && !pos) || (n_neg && !neg)) return DG_ERR_ARG;
+  auto write_all = [](const char* path, auto&& emit) -> bool {
+    if (!path) return true;
+    FILE* f = fopen(path, "w");
+    if (!f) return false;
+    bool ok = emit(f);
+    return fclose(f) == 0 && ok;
+  };
+  bool ok = write_all(train_path, [&](FILE* f) {
+    for (uint64_t i = 0; i < n_train; ++i)
+      if (fprintf(f, "%u %u %u\n", train[i].src, train[i].dst, (uint32_t)train[i].data) < 0) return false;
+    return true;
+  });
+  ok = ok && write_all(test_edges_path, [&](FILE* f) {
+    for (uint64_t i = 0; i < n_pos; ++i)
+      if (fprintf(f, "%u %u\n", pos[i].src, pos[i].dst) < 0) return false;
+    for (uint64_t i = 0; i < n_neg; ++i)
+      if (fprintf(f, "%u %u\n", neg[i].src, neg[i].dst) < 0) return false;
+    return true;
+  });
+  ok = ok && write_all(test_labels_path, [&](FILE* f) {
+    for (uint64_t i = 0; i < n_pos; ++i)
+      if (fputs("1\n", f) < 0) return false;
+    for (uint64_t i = 0; i < n_neg; ++i)
+      if (fputs("0\n", f) < 0) return false;
+    return true;
+  });
+  return ok ? DG_OK : DG_ERR_IO;
+}
+
+extern "C" float dg_lr_value(int kind, float initial, float a, float b, int step) {
+  switch (kind) {
+    case DG_LR_FIXED:
+      return initial;
+    case DG_LR_EXPONENTIAL:  // lr_scheduler.hpp:23-31: float exp of a float argument
+      return initial * std::exp(-a * (float)step);
+    case DG_LR_COSINE: {  // lr_scheduler.hpp:33-47: min_lr held as double, cos in double
+      const int t_max = (int)b;
+      if (t_max <= 0) return std::nanf("");
+      const float cosine = (float)std::cos(M_PI * step / t_max);
+      const double min_lr = (double)a;
+      return (float)(min_lr + 0.5 * (initial - min_lr) * (1 + cosine));
+    }
+    case DG_LR_STEP: {  // lr_scheduler.hpp:49-60: pow(float, int) in double
+      const int step_size = (int)b;
+      if (step_size <= 0) return std::nanf("");
+      const int factor = step / step_size;
+      return (float)(initial * std::pow((double)a, (double)factor));
+    }
+    default:
+      return std::nanf("");
+  }
+}

@@ -61,6 +61,10 @@ struct SgnsDev {
   uint32_t window;
   uint32_t negative;
   float alpha;
+  // the negative-reuse form (__sgNegReuse, word2vec.cu:231-303): `negative`
+  // negatives per sentence of the epoch, shared by all its positions; null
+  // for the plain form
+  const int32_t* reuse_negs;
 };
 
 // ---- wave helpers --------------------------------------------------------
@@ -146,10 +150,14 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
   for (uint64_t s = s_begin + wave; s < s_end; s += nwaves) {
     const uint64_t sb = a.sen_off[s], se = a.sen_off[s + 1];
     unsigned long long next_random = (unsigned long long)(s - s_begin);  // blockIdx.x
+    const int32_t* rn = a.reuse_negs ? a.reuse_negs + s * a.negative : nullptr;
     for (uint64_t pos = sb; pos < se; ++pos) {
       const int word = a.sen[pos];
-      next_random = next_random * 2514903917ULL + 11;  // word2vec.cu:321 (sic)
-      const int b = (int)(next_random % a.window);
+      int b = 0;  // the negative-reuse form trains the full window (word2vec.cu:257)
+      if (!rn) {
+        next_random = next_random * 2514903917ULL + 11;  // word2vec.cu:321 (sic)
+        b = (int)(next_random % a.window);
+      }
       for (int aa = b; aa < (int)a.window * 2 + 1 - b; ++aa) {
         if (aa == (int)a.window) continue;
         const int64_t c = (int64_t)pos - (int64_t)a.window + aa;
@@ -167,9 +175,13 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
             target = word;
             label = 1;
           } else {
-            next_random = next_random * 25214903917ULL + 11;
-            target = a.table[(next_random >> 16) % a.table_size];
-            if (target == 0) target = (int)(next_random % (a.vocab - 1) + 1);
+            if (rn) {
+              target = rn[d - 1];  // the sentence's shared negatives (word2vec.cu:247-249)
+            } else {
+              next_random = next_random * 25214903917ULL + 11;
+              target = a.table[(next_random >> 16) % a.table_size];
+              if (target == 0) target = (int)(next_random % (a.vocab - 1) + 1);
+            }
             if (target == word) continue;
             label = 0;
           }
@@ -459,10 +471,20 @@ __global__ void zero_u64_kernel(unsigned long long* p) {
   if (threadIdx.x == 0) *p = 0ull;
 }
 
-// Learning rate of every launch of an epoch from the device corpus offsets:
-// the linear schedule of dsgl.hpp:886-897 evaluated at each launch's first
-// sentence, the same double expression the host evaluates for a direct call,
-// so both give the same float.
+// The linear learning rate of dsgl.hpp:886-897 in its float arithmetic,
+// alpha0 * (1 - words / (iter * train_words + 1)) clamped at alpha0 * 1e-4,
+// with `words` the corpus words consumed before the launch (the current
+// epoch's and every earlier epoch's of the call) and train_words the call's
+// corpus words per epoch; host and device evaluate the same expression.
+__host__ __device__ __forceinline__ float linear_alpha(float alpha0, uint64_t words,
+                                                       uint64_t total_plus_one) {
+  float al = alpha0 * (1.0f - (float)words / (float)total_plus_one);
+  if (al < alpha0 * 0.0001f) al = alpha0 * 0.0001f;
+  return al;
+}
+
+// Learning rate of every launch of an epoch from the device corpus offsets
+// (the captured step does not know them on the host).
 __global__ void alpha_kernel(const uint64_t* woff, uint64_t n_walks, uint64_t mb, uint32_t n_mb,
                              float alpha0, int linear, uint32_t iter, uint32_t ep, float* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -471,13 +493,25 @@ __global__ void alpha_kernel(const uint64_t* woff, uint64_t n_walks, uint64_t mb
     out[i] = alpha0;
     return;
   }
-  const double n_tok = (double)(woff[n_walks] - woff[0]);
-  const double total_words = (double)iter * n_tok;
-  const double words_before = (double)ep * n_tok;
-  float al = alpha0 * (float)(1.0 - (words_before + (double)(woff[(uint64_t)i * mb] - woff[0])) /
-                                        (total_words + 1.0));
-  if (al < alpha0 * 0.0001f) al = alpha0 * 0.0001f;
-  out[i] = al;
+  const uint64_t n_tok = woff[n_walks] - woff[0];
+  const uint64_t words = (uint64_t)ep * n_tok + (woff[(uint64_t)i * mb] - woff[0]);
+  out[i] = linear_alpha(alpha0, words, (uint64_t)iter * n_tok + 1);
+}
+
+// Negatives of the negative-reuse form: `negative` per sentence of the epoch,
+// drawn as the reference's host loop draws them (word2vec.cu:1057-1062):
+// table[x % table_size], 0 -> x % (vocab - 1) + 1, with x word k of the
+// Philox stream (counter (k / 4, sentence, 0, REUSE), key of the epoch).
+__global__ void reuse_negs_kernel(uint64_t n_sen, uint32_t K, uint64_t key, const int32_t* table,
+                                  uint32_t table_size, uint32_t vocab, int32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_sen * K;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = i / K;
+    const uint32_t k = (uint32_t)(i - s * K);
+    const uint32_t x = stream_word(key, (uint32_t)s, 0u, DG_TAG_SGNS_REUSE, k);
+    const int v = table[x % table_size];
+    out[i] = v == 0 ? (int)(x % (vocab - 1) + 1) : v;
+  }
 }
 
 // Adds an epoch's kept-token count (the last subsampled offset) to *acc.
@@ -991,6 +1025,8 @@ struct dg_sgns_state {
   size_t shrink_cap = 0;
   int32_t* d_negs = nullptr;    // batched: negatives per kept position of a launch
   size_t negs_cap = 0;
+  int32_t* d_reuse = nullptr;   // negative-reuse form: negatives per sentence of an epoch
+  size_t reuse_cap = 0;
   uint8_t* d_ucount = nullptr;  // batched: distinct vertices per sentence of a launch
   size_t ucount_cap = 0;
   unsigned long long* d_pool_rows = nullptr;  // batched: distinct vertices per sentence, summed
@@ -1005,6 +1041,7 @@ struct dg_sgns_state {
     uint64_t n_walks, mb, pos_bound;
   } g{};
   uint32_t epoch = 0;       // epochs trained so far: part of every draw's Philox key
+  std::vector<float> last_alphas;  // learning rate of every launch of the last training call
   uint64_t version = 0;     // changes on every write of syn0 / syn1
   bool has_vocab = true;    // false for dg_sgns_tables_create tables
   uint32_t rank_size = 0;   // entries of d_rank (vertex ids accepted by score_pairs)
@@ -1016,7 +1053,8 @@ void dg_sgns_free(dg_ctx* ctx) {
   if (!s) return;
   void* bufs[] = {s->d_rank,  s->d_syn0,    s->d_syn1, s->d_table,   s->d_expt,
                   s->d_keep,  s->d_sen,     s->d_sen_off, s->d_cnt, s->d_shrink,
-                  s->d_negs,  s->d_ucount,  s->d_pool_rows, s->d_alpha, s->d_kept};
+                  s->d_negs,  s->d_ucount,  s->d_pool_rows, s->d_alpha, s->d_kept,
+                  s->d_reuse};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete s;
@@ -1078,11 +1116,18 @@ static int check_params(dg_ctx* ctx, const dg_sgns_params* p) {
   if (p->negative > (uint32_t)kMaxNeg)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "negative %u: at most %d", p->negative, kMaxNeg);
   if (p->table_size == 0) return dg_fail(ctx, DG_ERR_ARG, "table_size must be positive");
-  if (p->form > DG_SGNS_BATCHED_BF16) return dg_fail(ctx, DG_ERR_ARG, "unknown form %u", p->form);
+  if (p->form > DG_SGNS_NEGREUSE_FP32) return dg_fail(ctx, DG_ERR_ARG, "unknown form %u", p->form);
   if (p->negative_draws != 0 && p->negative_draws < p->negative)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "negative_draws %u: at least negative (%u)",
                    p->negative_draws, p->negative);
-  if (p->form != DG_SGNS_PLAIN_FP32) {
+  if (p->form == DG_SGNS_DSGL_FP32) {
+    if (p->negative > 14)
+      return dg_fail(ctx, DG_ERR_UNSUPPORTED, "DSGL form: at most 14 negatives (16 output rows per step)");
+    if (p->window > (uint32_t)kMaxSentWindow)
+      return dg_fail(ctx, DG_ERR_UNSUPPORTED, "DSGL form: window at most %d", kMaxSentWindow);
+    if (p->dim > 256) return dg_fail(ctx, DG_ERR_UNSUPPORTED, "DSGL form supports dim <= 256");
+  }
+  if (p->form == DG_SGNS_BATCHED_FP32 || p->form == DG_SGNS_BATCHED_BF16) {
     const uint32_t kd = p->negative_draws ? p->negative_draws : p->negative;
     if (kd + 1 > (uint32_t)kOutCols)
       return dg_fail(ctx, DG_ERR_UNSUPPORTED, "batched forms: at most %d negatives per position",
@@ -1139,9 +1184,11 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
   s->vertex_of_rank.clear();
   s->cn_of_rank.clear();
   s->train_words = 0;
+  // the DSGL form keeps every vertex (my_LearnVocabFromTrainFile, dsgl.hpp:314-350)
+  const uint64_t min_count = p->form == DG_SGNS_DSGL_FP32 ? 0 : p->min_count;
   for (uint32_t a = 0; a < V; ++a) {
     const uint32_t v = order[a];
-    if (cn[v] < p->min_count && a != 0) continue;
+    if (cn[v] < min_count && a != 0) continue;
     rank[v] = (uint32_t)s->vertex_of_rank.size();
     s->vertex_of_rank.push_back(v);
     s->cn_of_rank.push_back(cn[v]);
@@ -1192,10 +1239,24 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
     }
     for (auto& x : th) x.join();
   }
-  // unigram table (double accumulation, word2vec.cu:537-552)
+  // unigram table: word2vec.cu:537-552 (double cursor), or for the DSGL form
+  // dsgl.hpp:121-147 (float cursor, the index clamped before its term is added)
   s->table_size = p->table_size;
   std::vector<int32_t> table(p->table_size);
-  {
+  if (p->form == DG_SGNS_DSGL_FP32) {
+    double pw = 0.;
+    for (uint32_t a = 0; a < vocab; ++a) pw += std::pow((double)s->cn_of_rank[a], (double)0.75f);
+    int i = 0;
+    float d1 = (float)(std::pow((double)s->cn_of_rank[0], (double)0.75f) / pw);
+    for (uint32_t a = 0; a < p->table_size; ++a) {
+      table[a] = i;
+      if (a / (float)p->table_size > d1) {
+        i++;
+        if (i >= (int)vocab) i = (int)vocab - 1;
+        d1 += (float)(std::pow((double)s->cn_of_rank[i], (double)0.75f) / pw);
+      }
+    }
+  } else {
     double pw = 0;
     for (uint32_t a = 0; a < vocab; ++a) pw += std::pow((double)s->cn_of_rank[a], 0.75);
     uint32_t i = 0;
@@ -1216,16 +1277,24 @@ extern "C" int dg_sgns_init(dg_ctx* ctx, const dg_sgns_params* p) {
     expt[i] = expt[i] / (expt[i] + 1);
   }
   expt[kExpTable] = expt[kExpTable - 1];
-  // subsampling keep limits (float arithmetic of word2vec.cu:1015)
+  // subsampling keep limits: float arithmetic of word2vec.cu:1015, or of
+  // dsgl.hpp:922-926 (sqrt(r) + r, r = sample * train_words / cn; the word's
+  // own count) for the DSGL form
   std::vector<uint32_t> keep(vocab);
   for (uint32_t w = 0; w < vocab; ++w) {
-    if (!(p->sample > 0)) {
+    if (!(p->sample > 0) || s->cn_of_rank[w] == 0) {
       keep[w] = 65536;
       continue;
     }
     const float st = p->sample * (float)s->train_words;
     const float c = (float)s->cn_of_rank[w];
-    const float ran = (std::sqrt(c / st) + 1) * st / c;
+    float ran;
+    if (p->form == DG_SGNS_DSGL_FP32) {
+      const float ratio = st / c;
+      ran = std::sqrt(ratio) + ratio;
+    } else {
+      ran = (std::sqrt(c / st) + 1) * st / c;
+    }
     const float lim = ran * 65536.0f;
     keep[w] = lim >= 65536.0f ? 65536u : (uint32_t)lim;
   }
@@ -1454,7 +1523,6 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_HIP(ctx, hipMemcpy(h_woff.data(), d_off, sizeof(uint64_t) * (n_walks + 1), hipMemcpyDeviceToHost));
   const float alpha0 = alpha_in > 0 ? alpha_in : p.alpha;
-  const double total_words = (double)p.iter * (double)n_tok;
   const int D = (int)(s->dim / 64);
   dg_sgns_stats st;
   memset(&st, 0, sizeof(st));
@@ -1464,6 +1532,50 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
   DG_CHECK(dg_reserve(ctx, &s->d_sen_off, &s->sen_off_cap, (size_t)n_walks + 1));
   if (!s->d_pool_rows) DG_HIP(ctx, hipMalloc((void**)&s->d_pool_rows, sizeof(unsigned long long)));
   zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(s->d_pool_rows);
+  s->last_alphas.clear();
+  if (p.form == DG_SGNS_DSGL_FP32) {
+    // DSGL: every epoch one persistent launch of the DSGL kernel (dsgl.hip)
+    dg_dsgl_call c;
+    c.tok = d_tok;
+    c.off = d_off;
+    c.n_walks = n_walks;
+    c.n_tok = n_tok;
+    c.max_walk = 0;
+    for (uint64_t w = 0; w < n_walks; ++w)
+      c.max_walk = std::max<uint64_t>(c.max_walk, h_woff[w + 1] - h_woff[w]);
+    c.rank = s->d_rank;
+    c.keep = s->d_keep;
+    c.table = s->d_table;
+    c.table_size = s->table_size;
+    c.vocab = s->vocab;
+    c.expt = s->d_expt;
+    c.syn0 = s->d_syn0;
+    c.syn1 = s->d_syn1;
+    c.dim = s->dim;
+    c.window = p.window;
+    c.K = p.negative;
+    c.iter = p.iter;
+    c.workers = p.workers;
+    c.alpha0 = alpha0;
+    c.train_words = s->train_words;
+    c.seed = p.seed;
+    c.epoch0 = s->epoch;
+    c.exact = (int)p.exact_sigmoid;
+    c.deterministic = (int)p.deterministic;
+    s->epoch += p.iter;
+    uint64_t kept = 0;
+    DG_CHECK(dg_dsgl_train(ctx, c, &kept, &ms, &alpha));
+    dg_sgns_written(ctx);
+    DG_CHECK(dg_time_resolve(ctx));
+    st.words = (uint64_t)p.iter * n_tok;
+    st.kept = kept;
+    st.sentences = (uint64_t)p.iter * n_walks;
+    st.launches = p.iter;
+    st.ms = ms;
+    st.alpha_end = alpha;
+    if (out) *out = st;
+    return DG_OK;
+  }
   for (uint32_t ep = 0; ep < p.iter; ++ep) {
     const uint32_t epoch = s->epoch++;
     // subsample (two passes) into ranked sentences
@@ -1493,15 +1605,24 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
     a.vocab = s->vocab;
     a.window = p.window;
     a.negative = p.negative;
-    const double words_before = (double)ep * (double)n_tok;
+    a.reuse_negs = nullptr;
+    if (p.form == DG_SGNS_NEGREUSE_FP32 && p.negative > 0) {
+      DG_CHECK(dg_reserve(ctx, &s->d_reuse, &s->reuse_cap, (size_t)n_walks * p.negative));
+      reuse_negs_kernel<<<grid_for(ctx, n_walks * p.negative, 256), 256, 0, ctx->stream>>>(
+          n_walks, p.negative, dg_epoch_key(p.seed, epoch), s->d_table, s->table_size, s->vocab,
+          s->d_reuse);
+      DG_HIP(ctx, hipGetLastError());
+      a.reuse_negs = s->d_reuse;
+    }
     auto alpha_at = [&](uint64_t walk_index) -> float {
-      if (p.alpha_schedule != DG_ALPHA_LINEAR) return alpha0;
-      // dsgl.hpp:886-897
-      float al = alpha0 * (float)(1.0 - (words_before + (double)h_woff[walk_index]) / (total_words + 1.0));
-      if (al < alpha0 * 0.0001f) al = alpha0 * 0.0001f;
+      const float al = p.alpha_schedule != DG_ALPHA_LINEAR
+                           ? alpha0
+                           : linear_alpha(alpha0, (uint64_t)ep * n_tok + (h_woff[walk_index] - h_woff[0]),
+                                          (uint64_t)p.iter * n_tok + 1);
+      s->last_alphas.push_back(al);
       return al;
     };
-    if (p.form == DG_SGNS_PLAIN_FP32) {
+    if (p.form == DG_SGNS_PLAIN_FP32 || p.form == DG_SGNS_NEGREUSE_FP32) {
       // reference batches: <= 15000 sentences, closed at >= 299,980 tokens
       uint64_t sb = 0;
       while (sb < n_walks) {
@@ -1514,7 +1635,7 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
         }
         a.alpha = alpha = alpha_at(sb);
         size_t tk;
-        DG_CHECK(dg_time_begin(ctx, "sgns_plain", &tk));
+        DG_CHECK(dg_time_begin(ctx, p.form == DG_SGNS_PLAIN_FP32 ? "sgns_plain" : "sgns_reuse", &tk));
         switch (D) {
           case 1: launch_plain<1>(ctx, a, sb, se, p.deterministic, p.exact_sigmoid); break;
           case 2: launch_plain<2>(ctx, a, sb, se, p.deterministic, p.exact_sigmoid); break;
@@ -1553,7 +1674,7 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
   {
     unsigned long long pr = 0;
     DG_HIP(ctx, hipMemcpy(&pr, s->d_pool_rows, sizeof(pr), hipMemcpyDeviceToHost));
-    st.pool_rows = p.form == DG_SGNS_PLAIN_FP32 ? 0 : (uint64_t)pr;
+    st.pool_rows = (p.form == DG_SGNS_BATCHED_FP32 || p.form == DG_SGNS_BATCHED_BF16) ? (uint64_t)pr : 0;
   }
   st.words = (uint64_t)p.iter * n_tok;
   st.sentences = (uint64_t)p.iter * n_walks;
@@ -1578,9 +1699,10 @@ int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, u
     return dg_fail(ctx, DG_ERR_STATE, "tables from dg_sgns_tables_create have no vocabulary: run dg_sgns_init to train");
   DG_CHECK(check_params(ctx, pin));
   if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
-  if (pin->form == DG_SGNS_PLAIN_FP32)
+  if (pin->form != DG_SGNS_BATCHED_FP32 && pin->form != DG_SGNS_BATCHED_BF16)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED,
-                   "the captured pipeline step runs the batched forms (the plain form batches by host offsets)");
+                   "the captured pipeline step runs the batched forms (the plain forms batch by host offsets, "
+                   "the DSGL form runs one launch per epoch)");
   if (n_walks == 0) return dg_fail(ctx, DG_ERR_STATE, "empty corpus");
   auto& g = s->g;
   g.p = *pin;
@@ -1644,9 +1766,12 @@ int dg_sgns_graph_collect(dg_ctx* ctx, bool commit, double ms, dg_sgns_stats* ou
   DG_HIP(ctx, hipMemcpy(&kept, s->d_kept, sizeof(kept), hipMemcpyDeviceToHost));
   DG_HIP(ctx, hipMemcpy(&pr, s->d_pool_rows, sizeof(pr), hipMemcpyDeviceToHost));
   alpha_end = g.alpha0;  // no epoch (iter 0): nothing trained
-  if (g.p.iter > 0)
-    DG_HIP(ctx, hipMemcpy(&alpha_end, s->d_alpha + (size_t)g.p.iter * g.n_mb - 1, sizeof(float),
+  s->last_alphas.assign((size_t)g.p.iter * g.n_mb, 0.0f);
+  if (g.p.iter > 0) {
+    DG_HIP(ctx, hipMemcpy(s->last_alphas.data(), s->d_alpha, sizeof(float) * s->last_alphas.size(),
                           hipMemcpyDeviceToHost));
+    alpha_end = s->last_alphas.back();
+  }
   s->epoch = g.epoch0 + g.p.iter;
   dg_sgns_written(ctx);
   if (out) {
@@ -1679,7 +1804,7 @@ extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* p
     return dg_fail(ctx, DG_ERR_STATE, "tables from dg_sgns_tables_create have no vocabulary: run dg_sgns_init to train");
   DG_CHECK(check_params(ctx, pin));
   if (pin->dim != s->dim) return dg_fail(ctx, DG_ERR_ARG, "dim differs from dg_sgns_init");
-  if (pin->form == DG_SGNS_PLAIN_FP32)
+  if (pin->form != DG_SGNS_BATCHED_FP32 && pin->form != DG_SGNS_BATCHED_BF16)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED, "the launch shapes tuned here are those of the batched forms");
   DG_HIP(ctx, hipSetDevice(ctx->device));
   const uint32_t* d_tok = corpus == DG_CORPUS_TRAIN ? ctx->d_train_tokens : ctx->d_tokens;
@@ -1783,6 +1908,15 @@ extern "C" int dg_sgns_autotune(dg_ctx* ctx, int corpus, const dg_sgns_params* p
     words_per_s[0] = default_rate;
     words_per_s[1] = best_rate;
   }
+  return DG_OK;
+}
+
+extern "C" int dg_sgns_last_alphas(dg_ctx* ctx, float* alphas, uint32_t cap, uint32_t* n) {
+  if (!ctx) return DG_ERR_ARG;
+  if (!ctx->sgns) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
+  const std::vector<float>& v = ctx->sgns->last_alphas;
+  if (n) *n = (uint32_t)v.size();
+  if (alphas) memcpy(alphas, v.data(), sizeof(float) * std::min<size_t>(cap, v.size()));
   return DG_OK;
 }
 

@@ -20,4 +20,7 @@ from .ffi import (  # noqa: F401
     auc,
     write_edges_bin,
     read_edges_bin,
+    convert_text_edges,
+    lp_write_files,
+    lr_value,
 )

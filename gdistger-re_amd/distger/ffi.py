@@ -74,12 +74,15 @@ class RoundStats(ctypes.Structure):
 DG_SGNS_PLAIN_FP32 = 0
 DG_SGNS_BATCHED_FP32 = 1
 DG_SGNS_BATCHED_BF16 = 2
+DG_SGNS_DSGL_FP32 = 3
+DG_SGNS_NEGREUSE_FP32 = 4
 DG_VOCAB_DEGREE = 0
 DG_VOCAB_VISITS = 1
 DG_ALPHA_LINEAR = 0
 DG_ALPHA_FIXED = 1
 DG_CORPUS_ROUND = 0
 DG_CORPUS_TRAIN = 1
+DG_LR_FIXED, DG_LR_EXPONENTIAL, DG_LR_COSINE, DG_LR_STEP = 0, 1, 2, 3
 
 
 class SgnsParams(ctypes.Structure):
@@ -104,6 +107,8 @@ class SgnsParams(ctypes.Structure):
         ("block_waves", ctypes.c_uint32),
         ("pool_split", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
+        ("workers", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -202,6 +207,11 @@ def load_library():
         "dg_sgns_grad_rows": (c.c_int, [vp, c.c_int, vp, c.c_uint64, vp]),
         "dg_write_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64]),
         "dg_read_edges_bin": (c.c_int, [c.c_char_p, vp, c.c_uint64, c.POINTER(c.c_uint64)]),
+        "dg_convert_text_edges": (c.c_int, [c.c_char_p, c.c_char_p, c.c_int, c.POINTER(c.c_uint64)]),
+        "dg_lp_write_files": (c.c_int, [c.c_char_p, c.c_char_p, c.c_char_p, vp, c.c_uint64, vp,
+                                        c.c_uint64, vp, c.c_uint64]),
+        "dg_lr_value": (c.c_float, [c.c_int, c.c_float, c.c_float, c.c_float, c.c_int]),
+        "dg_sgns_last_alphas": (c.c_int, [vp, vp, c.c_uint32, c.POINTER(c.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -389,6 +399,13 @@ class Context:
         self._check(self.L.dg_pipeline_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
         return a.value, b.value, n.value
 
+    def sgns_last_alphas(self):
+        n = ctypes.c_uint32()
+        self._check(self.L.dg_sgns_last_alphas(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.float32)
+        self._check(self.L.dg_sgns_last_alphas(self.h, _vp(out), n.value, ctypes.byref(n)))
+        return out[: n.value]
+
     def sgns_vocab(self):
         n = ctypes.c_uint32()
         self._check(self.L.dg_sgns_vocab(self.h, None, ctypes.byref(n)))
@@ -542,3 +559,24 @@ def read_edges_bin(path):
     _check_host(L.dg_read_edges_bin(path.encode(), _vp(buf), n.value, ctypes.byref(n)),
                 "dg_read_edges_bin")
     return buf
+
+
+def convert_text_edges(in_path, out_path, weighted):
+    """Text edge list -> binary edge file (12-byte records if weighted, else 8)."""
+    L = load_library()
+    n = ctypes.c_uint64()
+    _check_host(L.dg_convert_text_edges(in_path.encode(), out_path.encode(), int(weighted),
+                                        ctypes.byref(n)), "dg_convert_text_edges")
+    return n.value
+
+
+def lp_write_files(train_path, test_edges_path, test_labels_path, train, pos, neg):
+    L = load_library()
+    arrs = [np.ascontiguousarray(x, dtype=EDGE_DTYPE) for x in (train, pos, neg)]
+    enc = [p.encode() if p else None for p in (train_path, test_edges_path, test_labels_path)]
+    _check_host(L.dg_lp_write_files(*enc, _vp(arrs[0]), len(arrs[0]), _vp(arrs[1]), len(arrs[1]),
+                                    _vp(arrs[2]), len(arrs[2])), "dg_lp_write_files")
+
+
+def lr_value(kind, initial, a=0.0, b=0.0, step=0):
+    return load_library().dg_lr_value(int(kind), float(initial), float(a), float(b), int(step))

@@ -150,9 +150,10 @@ int main(int argc, char** argv) {
       if ((rc = dg_corpus_write_text(ctx, path.c_str(), 1)) != DG_OK) return fail(ctx, rc, "write walks");
     }
     if (tf.per_file_alpha) {
-      // one walk file, one learning rate (src/lr_scheduler.hpp:23-31)
-      float alpha = tf.p.alpha;
-      for (uint32_t k = 1; k < iter; ++k) alpha *= 0.904837418f;  // e^{-0.1}
+      // one walk file, one learning rate: ExponentialDecayLR(alpha, 0.1), its
+      // step advanced once per file (src/lr_scheduler.hpp:23-31,
+      // src/word2vec.cu:1279-1290)
+      const float alpha = dg_lr_value(DG_LR_EXPONENTIAL, tf.p.alpha, 0.1f, 0.0f, (int)(iter - 1));
       dg_sgns_stats st;
       if ((rc = dg_sgns_train(ctx, DG_CORPUS_ROUND, &tf.p, alpha, &st)) != DG_OK)
         return fail(ctx, rc, "train");

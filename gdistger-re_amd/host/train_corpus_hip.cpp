@@ -105,7 +105,12 @@ int dg_parse_train_flags(int argc, char** argv, dg_train_flags* out, std::string
   out->binary = (int)binary;
   out->debug = (int)debug;
   out->per_file_alpha = (int)per_file;
-  if (per_file) out->p.alpha_schedule = DG_ALPHA_FIXED;
+  if (per_file) {
+    // the GPU trainer's per-file schedule starts at 0.025 whatever -alpha
+    // says (src/word2vec.cu:1279); here -alpha, when given, sets the start
+    out->p.alpha_schedule = DG_ALPHA_FIXED;
+    if (arg_pos("-alpha", argc, argv) < 0) out->p.alpha = 0.025f;
+  }
   return DG_OK;
 }
 

@@ -190,6 +190,12 @@ int dg_train_corpus_clear(dg_ctx* ctx);
 #define DG_SGNS_PLAIN_FP32 0   /* skip_gram_kernel semantics (word2vec.cu:305-399), FP32 */
 #define DG_SGNS_BATCHED_FP32 1 /* per-position shared negatives, sentence pools on FP32 MFMA */
 #define DG_SGNS_BATCHED_BF16 2 /* same with rows rounded to BF16 for BF16 MFMA, FP32 rows in HBM */
+#define DG_SGNS_DSGL_FP32 3    /* DSGL, the reference's CPU trainer (dsgl.hpp:783-1427): two sentences
+                                  in lockstep, a shared negative batch, [target, negatives with
+                                  multiplicity, target2] from one snapshot per step; one wavefront
+                                  per compute thread; DSGL's unigram table and subsampling rule */
+#define DG_SGNS_NEGREUSE_FP32 4 /* __sgNegReuse (word2vec.cu:231-303): K negatives per sentence shared
+                                   by all its positions, the full window, FP32 */
 
 #define DG_VOCAB_DEGREE 0 /* word counts = in + out degree (word2vec.cu:791-809, huge_walk.cpp:43-46) */
 #define DG_VOCAB_VISITS 1 /* word counts = walk visit counts (dsgl.hpp:314-382) */
@@ -227,6 +233,9 @@ typedef struct {
                               vertices on a 16-row pool, the rest on 32 rows; 32 = every
                               sentence on the 32-row pool (0 = 16); does not change results */
   uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
+  uint32_t workers;        /* DSGL form: compute threads the corpus is split over (0 = one per
+                              resident wavefront; dsgl.hpp:814-825 splits over its threads) */
+  uint32_t reserved;
 } dg_sgns_params;
 
 void dg_sgns_params_default(dg_sgns_params* p);
@@ -256,6 +265,14 @@ typedef struct {
  * when > 0 (the reference sets a learning rate per walk file). */
 int dg_sgns_train(dg_ctx* ctx, int corpus, const dg_sgns_params* p, float alpha,
                   dg_sgns_stats* out);
+
+/* The learning rate of every kernel launch of the last dg_sgns_train (or
+ * captured step) in launch order; *n receives the count, at most `cap` are
+ * copied.  The linear schedule is dsgl.hpp:886-897's float expression,
+ * alpha0 * (1 - words / (iter * corpus_words + 1)) clamped at alpha0 * 1e-4,
+ * evaluated at each launch's first sentence (the reference steps it every
+ * 10000 words).  Empty for the DSGL form (its rate steps inside the launch). */
+int dg_sgns_last_alphas(dg_ctx* ctx, float* alphas, uint32_t cap, uint32_t* n);
 
 /* Vocabulary: vertex id of each rank, and its size. */
 int dg_sgns_vocab(dg_ctx* ctx, uint32_t* vertex_of_rank, uint32_t* vocab_size);
@@ -403,6 +420,40 @@ double dg_auc(const float* pos_scores, uint64_t n_pos, const float* neg_scores, 
 /* Reference binary edge file I/O (include/storage.hpp:15-38, :108-115). */
 int dg_write_edges_bin(const char* path, const dg_edge* edges, uint64_t e_num);
 int dg_read_edges_bin(const char* path, dg_edge* edges, uint64_t e_cap, uint64_t* e_num);
+
+/* Text edge list -> reference binary edge file (src/tools/gconverter.cpp:
+ * 45-76): weighted != 0 reads "%u %u %f" triples and writes 12-byte records
+ * (Edge<real_t>, the input huge_walk reads); weighted == 0 reads "%u %u" pairs
+ * and writes 8-byte records (Edge<EmptyData>: src, dst).  Reading stops at the
+ * first entry that does not scan, as the reference's fscanf loop does.
+ * *e_num receives the number of edges written. */
+int dg_convert_text_edges(const char* in_path, const char* out_path, int weighted,
+                          uint64_t* e_num);
+
+/* The link-prediction files of src/tools/gtrain.cpp:345-384: the training
+ * edges as "%u %u %u\n" (source, destination, the edge data as an unsigned
+ * integer), the held-out pairs as "%u %u\n" (positives, then non-edges), and
+ * their labels as "1\n" per positive and "0\n" per non-edge.  Any path may
+ * be NULL to skip that file. */
+int dg_lp_write_files(const char* train_path, const char* test_edges_path,
+                      const char* test_labels_path, const dg_edge* train, uint64_t n_train,
+                      const dg_edge* pos, uint64_t n_pos, const dg_edge* neg, uint64_t n_neg);
+
+/* Learning-rate schedules of src/lr_scheduler.hpp:6-60, the value at a given
+ * step (the reference's get_lr() returns the value at its step counter and
+ * then advances it; the GPU trainer calls it once per walk file,
+ * src/word2vec.cu:1279-1290):
+ *   DG_LR_FIXED        initial
+ *   DG_LR_EXPONENTIAL  initial * exp(-a * step)                (float, a = decay rate)
+ *   DG_LR_COSINE       a + 0.5 (initial - a)(1 + cos(pi step / b))  (a = min lr, b = T_max)
+ *   DG_LR_STEP         initial * a^(step / b)                  (a = factor, b = step size)
+ * each in the reference's float/double mix.  Returns NaN for an unknown kind
+ * or b <= 0 where b divides. */
+#define DG_LR_FIXED 0
+#define DG_LR_EXPONENTIAL 1
+#define DG_LR_COSINE 2
+#define DG_LR_STEP 3
+float dg_lr_value(int kind, float initial, float a, float b, int step);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,8 @@ struct dg_train_flags {
   std::string emb_output;   // -emb_output / -eoutput: embedding file ("" = none)
   int binary = 0;           // -binary
   int debug = 0;            // -debug
-  int per_file_alpha = 0;   // -alpha-per-file 1: alpha_n = alpha e^{-0.1 n} per round (src/lr_scheduler.hpp:23-31)
+  int per_file_alpha = 0;   // -alpha-per-file 1: alpha_n = alpha e^{-0.1 n} per round, alpha 0.025
+                            // unless -alpha is given (src/lr_scheduler.hpp:23-31, word2vec.cu:1279)
 };
 
 // Scan argv for the trainer flags (src/word2vec.cu:1548-1566) into `out`,

@@ -24,6 +24,11 @@ void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_
                         uint32_t window, uint32_t negative, const int32_t* table,
                         uint32_t table_size, uint32_t vocab, float alpha, const float* expt,
                         uint32_t dim, float* syn0, float* syn1);
+void dgo_sg_reuse_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_sen,
+                        uint32_t window, uint32_t negative, const int32_t* negs, float alpha,
+                        const float* expt, uint32_t dim, float* syn0, float* syn1);
+void dgo_reuse_negs(uint64_t n_sen, uint32_t negative, uint64_t seed, uint32_t epoch,
+                    const int32_t* table, uint32_t table_size, uint32_t vocab, int32_t* out);
 uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,
                            uint64_t cap);
 /* Philox draws of one position: window shrink b and `draws` negatives. */
@@ -55,5 +60,32 @@ double dgo_tuple_fd(const uint32_t* tuples, uint64_t n, uint32_t K, const float*
                     double h);
 void dgo_tuple_inplace(const uint32_t* tuples, uint64_t n, uint32_t K, double lr, uint32_t dim,
                        double* syn0, double* syn1);
+
+/* DSGL (include/dsgl.hpp:783-1427), see dsgl_oracle.c for what is kept and
+ * what is chosen. */
+typedef struct {
+  const uint32_t* tokens;          /* corpus (vertex ids) */
+  const uint64_t* offsets;         /* n_walks + 1 */
+  uint64_t n_walks;
+  const uint32_t* rank_of_vertex;  /* vocabulary rank of each vertex, or 0xffffffff */
+  const uint32_t* keep;            /* per rank: dgo_dsgl_keep_limits */
+  const int32_t* table;            /* unigram table */
+  uint32_t table_size;
+  uint32_t vocab;
+  const float* expt;               /* sigmoid table (1001 entries) */
+  int exact_sigmoid;
+  uint32_t dim, window, negative, iter;
+  uint32_t workers;                /* compute threads the corpus is split over */
+  float alpha0;
+  uint64_t train_words;            /* sum of the vocabulary's counts */
+  uint64_t seed;
+  uint32_t epoch0;                 /* epoch number of the first epoch (Philox key) */
+} dgo_dsgl_args;
+void dgo_dsgl_keep_limits(const uint64_t* cn_of_rank, uint32_t vocab, float sample,
+                          uint64_t train_words, uint32_t* keep);
+void dgo_dsgl_unigram_table(const uint64_t* cn_of_rank, uint32_t vocab, uint32_t table_size,
+                            int32_t* table);
+int dgo_dsgl_train_f64(const dgo_dsgl_args* a, float* syn0, float* syn1);
+int dgo_dsgl_train_f32(const dgo_dsgl_args* a, float* syn0, float* syn1, int threads);
 
 #endif

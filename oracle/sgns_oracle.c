@@ -223,6 +223,65 @@ void dgo_sg_plain_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_
   free(neu1e);
 }
 
+/* ------------------------------------------------------------------------
+ * S6: __sgNegReuse (src/word2vec.cu:231-303): a sentence's `negative`
+ * negatives (negs[s * negative + k], drawn per sentence as the host loop at
+ * :1057-1062 does) are shared by all its positions; every position trains
+ * the full window (a from 0 to 2 window, no shrink); per (context, target)
+ * pair the same dot product, table sigmoid and updates as skip_gram_kernel,
+ * a negative equal to the centre word skipped (:263).  Sentences in order,
+ * the outputs of a pair in order (the reference runs them in parallel
+ * warps, which differs only when a sentence draws the same negative twice).
+ * ---------------------------------------------------------------------- */
+void dgo_sg_reuse_batch(const int32_t* sen, const uint64_t* sen_off, uint64_t n_sen,
+                        uint32_t window, uint32_t negative, const int32_t* negs, float alpha,
+                        const float* expt, uint32_t dim, float* syn0, float* syn1) {
+  float* neu1e = (float*)malloc(sizeof(float) * dim);
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    for (uint64_t pos = sen_off[s]; pos < sen_off[s + 1]; ++pos) {
+      int word = sen[pos];
+      for (int a = 0; a < (int)window * 2 + 1; a++) {
+        if (a == (int)window) continue;
+        int64_t c = (int64_t)pos - (int64_t)window + a;
+        if (c < (int64_t)sen_off[s] || c >= (int64_t)sen_off[s + 1]) continue;
+        float* x = syn0 + (uint64_t)sen[c] * dim;
+        for (uint32_t i = 0; i < dim; ++i) neu1e[i] = 0;
+        for (uint32_t d = 0; d < negative + 1; d++) {
+          int target = d == 0 ? word : negs[s * negative + d - 1];
+          int label = d == 0 ? 1 : 0;
+          if (d > 0 && target == word) continue;
+          float* y = syn1 + (uint64_t)target * dim;
+          float f = tree_dot(x, y, dim);
+          float g = table_sigmoid_grad(f, label, alpha, expt);
+          for (uint32_t i = 0; i < dim; ++i) {
+            neu1e[i] += g * y[i];
+            y[i] += g * x[i];
+          }
+        }
+        for (uint32_t i = 0; i < dim; ++i) x[i] += neu1e[i];
+      }
+    }
+  }
+  free(neu1e);
+}
+
+/* The negative-reuse form's negatives of sentence s of an epoch: word k of
+ * Philox(counter (k / 4, s, 0, REUSE), key dg_epoch_key(seed, epoch)) as x,
+ * table[x % table_size], 0 -> x % (vocab - 1) + 1 (word2vec.cu:1057-1062). */
+void dgo_reuse_negs(uint64_t n_sen, uint32_t negative, uint64_t seed, uint32_t epoch,
+                    const int32_t* table, uint32_t table_size, uint32_t vocab, int32_t* out) {
+  const uint64_t key = dg_epoch_key(seed, epoch);
+  for (uint64_t s = 0; s < n_sen; ++s) {
+    dg_stream st;
+    dg_stream_init(&st, key, (uint32_t)s, 0u, DG_TAG_SGNS_REUSE);
+    for (uint32_t k = 0; k < negative; ++k) {
+      uint32_t x = dg_stream_next(&st);
+      int32_t v = table[x % table_size];
+      out[s * negative + k] = v == 0 ? (int32_t)(x % (vocab - 1) + 1) : v;
+    }
+  }
+}
+
 /* Reference batch boundaries (src/word2vec.cu:999-1041): at most 15000
  * sentences, and a batch closes once it holds >= (15000 - 1) * 20 tokens. */
 uint64_t dgo_plain_batches(const uint64_t* sen_off, uint64_t n_sen, uint64_t* batch_first,

@@ -114,10 +114,11 @@ def test_c2_plain_bit_exact(ctx, c2, c2_walks):
     log(f"plain FP32 on 300 walks ({len(sen)} kept words) bit-exact")
 
 
+# measured on these walks: FP32 3.3e-7, BF16 3.4e-3 (dim 128) and 3.8e-3 (dim 256)
 @pytest.mark.parametrize("form,dim,tol", [
     (ffi.DG_SGNS_BATCHED_FP32, 128, 1e-5),
-    (ffi.DG_SGNS_BATCHED_BF16, 128, 1.5e-2),
-    (ffi.DG_SGNS_BATCHED_BF16, 256, 1.5e-2),
+    (ffi.DG_SGNS_BATCHED_BF16, 128, 1e-2),
+    (ffi.DG_SGNS_BATCHED_BF16, 256, 1e-2),
 ])
 def test_c2_batched_vs_fp64_oracle(ctx, c2, c2_walks, form, dim, tol):
     """Max error over every row the epoch touched, relative to the largest

@@ -103,3 +103,15 @@ def test_cli_end_to_end(tmp_path, c1_edges):
     assert int(head[1]) == 64 and len(rows) == int(head[0])
     vals = np.array([[float(x) for x in r[1:]] for r in rows[:50]])
     assert vals.shape == (50, 64) and np.isfinite(vals).all() and np.abs(vals).max() > 0
+
+
+def test_per_file_alpha_starts_at_the_gpu_trainers_rate():
+    """-alpha-per-file: the rate of each walk file follows ExponentialDecayLR
+    from 0.025 (word2vec.cu:1279) unless -alpha sets the start."""
+    p = json.loads(run("-g", "g.bin", "-v", "10", "-alpha-per-file", "1", "--print-params").stdout)
+    assert p["alpha_per_file"] == 1 and abs(p["alpha"] - 0.025) < 1e-9
+    p = json.loads(run("-g", "g.bin", "-v", "10", "-alpha-per-file", "1", "-alpha", "0.05",
+                       "--print-params").stdout)
+    assert abs(p["alpha"] - 0.05) < 1e-9
+    p = json.loads(run("-g", "g.bin", "-v", "10", "--print-params").stdout)
+    assert p["alpha_per_file"] == 0 and abs(p["alpha"] - 0.01) < 1e-9
