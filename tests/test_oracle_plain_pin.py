@@ -229,3 +229,64 @@ def test_sigmoid_boundaries_exactly_at_six():
         g = (F(1) - expt[996 if sign > 0 else 0]) * F(0.5)
         assert s1[1, 0] != F(1.0) or g == 0
         assert s1[1, 0] == F(F(1.0) + F(g * F(6.0 * sign)))
+
+
+def reuse_batch(sen, off, window, negs, K, alpha, expt, syn0, syn1):
+    """Hand statement of __sgNegReuse (word2vec.cu:231-303): the full window,
+    the sentence's K shared negatives, a negative equal to the centre skipped;
+    the outputs of a pair one after another from the same context row."""
+    dim = syn0.shape[1]
+    for s in range(len(off) - 1):
+        for pos in range(int(off[s]), int(off[s + 1])):
+            word = int(sen[pos])
+            for a in range(2 * window + 1):
+                if a == window:
+                    continue
+                c = pos - window + a
+                if c < off[s] or c >= off[s + 1]:
+                    continue
+                x = syn0[int(sen[c])]
+                neu1e = np.zeros(dim, np.float32)
+                for d in range(K + 1):
+                    target, label = (word, 1) if d == 0 else (int(negs[s * K + d - 1]), 0)
+                    if d > 0 and target == word:
+                        continue
+                    y = syn1[target]
+                    g = grad(tree_dot(x, y), label, alpha, expt)
+                    for i in range(dim):
+                        neu1e[i] = F(neu1e[i] + F(g * y[i]))
+                        y[i] = F(y[i] + F(g * x[i]))
+                for i in range(dim):
+                    x[i] = F(x[i] + neu1e[i])
+
+
+def test_reuse_batch_by_hand():
+    vocab, dim, window, K, alpha = 9, 64, 2, 3, F(0.05)
+    sen = np.array([4, 2, 6, 2, 8, 1, 3, 5, 5, 0, 7, 1, 2, 3], np.int32)
+    off = np.array([0, 6, 7, 14], np.uint64)
+    negs = np.array([2, 7, 7, 3, 3, 3, 1, 5, 2], np.int32)  # repeats; a centre word among them
+    expt = exp_table()
+    s0, s1 = _setup(vocab, dim, 4)
+    h0, h1 = s0.copy(), s1.copy()
+    L = dgsgns_oracle.lib()
+    vp = dgsgns_oracle._vp
+    L.dgo_sg_reuse_batch(vp(sen), vp(off), 3, window, K, vp(negs), alpha, vp(expt), dim, vp(s0), vp(s1))
+    reuse_batch(sen, off, window, negs, K, alpha, expt, h0, h1)
+    assert np.array_equal(s0, h0) and np.array_equal(s1, h1)
+
+
+def test_reuse_negatives_by_hand():
+    """x = word k of Philox(counter (k / 4, s, 0, REUSE), key seed ^ epoch << 32);
+    table[x % table_size], an entry 0 replaced by x % (vocab - 1) + 1."""
+    import dgoracle
+    table = np.array([0, 3, 0, 5, 1, 2, 0], np.int32)
+    vocab, K, seed, epoch = 8, 6, 77, 3
+    out = np.zeros(4 * K, np.int32)
+    dgsgns_oracle.lib().dgo_reuse_negs(4, K, seed, epoch, dgsgns_oracle._vp(table), len(table), vocab,
+                                       dgsgns_oracle._vp(out))
+    key = seed ^ (epoch << 32)
+    for s in range(4):
+        for k in range(K):
+            x = int(dgoracle.philox((k >> 2, s, 0, 0x52555345), (key & 0xFFFFFFFF, key >> 32))[k & 3])
+            v = int(table[x % len(table)])
+            assert out[s * K + k] == (v if v else x % (vocab - 1) + 1)
