@@ -236,6 +236,9 @@ int dg_walk_graph_plan(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
 int dg_walk_graph_enqueue(dg_ctx* ctx, bool timed);
 int dg_walk_graph_collect(dg_ctx* ctx, double walk_ms, double total_ms, dg_round_stats* out,
                           bool* retry);
+// Whether the trainer can run as the captured step with these parameters
+// (state, form, dim); checked before the step plans anything.
+int dg_sgns_graph_check(dg_ctx* ctx, const dg_sgns_params* p);
 int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* p, float alpha, uint64_t n_walks,
                        uint64_t tok_bound, uint64_t slot_tokens, uint64_t overflow_tokens);
 int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off, bool timed);
@@ -250,6 +253,7 @@ struct dg_dsgl_call {
   uint64_t n_walks, n_tok;
   uint64_t max_walk;        // longest walk of the corpus
   const uint32_t* rank;
+  uint32_t rank_size;       // entries of rank (vertex ids at or above it have no rank)
   const uint32_t* keep;     // DSGL keep limits per rank
   const int32_t* table;
   uint32_t table_size, vocab;

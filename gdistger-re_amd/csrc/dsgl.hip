@@ -100,6 +100,7 @@ struct DsglArgs {
   uint64_t n_walks;
   uint64_t words_num;      // tokens + one separator per walk
   const uint32_t* rank;
+  uint32_t rank_size;
   const uint32_t* keep;
   const int32_t* table;
   uint32_t table_size;
@@ -277,7 +278,8 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
             int r = -1;
             if ((uint64_t)lane < span) {
               const uint64_t ti = p - walk;
-              const uint32_t rr = a.rank[a.tok[ti]];
+              const uint32_t v = a.tok[ti];
+              const uint32_t rr = v < a.rank_size ? a.rank[v] : 0xffffffffu;
               if (rr != 0xffffffffu) {
                 const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)ti, (uint32_t)(ti >> 32), DG_TAG_SAMPLE,
                                                     (uint32_t)a.key, (uint32_t)(a.key >> 32));
@@ -688,6 +690,7 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   a.n_walks = c.n_walks;
   a.words_num = c.n_tok + c.n_walks;
   a.rank = c.rank;
+  a.rank_size = c.rank_size;
   a.keep = c.keep;
   a.table = c.table;
   a.table_size = c.table_size;

@@ -94,6 +94,8 @@ extern "C" int dg_pipeline_step(dg_ctx* ctx, const dg_walk_params* wp, uint32_t 
   if (!ctx || !wp || !sp) return DG_ERR_ARG;
   if (walker_num == 0) return dg_fail(ctx, DG_ERR_ARG, "walker_num must be positive");
   DG_HIP(ctx, hipSetDevice(ctx->device));
+  // the trainer's side first: a step it cannot run must not touch the round
+  DG_CHECK(dg_sgns_graph_check(ctx, sp));
   if (!ctx->pipe) {
     dg_pipe_state* ps = new dg_pipe_state();
     if (hipEventCreate(&ps->t0) != hipSuccess || hipEventCreate(&ps->t1) != hipSuccess) {

@@ -101,8 +101,9 @@ __device__ __forceinline__ float sigmoid_of(float f, const float* expt, int exac
 // ---- subsampling (word2vec.cu:1011-1020 with a Philox draw per token) ------
 // The draw of corpus token i in epoch e is word 0 of Philox(counter (0, i lo,
 // i hi, SAMPLE), key dg_epoch_key(seed, e)).
+// A token whose vertex id is not below rank_size has no rank (skipped).
 __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* offsets,
-                                       uint64_t n_walks, const uint32_t* rank,
+                                       uint64_t n_walks, const uint32_t* rank, uint32_t rank_size,
                                        const uint32_t* keep, uint64_t seed, uint32_t epoch,
                                        uint32_t* kept) {
   const uint64_t key = dg_epoch_key(seed, epoch);
@@ -110,7 +111,8 @@ __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* o
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = 0;
     for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
-      const uint32_t r = rank[tokens[i]];
+      const uint32_t v = tokens[i];
+      const uint32_t r = v < rank_size ? rank[v] : kNoRank;
       if (r == kNoRank) continue;
       const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
                                           (uint32_t)key, (uint32_t)(key >> 32));
@@ -121,7 +123,7 @@ __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* o
 }
 
 __global__ void subsample_write_kernel(const uint32_t* tokens, const uint64_t* offsets,
-                                       uint64_t n_walks, const uint32_t* rank,
+                                       uint64_t n_walks, const uint32_t* rank, uint32_t rank_size,
                                        const uint32_t* keep, uint64_t seed, uint32_t epoch,
                                        const uint64_t* sen_off, int32_t* sen) {
   const uint64_t key = dg_epoch_key(seed, epoch);
@@ -129,7 +131,8 @@ __global__ void subsample_write_kernel(const uint32_t* tokens, const uint64_t* o
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t o_pos = sen_off[w];
     for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
-      const uint32_t r = rank[tokens[i]];
+      const uint32_t v = tokens[i];
+      const uint32_t r = v < rank_size ? rank[v] : kNoRank;
       if (r == kNoRank) continue;
       const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
                                           (uint32_t)key, (uint32_t)(key >> 32));
@@ -1544,6 +1547,7 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
     for (uint64_t w = 0; w < n_walks; ++w)
       c.max_walk = std::max<uint64_t>(c.max_walk, h_woff[w + 1] - h_woff[w]);
     c.rank = s->d_rank;
+    c.rank_size = s->rank_size;
     c.keep = s->d_keep;
     c.table = s->d_table;
     c.table_size = s->table_size;
@@ -1580,7 +1584,7 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
     const uint32_t epoch = s->epoch++;
     // subsample (two passes) into ranked sentences
     subsample_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
-        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_cnt);
+        d_tok, d_off, n_walks, s->d_rank, s->rank_size, s->d_keep, p.seed, epoch, s->d_cnt);
     DG_CHECK(dg_scan_lengths(ctx, s->d_cnt, n_walks, s->d_sen_off));
     uint64_t n_kept = 0;
     DG_HIP(ctx, hipMemcpyAsync(&n_kept, s->d_sen_off + n_walks, sizeof(uint64_t),
@@ -1588,7 +1592,8 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
     DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     DG_CHECK(dg_reserve(ctx, &s->d_sen, &s->sen_cap, (size_t)n_kept + 1));
     subsample_write_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
-        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_sen_off, s->d_sen);
+        d_tok, d_off, n_walks, s->d_rank, s->rank_size, s->d_keep, p.seed, epoch, s->d_sen_off,
+        s->d_sen);
     DG_HIP(ctx, hipGetLastError());
     std::vector<uint64_t> h_soff(n_walks + 1);
     DG_HIP(ctx, hipMemcpy(h_soff.data(), s->d_sen_off, sizeof(uint64_t) * (n_walks + 1),
@@ -1691,8 +1696,7 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
 // sentence plus the walker's overflow region), and the learning rate of each
 // launch comes from alpha_kernel on the device.
 
-int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, uint64_t n_walks,
-                       uint64_t tok_bound, uint64_t slot_tokens, uint64_t overflow_tokens) {
+int dg_sgns_graph_check(dg_ctx* ctx, const dg_sgns_params* pin) {
   dg_sgns_state* s = ctx->sgns;
   if (!s) return dg_fail(ctx, DG_ERR_STATE, "dg_sgns_init has not run");
   if (!s->has_vocab)
@@ -1703,6 +1707,13 @@ int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, u
     return dg_fail(ctx, DG_ERR_UNSUPPORTED,
                    "the captured pipeline step runs the batched forms (the plain forms batch by host offsets, "
                    "the DSGL form runs one launch per epoch)");
+  return DG_OK;
+}
+
+int dg_sgns_graph_plan(dg_ctx* ctx, const dg_sgns_params* pin, float alpha_in, uint64_t n_walks,
+                       uint64_t tok_bound, uint64_t slot_tokens, uint64_t overflow_tokens) {
+  dg_sgns_state* s = ctx->sgns;
+  DG_CHECK(dg_sgns_graph_check(ctx, pin));
   if (n_walks == 0) return dg_fail(ctx, DG_ERR_STATE, "empty corpus");
   auto& g = s->g;
   g.p = *pin;
@@ -1736,10 +1747,11 @@ int dg_sgns_graph_enqueue(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_
   for (uint32_t ep = 0; ep < p.iter; ++ep) {
     const uint32_t epoch = g.epoch0 + ep;
     subsample_count_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
-        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_cnt);
+        d_tok, d_off, n_walks, s->d_rank, s->rank_size, s->d_keep, p.seed, epoch, s->d_cnt);
     DG_CHECK(dg_scan_lengths(ctx, s->d_cnt, n_walks, s->d_sen_off));
     subsample_write_kernel<<<grid_for(ctx, n_walks, 256), 256, 0, ctx->stream>>>(
-        d_tok, d_off, n_walks, s->d_rank, s->d_keep, p.seed, epoch, s->d_sen_off, s->d_sen);
+        d_tok, d_off, n_walks, s->d_rank, s->rank_size, s->d_keep, p.seed, epoch, s->d_sen_off,
+        s->d_sen);
     kept_add_kernel<<<1, 64, 0, ctx->stream>>>(s->d_sen_off, n_walks, s->d_kept);
     float* alphas = s->d_alpha + (size_t)ep * g.n_mb;
     alpha_kernel<<<(g.n_mb + 255) / 256, 256, 0, ctx->stream>>>(

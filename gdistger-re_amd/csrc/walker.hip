@@ -748,6 +748,11 @@ static int plan_round(dg_ctx* ctx, const dg_walk_params* p, uint32_t round,
   if (p->min_length + 2 > length_limit)
     return dg_fail(ctx, DG_ERR_ARG, "min_length %u needs length_limit > %u", p->min_length,
                    p->min_length + 1);
+  // the buffers below may be reallocated (their contents dropped): the
+  // previous round's corpus is given up here, so a round that fails leaves an
+  // empty corpus behind, never one whose buffers no longer hold it
+  ctx->cur_walks = 0;
+  ctx->cur_tokens = 0;
 
   // persistent grid: every resident block slot on the device, fewer for small rounds
   int per_cu = 0;

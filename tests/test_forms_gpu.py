@@ -122,8 +122,13 @@ def test_dsgl_hogwild_trains(ctx, walked):
                     ctx.sgns_score_pairs(rnd[:, 0], rnd[:, 1]))
     log(f"DSGL Hogwild: {st.ms:.2f} ms for 3 epochs, co-occurrence AUC {a:.4f}")
     assert a > 0.6
-    with pytest.raises(distger.DgError):
+    # the captured step rejects the DSGL form before it plans anything: the
+    # round corpus stays as it was
+    with pytest.raises(distger.DgError) as e:
         ctx.pipeline_step(ctx.default_walk_params(seed=1), 2, 0, 100, p)
+    assert e.value.code == -8
+    tok2, off2 = ctx.corpus()
+    assert np.array_equal(tok2, tok) and np.array_equal(off2, off)
 
 
 def test_dsgl_rejects_too_many_negatives(ctx, walked):
