@@ -106,11 +106,11 @@ def test_dsgl_table_sigmoid_close_to_oracle(ctx, walked):
 
 
 def test_dsgl_hogwild_trains(ctx, walked):
-    """Every resident wavefront a compute thread, updates racing: finite
+    """64 compute threads on as many wavefronts, updates racing: finite
     tables, walk neighbours score above random pairs."""
     tok, off, counts, _ = walked
     p = _params(ctx, form=ffi.DG_SGNS_DSGL_FP32, deterministic=0, iter=3, init_float_ops=1,
-                workers=0)
+                workers=64)
     ctx.sgns_init(p)
     st = ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
     s0, s1 = ctx.sgns_tables(128)
