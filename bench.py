@@ -241,36 +241,73 @@ def rmat25(args):
         **out}
 
 
-def cpu_baseline(ctx, walk_sample, sgns_sample, threads, counts, seed):
-    """The CPU oracle on a bounded sample of the same pipeline: walks on
-    `threads` host threads, then the plain SGNS restatement on one thread."""
+def _held_out_auc(syn0, rank_of_vertex, pos, neg):
+    """Link-prediction AUC of host tables: dot(syn0[u], syn0[v]) per pair."""
+    def scores(e):
+        out = np.zeros(len(e), np.float32)
+        for i in range(0, len(e), 1 << 20):
+            ru = rank_of_vertex[e["src"][i:i + (1 << 20)]].astype(np.int64)
+            rv = rank_of_vertex[e["dst"][i:i + (1 << 20)]].astype(np.int64)
+            out[i:i + len(ru)] = np.einsum("ij,ij->i", syn0[ru], syn0[rv])
+        return out
+    return distger.auc(scores(pos), scores(neg))
+
+
+def cpu_baseline(ctx, rnd, V, pos, neg, threads, seed, table_size):
+    """The reference's CPU path restated on the host's cores, on one full
+    round of the workload: the walker oracle walks round `rnd` (V walks, the
+    device's current round: the same corpus bit for bit) on `threads`
+    threads, then DSGL, the reference's CPU trainer (include/dsgl.hpp:783-1427,
+    restated in FP32 in oracle/dsgl_oracle.c, optimised build), trains that
+    corpus with one compute thread per host thread (Hogwild on shared
+    tables, as the reference's threads), vocabulary from the walk visit
+    counts; its embeddings are scored on the held-out edges.  The GPU's DSGL
+    form trains the same corpus from the same tables for comparison."""
     import dgoracle
     import dgsgns_oracle
 
     rp, nbr, cn = ctx.graph_csr()
-    g = dgoracle.OracleGraph.__new__(dgoracle.OracleGraph)
-    g.L = dgoracle.load("dg")
-    g.v_num = len(rp) - 1
-    g.row_ptr, g.nbr, g.cn, g.m = rp, nbr, cn, len(nbr)
-    g._g = dgoracle._Graph(g.v_num, g.m, rp.ctypes.data, nbr.ctypes.data, cn.ctypes.data)
+    g = dgoracle.OracleGraph.from_csr(rp, nbr, cn)
+    del nbr, cn
     t0 = time.time()
-    tok, off = g.walk_round(g.cfg(seed, 20, 0, 0.999), 2, 0, walk_sample, threads=threads)
+    tok, off = g.walk_round(g.cfg(seed, 20, 0, 0.999), rnd, 0, V, threads=threads, tok_cap=V * 24 + 4096)
     walk_s = time.time() - t0
+    dtok, doff = ctx.corpus()
+    same = bool(np.array_equal(dtok, tok) and np.array_equal(doff, off))
+    del dtok, doff
     walk_rate = len(tok) / walk_s
-    tr = dgsgns_oracle.Trainer(counts, 128, table_size=10_000_000)
-    sub_tok = tok[: int(off[sgns_sample])]
-    sub_off = off[: sgns_sample + 1]
-    sen, soff = tr.subsample(sub_tok, sub_off, seed, 0)
+    freq = ctx.vertex_freq().astype(np.uint64)
+    tr = dgsgns_oracle.Trainer(freq, 128, table_size=table_size, sample=1e-3, float_init=True)
+    tr.dsgl_setup(sample=1e-3, table_size=table_size)
     t0 = time.time()
-    tr.plain(sen, soff, 10, 5, 0.01)
+    tr.dsgl_f32(tok, off, 10, 5, 0.01, threads, seed, threads=threads, baseline=True)
     sgns_s = time.time() - t0
-    sgns_rate = len(sub_tok) / sgns_s
+    sgns_rate = len(tok) / sgns_s
+    cpu_auc = _held_out_auc(tr.syn0, tr.rank_of_vertex, pos, neg)
+    log(f"CPU: walk {walk_rate:.3e} vertices/s, DSGL {sgns_rate:.3e} words/s on {threads} threads, "
+        f"AUC {cpu_auc:.4f}")
+    # the GPU's DSGL form on the same round from the same initial tables
+    p = ctx.default_sgns_params(form=ffi.DG_SGNS_DSGL_FP32, dim=128, window=10, negative=5,
+                                sample=1e-3, alpha=0.01, alpha_schedule=ffi.DG_ALPHA_LINEAR,
+                                table_size=table_size, seed=seed, vocab_source=ffi.DG_VOCAB_VISITS,
+                                init_float_ops=1)
+    ctx.sgns_init(p)
+    st = ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
+    gpu_auc = auc_of(ctx, pos, neg)
+    gpu_rate = st.words / (st.ms * 1e-3)
+    log(f"GPU DSGL form on the same round: {gpu_rate:.3e} words/s, AUC {gpu_auc:.4f}")
     rate = 1.0 / (1.0 / walk_rate + 1.0 / sgns_rate)
-    return {"value": rate, "unit": "words/s", "cores": threads, "kind": "port",
-            "sample": (f"walks: {walk_sample} walks of one round on {threads} threads "
-                       f"({walk_rate:.3e} vertices/s, {walk_s:.1f} s); SGNS: plain restatement "
-                       f"on 1 thread over the first {sgns_sample} walks ({len(sub_tok)} words, "
-                       f"{sgns_rate:.3e} words/s, {sgns_s:.1f} s); pipeline rate combines both")}
+    base = {"value": rate, "unit": "words/s", "cores": threads, "kind": "port",
+            "sample": (f"one full round of the workload ({V} walks, {len(tok)} words): walker "
+                       f"oracle on {threads} threads ({walk_rate:.3e} vertices/s, {walk_s:.1f} s), "
+                       f"then DSGL (dsgl.hpp semantics, FP32, vectorised) with {threads} compute "
+                       f"threads on {threads} host threads ({sgns_rate:.3e} words/s, {sgns_s:.1f} s); "
+                       f"value = walk + train words/s of the two legs in sequence"),
+            "walk_vertices_per_s": walk_rate, "dsgl_words_per_s": sgns_rate, "auc": cpu_auc,
+            "corpus_equals_device_round": same}
+    gpu = {"words_per_s": gpu_rate, "auc": gpu_auc, "kernel_ms": st.ms, "words": st.words,
+           "speedup_vs_cpu_dsgl": gpu_rate / sgns_rate, "auc_gap_vs_cpu_dsgl": gpu_auc - cpu_auc}
+    return base, gpu
 
 
 def main():
@@ -285,8 +322,6 @@ def main():
     ap.add_argument("--table-size", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-vs-BF16 AUC run")
-    ap.add_argument("--cpu-walks", type=int, default=400_000)
-    ap.add_argument("--cpu-sgns-walks", type=int, default=20_000)
     ap.add_argument("--workload", default="chung-lu-2.2m", choices=["chung-lu-2.2m", "rmat25"],
                     help="rmat25: weighted walks and BF16 SGNS on R-MAT scale 25")
     ap.add_argument("--no-tune", action="store_true",
@@ -401,15 +436,19 @@ def main():
         pool_rows = sum(s.pool_rows for s in sstats)
         bf_bytes = sgns_sent_bytes(kept, pool_rows, args.dim, 5) / n_bf
         bf_gbs = bf_bytes / (bf_ms / n_bf * 1e-3) / 1e9
+        # the walker's fraction of HBM: the 64-B transactions it issues (one per
+        # drawn adjacency unit, one per row-offset pair read; a bounce-back
+        # append reads nothing) plus 4 B per emitted vertex, over its time; the
+        # survey's per-step figure (two transactions for every step) is kept
+        # beside it, labelled, as it overstates what the kernel reads
         out["walk"] = {"walk_steps_per_s": steps / (walk_ms * 1e-3),
                        "round_steps_per_s": steps / ((walk_ms + pack_ms + re_ms) * 1e-3),
                        "kernel_ms": walk_ms / n_walk, "pack_ms": pack_ms / n_walk,
                        "re_ms": re_ms / n_walk,
-                       "algorithmic_bytes_per_launch": walk_bytes,
-                       "achieved_gbs": walk_gbs, "frac_hbm": walk_gbs / HBM_GBS,
-                       "issued_transaction_bytes_per_launch": walk_tbytes,
-                       "issued_transaction_gbs": walk_tgbs,
-                       "issued_transaction_frac_hbm": walk_tgbs / HBM_GBS}
+                       "bytes_per_launch": walk_tbytes,
+                       "achieved_gbs": walk_tgbs, "frac_hbm": walk_tgbs / HBM_GBS,
+                       "per_step_figure_bytes_per_launch": walk_bytes,
+                       "per_step_figure_frac_hbm": walk_gbs / HBM_GBS}
         out["sgns_bf16_pipeline"] = {"words_per_s": words / ((bf_ms + prep_ms) * 1e-3),
                                      "kernel_ms_per_launch": bf_ms / n_bf, "launches": n_bf,
                                      "prep_ms_per_launch": prep_ms / n_bf,
@@ -464,8 +503,8 @@ def main():
             out["sgns_forms"]["corpus"] = f"{nw} walks, {nt} words (round 1 + {args.warmup} round(s))"
         cpu = None
         if not args.no_cpu:
-            cpu = cpu_baseline(ctx, args.cpu_walks, args.cpu_sgns_walks, min(16, os.cpu_count() or 1),
-                               counts, args.seed)
+            cpu, out["dsgl_gpu_same_corpus"] = cpu_baseline(
+                ctx, rnd - 1, V, pos, neg, min(16, os.cpu_count() or 1), args.seed, args.table_size)
         copy_gbs = ctx.copy_bench(1 << 30, 3)
         out["copy_kernel_gbs"] = copy_gbs
         if not args.no_tuples:

@@ -144,7 +144,9 @@ __global__ void subsample_write_kernel(const uint32_t* tokens, const uint64_t* o
 // ---- plain form: skip_gram_kernel (word2vec.cu:305-399), negative sampling --
 // One wavefront per sentence; lane l owns dims l + 64 j.  `single` runs the
 // whole batch in one wavefront in sentence order (the deterministic mode).
-template <int D>
+// REUSE: the negative-reuse form (a separate instance, so the plain form's
+// code is unchanged by it).
+template <int D, bool REUSE>
 __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_begin, uint64_t s_end,
                                                        int single, int exact) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -153,11 +155,11 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
   for (uint64_t s = s_begin + wave; s < s_end; s += nwaves) {
     const uint64_t sb = a.sen_off[s], se = a.sen_off[s + 1];
     unsigned long long next_random = (unsigned long long)(s - s_begin);  // blockIdx.x
-    const int32_t* rn = a.reuse_negs ? a.reuse_negs + s * a.negative : nullptr;
+    const int32_t* rn = REUSE ? a.reuse_negs + s * a.negative : nullptr;
     for (uint64_t pos = sb; pos < se; ++pos) {
       const int word = a.sen[pos];
       int b = 0;  // the negative-reuse form trains the full window (word2vec.cu:257)
-      if (!rn) {
+      if constexpr (!REUSE) {
         next_random = next_random * 2514903917ULL + 11;  // word2vec.cu:321 (sic)
         b = (int)(next_random % a.window);
       }
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256) void sg_plain_kernel(SgnsDev a, uint64_t s_beg
             target = word;
             label = 1;
           } else {
-            if (rn) {
+            if constexpr (REUSE) {
               target = rn[d - 1];  // the sentence's shared negatives (word2vec.cu:247-249)
             } else {
               next_random = next_random * 25214903917ULL + 11;
@@ -1349,15 +1351,24 @@ extern "C" int dg_sgns_tables_create(dg_ctx* ctx, uint32_t rows, uint32_t dim, c
   return DG_OK;
 }
 
+template <int D, bool REUSE>
+static void launch_plain_form(dg_ctx* ctx, const SgnsDev& a, uint64_t sb, uint64_t se, int single,
+                              int exact) {
+  if (single) {
+    sg_plain_kernel<D, REUSE><<<1, 64, 0, ctx->stream>>>(a, sb, se, 1, exact);
+  } else {
+    const unsigned blocks = (unsigned)((se - sb + 3) / 4);
+    sg_plain_kernel<D, REUSE><<<blocks ? blocks : 1, 256, 0, ctx->stream>>>(a, sb, se, 0, exact);
+  }
+}
+
 template <int D>
 static void launch_plain(dg_ctx* ctx, const SgnsDev& a, uint64_t sb, uint64_t se, int single,
                          int exact) {
-  if (single) {
-    sg_plain_kernel<D><<<1, 64, 0, ctx->stream>>>(a, sb, se, 1, exact);
-  } else {
-    const unsigned blocks = (unsigned)((se - sb + 3) / 4);
-    sg_plain_kernel<D><<<blocks ? blocks : 1, 256, 0, ctx->stream>>>(a, sb, se, 0, exact);
-  }
+  if (a.reuse_negs)
+    launch_plain_form<D, true>(ctx, a, sb, se, single, exact);
+  else
+    launch_plain_form<D, false>(ctx, a, sb, se, single, exact);
 }
 
 // Blocks resident per CU for a kernel, times the CUs.
@@ -1611,12 +1622,14 @@ static int train_impl(dg_ctx* ctx, const uint32_t* d_tok, const uint64_t* d_off,
     a.window = p.window;
     a.negative = p.negative;
     a.reuse_negs = nullptr;
-    if (p.form == DG_SGNS_NEGREUSE_FP32 && p.negative > 0) {
-      DG_CHECK(dg_reserve(ctx, &s->d_reuse, &s->reuse_cap, (size_t)n_walks * p.negative));
-      reuse_negs_kernel<<<grid_for(ctx, n_walks * p.negative, 256), 256, 0, ctx->stream>>>(
-          n_walks, p.negative, dg_epoch_key(p.seed, epoch), s->d_table, s->table_size, s->vocab,
-          s->d_reuse);
-      DG_HIP(ctx, hipGetLastError());
+    if (p.form == DG_SGNS_NEGREUSE_FP32) {
+      DG_CHECK(dg_reserve(ctx, &s->d_reuse, &s->reuse_cap, (size_t)n_walks * p.negative + 1));
+      if (p.negative > 0) {
+        reuse_negs_kernel<<<grid_for(ctx, n_walks * p.negative, 256), 256, 0, ctx->stream>>>(
+            n_walks, p.negative, dg_epoch_key(p.seed, epoch), s->d_table, s->table_size, s->vocab,
+            s->d_reuse);
+        DG_HIP(ctx, hipGetLastError());
+      }
       a.reuse_negs = s->d_reuse;
     }
     auto alpha_at = [&](uint64_t walk_index) -> float {
