@@ -46,6 +46,10 @@ extern "C" int dg_open(int device, dg_ctx** out) {
     return DG_ERR_NOMEM;
   }
   memset(ctx->h_queue, 0, sizeof(dg_walk_queue));
+  if (dg_tuples_scratch_setup(ctx) != DG_OK) {
+    dg_close(ctx);
+    return DG_ERR_HIP;
+  }
   *out = ctx;
   return DG_OK;
 }

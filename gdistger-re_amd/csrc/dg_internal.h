@@ -116,6 +116,9 @@ struct dg_ctx {
   dg_tuple_state* tuples = nullptr;
   dg_pipe_state* pipe = nullptr;
   dg_dsgl_state* dsgl = nullptr;
+  // this device's copy of the allocation-free entry point's scratch array
+  // (its address from hipGetSymbolAddress, set when the context opens)
+  float* noalloc_scratch = nullptr;
 
   // timing
   std::map<std::string, dg_timing_entry> timing;
@@ -207,6 +210,9 @@ void dg_graph_free(dg_ctx* ctx);
 void dg_walk_free(dg_ctx* ctx);
 void dg_sgns_free(dg_ctx* ctx);
 void dg_tuples_free(dg_ctx* ctx);
+// Resolve the device address of the allocation-free entry point's scratch
+// array for ctx's device (once per device; the device must be current).
+int dg_tuples_scratch_setup(dg_ctx* ctx);
 // Forget the gradients of the last tuple backward (the tables they belong to
 // are being replaced).
 void dg_tuples_invalidate(dg_ctx* ctx);

@@ -32,6 +32,7 @@
 // gather bytes), products and sums in FP32, updates to the FP32 rows.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "dg_internal.h"
@@ -904,6 +905,96 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_CHECK(dg_time_resolve(ctx));
   if (loss) *loss = h_loss;
+  return DG_OK;
+}
+
+namespace {
+
+// ---- the allocation-free forward ---------------------------------------------
+// Scratch of dg_sgns_tuples_forward_noalloc: the loss's partial sums
+// (kLossParts doubles), a range-check flag, then one loss term per tuple.
+// Each device holds its own copy; host code reaches it only through the
+// address hipGetSymbolAddress gives on that device (dg_tuples_scratch_setup).
+constexpr uint64_t kNoAllocMaxTuples = 1ull << 20;
+constexpr int kNoAllocParts = 2 * kLossParts;    // floats holding the partial sums
+constexpr int kNoAllocFlag = kNoAllocParts;      // float index of the flag
+constexpr int kNoAllocTerms = kNoAllocParts + 64;  // first loss term (256-byte aligned)
+static __device__ __align__(256) float scratch[kNoAllocTerms + kNoAllocMaxTuples];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxScratchDevices] = {};
+
+__global__ void noalloc_clear_kernel(uint32_t* flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0u;
+}
+
+__global__ void tuple_check_kernel(const uint32_t* tuples, uint64_t count, uint32_t rows, uint32_t* flag) {
+  bool bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    bad |= tuples[i] >= rows;
+  if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+}  // namespace
+
+int dg_tuples_scratch_setup(dg_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "device %d: the allocation-free scratch covers devices "
+                   "0..%d", ctx->device, kMaxScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return dg_fail(ctx, DG_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[ctx->device] = static_cast<float*>(p);
+  }
+  ctx->noalloc_scratch = g_scratch_dev[ctx->device];
+  return DG_OK;
+}
+
+extern "C" int dg_sgns_tuples_forward_noalloc(dg_ctx* ctx, const uint32_t* d_tuples, uint64_t n,
+                                              uint32_t K, float* d_logits, double* d_loss) {
+  if (!ctx || !d_tuples || !d_logits || !d_loss) return DG_ERR_ARG;
+  if (!ctx->noalloc_scratch) return dg_fail(ctx, DG_ERR_STATE, "no allocation-free scratch on this context");
+  dg_table_view tv;
+  DG_CHECK(dg_sgns_view(ctx, &tv));
+  if (K > (uint32_t)kMaxTupleNeg)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "K = %u negatives per tuple: at most %d", K, kMaxTupleNeg);
+  if (tv.dim != 64 && tv.dim != 128 && tv.dim != 256)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "tuple interface: dim %u (64, 128 or 256)", tv.dim);
+  if (n == 0 || n > kNoAllocMaxTuples)
+    return dg_fail(ctx, DG_ERR_ARG, "%llu tuples: the allocation-free forward takes 1 to %llu",
+                   (unsigned long long)n, (unsigned long long)kNoAllocMaxTuples);
+  DG_HIP(ctx, hipSetDevice(ctx->device));
+  float* base = ctx->noalloc_scratch;
+  double* parts = reinterpret_cast<double*>(base);
+  uint32_t* flag = reinterpret_cast<uint32_t*>(base + kNoAllocFlag);
+  float* terms = base + kNoAllocTerms;
+  // every row id is dereferenced by the forward: check them on the device
+  const uint64_t count = n * (K + 2);
+  noalloc_clear_kernel<<<1, 64, 0, ctx->stream>>>(flag);
+  tuple_check_kernel<<<waves_grid(ctx, count / 64 + 1), 256, 0, ctx->stream>>>(d_tuples, count, tv.rows,
+                                                                               flag);
+  DG_HIP(ctx, hipGetLastError());
+  uint32_t bad = 0;
+  DG_HIP(ctx, hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (bad) return dg_fail(ctx, DG_ERR_ARG, "a tuple row is out of range (%u rows)", tv.rows);
+  TupleArgs a{};
+  a.tuples = d_tuples;
+  a.n = n;
+  a.K = K;
+  a.xt = tv.syn0;
+  a.yt = tv.syn1;
+  a.logits = d_logits;
+  a.tloss = terms;
+  launch_fwd_form<false>(ctx, a, 0, tv.dim);
+  loss_part_kernel<<<kLossParts, 256, 0, ctx->stream>>>(terms, n, parts);
+  loss_final_kernel<<<1, 256, 0, ctx->stream>>>(parts, d_loss);
+  DG_HIP(ctx, hipGetLastError());
   return DG_OK;
 }
 

@@ -212,6 +212,8 @@ def load_library():
                                         c.c_uint64, vp, c.c_uint64]),
         "dg_lr_value": (c.c_float, [c.c_int, c.c_float, c.c_float, c.c_float, c.c_int]),
         "dg_sgns_last_alphas": (c.c_int, [vp, vp, c.c_uint32, c.POINTER(c.c_uint32)]),
+        "dg_sgns_tuples_forward_noalloc": (c.c_int, [vp, c.c_void_p, c.c_uint64, c.c_uint32,
+                                                     c.c_void_p, c.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -346,6 +346,20 @@ int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint
  * gradients stay readable through dg_sgns_grad_rows until the next backward. */
 int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
                             const float* dlogits, float lr, int apply);
+/* The forward without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  No counterpart in the reference.  The caller
+ * passes device pointers: d_tuples[n * (K + 2)] (rows of the FP32 tables),
+ * d_logits[n * (1 + K)] and d_loss (one double); the results are those of
+ * dg_sgns_tuples_forward with bf16 = 0, bit for bit.  Working memory is a
+ * fixed array in each device's memory (4 MiB, held until the process ends):
+ * n is at most 1,048,576 (DG_ERR_ARG beyond).  The row ids are checked on the
+ * device first (DG_ERR_ARG if one is out of range; this waits for the
+ * context's stream once); the forward itself is queued on the context's
+ * stream (dg_sync waits for it).  Calls on one device share that array:
+ * calls that overlap on one device, from any contexts, race. */
+int dg_sgns_tuples_forward_noalloc(dg_ctx* ctx, const uint32_t* d_tuples, uint64_t n, uint32_t K,
+                                   float* d_logits, double* d_loss);
 /* Gradient rows of the last backward: `table` 0 = syn0, 1 = syn1, rows by
  * rank; out[n_rows * dim] (rows that got no gradient read 0). */
 int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, uint64_t n_rows, float* out);
