@@ -600,7 +600,10 @@ void sg_sent_kernel(SentArgs t) {
   static_assert(KC % 4 == 0 && KC <= 16 && YCH >= 1, "shape");
   __shared__ Lds lds_all[NW];
   __shared__ float s_expt[kExpTable + 1];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // the wavefront index as a scalar: the per-wavefront LDS base in scalar
+  // registers (as a lane value it and the addresses formed from it take
+  // vector registers, and several of them spilled)
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int l16 = lane & 15, g4 = lane >> 4;
   const bool krow = l16 < KC;      // lane supplies an output row (m index) of an MFMA
   const bool kchk = 4 * g4 < KC;   // lane supplies output columns (k index) of an MFMA
