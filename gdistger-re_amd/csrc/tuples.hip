@@ -83,6 +83,17 @@ struct RowIO<uint16_t, D> {
       const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
       v[0] = __uint_as_float(q << 16);
       v[1] = __uint_as_float(q & 0xffff0000u);
+    } else if constexpr (D % 8 == 0) {
+#pragma unroll
+      for (int i = 0; i < D; i += 8) {
+        const uint4 q = *reinterpret_cast<const uint4*>(p + i);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          v[i + 2 * k] = __uint_as_float(w[k] << 16);
+          v[i + 2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+        }
+      }
     } else {
       static_assert(D % 4 == 0, "D");
 #pragma unroll
@@ -110,16 +121,17 @@ __device__ __forceinline__ void store_row(float* p, const float (&v)[D]) {
   }
 }
 
-// Sum NV per-lane values over each half-wavefront (32 lanes) at once.  Each
-// butterfly step exchanges half of the remaining values with the lane `off`
-// away, so after log2(NV) steps every lane holds one value index,
-// idx = (lane % 32) / (32 / NV), summed over the lanes that differ in the
-// exchanged bits; the remaining steps finish the sum.  Every lane of a half
-// then holds its half's full sum of value idx.
-template <int NV>
-__device__ __forceinline__ float half_multi_sum(float (&p)[NV], uint32_t lane) {
+// Sum NV per-lane values over each group of G lanes (G = 32 or 16) at once.
+// Each butterfly step exchanges half of the remaining values with the lane
+// `off` away, so after log2(NV) steps every lane holds one value index,
+// idx = (lane % G) / (G / NV), summed over the lanes that differ in the
+// exchanged bits; the remaining steps finish the sum.  Every lane of a group
+// then holds its group's full sum of value idx.
+template <int NV, int G>
+__device__ __forceinline__ float group_multi_sum(float (&p)[NV], uint32_t lane) {
+  static_assert(NV <= G, "values per group");
 #pragma unroll
-  for (int h = NV / 2, off = 16; h >= 1; h >>= 1, off >>= 1) {
+  for (int h = NV / 2, off = G / 2; h >= 1; h >>= 1, off >>= 1) {
     const bool up = (lane & (uint32_t)off) != 0;
 #pragma unroll
     for (int i = 0; i < h; ++i) {
@@ -130,7 +142,7 @@ __device__ __forceinline__ float half_multi_sum(float (&p)[NV], uint32_t lane) {
   }
   float v = p[0];
 #pragma unroll
-  for (int off = 16 / NV; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = G / 2 / NV; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
@@ -154,38 +166,42 @@ struct TupleArgs {
   uint32_t row_bits;
 };
 
-// Two tuples per wavefront, one per half (32 lanes, D = dim/32 consecutive
-// elements per lane, so a row is one 16-byte load per lane at dim 128).
-// The next pair's ids are loaded before the current pair is reduced.
+// 64 / G tuples per wavefront, one per group of G lanes (D = dim / G
+// consecutive elements per lane): G = 32 makes an FP32 row one 16-byte load
+// per lane at dim 128, G = 16 does the same for a BF16 row (four tuples per
+// wavefront, twice the rows in flight).  The next group of tuples' ids are
+// loaded before the current one is reduced.
 // BWD = false: logits and per-tuple loss.  BWD = true: dL/dz, the staged
 // context-row gradient and the sort pairs (entries [0, n) for syn0 rows,
 // [n, N) for syn1 rows, in tuple order).
-template <typename E, int DIM, int NV, bool BWD>
+template <typename E, int DIM, int NV, bool BWD, int G>
 __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
-  constexpr int D = DIM / 32;
-  constexpr int kStride = 32 / NV;          // lanes per value index after the sum
+  constexpr int T = 64 / G;                 // tuples per wavefront
+  constexpr int D = DIM / G;
+  constexpr int kStride = G / NV;           // lanes per value index after the sum
   constexpr bool kKeepY = NV * D <= 64;     // else the stage pass reloads y rows
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t hl = lane & 31u, hb = lane & 32u;  // lane within the half, half base
+  const uint32_t hl = lane & (uint32_t)(G - 1), hb = lane & ~(uint32_t)(G - 1);  // lane within the group, group base
+  const uint32_t grp = lane / G;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const E* xt = static_cast<const E*>(a.xt);
   const E* yt = static_cast<const E*>(a.yt);
   const uint32_t K1 = a.K + 1, W = a.K + 2;
   const uint32_t idx = hl / kStride;  // value index this lane ends with
-  const uint64_t pairs = (a.n + 1) / 2;
+  const uint64_t groups = (a.n + T - 1) / T;
   uint64_t pr = wave;
   uint32_t id = 0;
-  if (pr < pairs) {
-    const uint64_t t = 2 * pr + (hb ? 1 : 0);
+  if (pr < groups) {
+    const uint64_t t = (uint64_t)T * pr + grp;
     id = (t < a.n && hl < W) ? a.tuples[t * W + hl] : 0u;
   }
-  for (; pr < pairs; pr += nwaves) {
-    const uint64_t t = 2 * pr + (hb ? 1 : 0);
-    const bool tv = t < a.n;  // this half has a tuple (the last pair may be odd)
+  for (; pr < groups; pr += nwaves) {
+    const uint64_t t = (uint64_t)T * pr + grp;
+    const bool tv = t < a.n;  // this group has a tuple (the last wavefront may be short)
     const uint32_t center = __shfl(id, (int)hb, 64);
     const uint32_t ctx = __shfl(id, (int)hb + 1, 64);
-    const uint32_t my_target = __shfl(id, (int)hb + (idx == 0 ? 0 : (int)(1 + idx) & 31), 64);
+    const uint32_t my_target = __shfl(id, (int)hb + (idx == 0 ? 0 : (int)(1 + idx) & (G - 1)), 64);
     const bool my_live = tv && idx < K1 && (idx == 0 || my_target != center);
     float x[D];
     if (tv) RowIO<E, D>::load(xt + (uint64_t)ctx * DIM + hl * D, x);
@@ -208,14 +224,14 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
         }
       }
     }
-    // ids of the next pair, in flight during the sums below
+    // ids of the next group of tuples, in flight during the sums below
     const uint64_t pn = pr + nwaves;
     uint32_t id_next = 0;
-    if (pn < pairs) {
-      const uint64_t tn = 2 * pn + (hb ? 1 : 0);
+    if (pn < groups) {
+      const uint64_t tn = (uint64_t)T * pn + grp;
       id_next = (tn < a.n && hl < W) ? a.tuples[tn * W + hl] : 0u;
     }
-    const float z = half_multi_sum<NV>(p, lane);
+    const float z = group_multi_sum<NV, G>(p, lane);
     const bool writer = tv && (hl % kStride) == 0 && idx < K1;
     if constexpr (!BWD) {
       if (writer) a.logits[t * K1 + idx] = z;
@@ -830,13 +846,29 @@ int ensure_bf16(dg_ctx* ctx, dg_tuple_state* st, const dg_table_view& tv) {
   return DG_OK;
 }
 
+// Lanes per tuple: 8 for BF16 rows with K + 2 <= 8 ids per tuple (a BF16
+// row is half an FP32 row, so twice the tuples fit the same registers), 16
+// for K + 2 <= 16, otherwise 32.
 template <typename E, int DIM, bool BWD>
 void launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
+  if (sizeof(E) == 2 && DIM >= 128 && a.K + 2 <= 8) {
+    const int g = waves_grid(ctx, (a.n + 7) / 8);  // a tuple per 8 lanes
+    tuple_fwd_kernel<E, DIM, 8, BWD, 8><<<g, 256, 0, ctx->stream>>>(a);
+    return;
+  }
+  if (a.K + 2 <= 16) {
+    const int g = waves_grid(ctx, (a.n + 3) / 4);  // a tuple per 16 lanes
+    if (a.K + 1 <= 8)
+      tuple_fwd_kernel<E, DIM, 8, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
+    else
+      tuple_fwd_kernel<E, DIM, 16, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
+    return;
+  }
   const int g = waves_grid(ctx, (a.n + 1) / 2);  // a tuple per half-wavefront
   if (a.K + 1 <= 8)
-    tuple_fwd_kernel<E, DIM, 8, BWD><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fwd_kernel<E, DIM, 8, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
   else
-    tuple_fwd_kernel<E, DIM, 16, BWD><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fwd_kernel<E, DIM, 16, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
 }
 
 template <bool BWD>
