@@ -578,11 +578,12 @@ struct alignas(16) SentLds {
 // Wavefronts per SIMD the register allocation aims at.  The 16-row pools of
 // the small shapes fit three (a few values spill to scratch; the third
 // wavefront hides more of each position's dependency chain than the spills
-// cost: 18% faster on the 2.2M-vertex corpus); the larger ones two, the
-// 32-row pools one.
+// cost: 18% faster on the 2.2M-vertex corpus); the larger ones two.  The
+// 32-row BF16 pools with 8 output columns up to dim 128 fit two in 256
+// registers without spilling; the other 32-row pools one.
 template <typename E, int DIM, int P, int KC>
 constexpr int sent_waves_per_eu() {
-  if (P != 16) return 1;
+  if (P != 16) return (P == 32 && KC == 8 && sizeof(E) == 2 && DIM <= 128) ? 2 : 1;
   if (KC == 8 && (sizeof(E) == 2 ? DIM <= 128 : DIM <= 64)) return 3;
   return 2;
 }
