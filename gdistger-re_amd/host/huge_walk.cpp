@@ -59,6 +59,15 @@ bool num_flag(const char* flag, int argc, char** argv, unsigned long long def,
   return true;
 }
 
+// With -debug: one line per phase on stderr, flushed, so a run that stops
+// shows where.
+bool g_debug = false;
+void phase(const char* what) {
+  if (!g_debug) return;
+  std::fprintf(stderr, "huge_walk: %s\n", what);
+  std::fflush(stderr);
+}
+
 int fail(dg_ctx* ctx, int rc, const char* what) {
   std::fprintf(stderr, "huge_walk: %s: %s (%d)\n", what, ctx ? dg_last_error(ctx) : "", rc);
   if (ctx) dg_close(ctx);
@@ -114,12 +123,16 @@ int main(int argc, char** argv) {
     return 0;
   }
 
+  g_debug = tf.debug > 0;
+  phase("opening the device");
   dg_ctx* ctx = nullptr;
   int rc = dg_open(0, &ctx);
   if (rc != DG_OK) return fail(nullptr, rc, "dg_open");
+  phase(cn ? "loading the graph (common neighbours computed)" : "loading the graph");
   rc = dg_graph_load_bin(ctx, argv[gi + 1], (uint32_t)v_num, undirected ? 1 : 0,
                          cn ? DG_CN_COMPUTE : DG_CN_FROM_DATA);
   if (rc != DG_OK) return fail(ctx, rc, "load_graph");
+  phase("graph loaded");
 
   dg_walk_params wp;
   dg_walk_params_default(&wp, (uint32_t)v_num);
@@ -132,7 +145,10 @@ int main(int argc, char** argv) {
   wp.rounds_mode = (uint32_t)rounds_mode;
   if ((rc = dg_walk_reset(ctx)) != DG_OK || (rc = dg_train_corpus_clear(ctx)) != DG_OK)
     return fail(ctx, rc, "reset");
-  if (tf.per_file_alpha && (rc = dg_sgns_init(ctx, &tf.p)) != DG_OK) return fail(ctx, rc, "sgns_init");
+  if (tf.per_file_alpha) {
+    phase("initialising the trainer");
+    if ((rc = dg_sgns_init(ctx, &tf.p)) != DG_OK) return fail(ctx, rc, "sgns_init");
+  }
 
   for (uint32_t iter = 1; iter <= wp.max_rounds; ++iter) {
     const uint32_t n = iter == 1 ? wp.walkers_per_round * wp.init_round : wp.walkers_per_round;
@@ -147,6 +163,7 @@ int main(int argc, char** argv) {
                    (unsigned long long)rs.tokens, rs.re, rs.walk_ms);
     if (oi > 0) {
       const std::string path = std::string(argv[oi + 1]) + "-0-" + std::to_string(iter) + ".txt";
+      phase("writing the walk file");
       if ((rc = dg_corpus_write_text(ctx, path.c_str(), 1)) != DG_OK) return fail(ctx, rc, "write walks");
     }
     if (tf.per_file_alpha) {
@@ -164,6 +181,7 @@ int main(int argc, char** argv) {
   }
 
   if (tf.per_file_alpha) {
+    phase("writing the embeddings");
     if (!tf.emb_output.empty() && (rc = dg_emb_write(ctx, tf.emb_output.c_str(), tf.binary)) != DG_OK)
       return fail(ctx, rc, "write embeddings");
   } else {
@@ -175,8 +193,11 @@ int main(int argc, char** argv) {
     dg_graph_get_csr(ctx, row_ptr.data(), nullptr, nullptr);
     std::vector<uint32_t> degrees(vn);
     for (uint32_t v = 0; v < vn; ++v) degrees[v] = (uint32_t)(2 * (row_ptr[v + 1] - row_ptr[v]));
+    phase("training on the walks");
     if ((rc = train_corpus_hip(argc, argv, degrees, ctx, 0)) != DG_OK) return fail(ctx, rc, "train");
   }
+  phase("closing the device");
   dg_close(ctx);
+  phase("done");
   return 0;
 }

@@ -142,12 +142,21 @@ int train_corpus_hip(int argc, char** argv, const std::vector<uint32_t>& degrees
         return DG_ERR_ARG;
       }
   }
+  auto phase = [&](const char* what) {
+    if (fl.debug > 0) {
+      std::fprintf(stderr, "train_corpus_hip: %s\n", what);
+      std::fflush(stderr);
+    }
+  };
+  phase("building the vocabulary and tables");
   if ((rc = dg_sgns_init(ctx, &fl.p)) != DG_OK) return rc;
+  phase("training");
   dg_sgns_stats st;
   if ((rc = dg_sgns_train(ctx, DG_CORPUS_TRAIN, &fl.p, 0.0f, &st)) != DG_OK) return rc;
   if (fl.debug > 0)
     std::fprintf(stderr, "train_corpus_hip: %llu words, %llu kept, %.1f ms, alpha %.6f\n",
                  (unsigned long long)st.words, (unsigned long long)st.kept, st.ms, st.alpha_end);
+  phase("writing the embeddings");
   if (!fl.emb_output.empty() && (rc = dg_emb_write(ctx, fl.emb_output.c_str(), fl.binary)) != DG_OK)
     return rc;
   return DG_OK;

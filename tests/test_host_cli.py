@@ -61,6 +61,15 @@ def test_flag_errors(args, msg):
     assert r.returncode != 0 and msg in r.stderr
 
 
+def test_debug_prints_each_phase_before_it_runs(tmp_path):
+    """-debug 1: a phase line on stderr before each step, so a run that stops shows where.
+    The graph file is missing: without a GPU the run stops at dg_open, with one at the load."""
+    r = run("-g", str(tmp_path / "missing.bin"), "-v", "10", "-debug", "1")
+    assert r.returncode != 0
+    assert r.stderr.startswith("huge_walk: opening the device\n")
+    assert "huge_walk: dg_open" in r.stderr or "huge_walk: load_graph" in r.stderr
+
+
 def test_graph_and_vertex_count_required():
     assert run("-v", "10", "--print-params").returncode != 0
     assert run("-g", "g.bin", "--print-params").returncode != 0
@@ -79,10 +88,12 @@ def test_cli_end_to_end(tmp_path, c1_edges):
     out = str(tmp_path / "w")
     emb = str(tmp_path / "emb.txt")
     print("[cli-gpu] huge_walk on the 10,000-vertex Chung-Lu graph", flush=True)
-    r = run("-g", g, "-v", "10000", "-o", out, "--make-undirected", "-cn", "1", "-max-rounds", "3",
-            "-max-length", "80", "-size", "64", "-form", "2", "-emb_output", emb, "-seed", "4",
-            timeout=300)
-    assert r.returncode == 0, r.stderr
+    # -debug: the child's phase lines go straight to this process's stderr
+    r = subprocess.run([CLI, "-g", g, "-v", "10000", "-o", out, "--make-undirected", "-cn", "1",
+                        "-max-rounds", "3", "-max-length", "80", "-size", "64", "-form", "2",
+                        "-emb_output", emb, "-seed", "4", "-debug", "1"],
+                       stdout=subprocess.PIPE, stderr=None, text=True, timeout=300)
+    assert r.returncode == 0
     with distger.Context(0) as c:
         c.graph_from_edges(10000, c1_edges, undirected=True, cn_mode=ffi.DG_CN_COMPUTE)
         wp = c.default_walk_params(seed=4, max_length=80)
