@@ -14,6 +14,9 @@
 // -max-rounds, -rounds-mode (0: 30 rounds as walk.hpp:740-744, 1: also stop
 // at |delta RE| <= 0.001), -cn (0: common-neighbour counts from the edge
 // data as the reference reads them, 1: computed on the device).
+// -trainer dsgl trains with the CPU trainer's entry point instead
+// (dsgl_hip, include/dsgl.hpp:1464): its flags and defaults (-threads for
+// the compute threads, -eoutput), the device's DSGL form.
 // --print-params prints the parsed parameters and exits without a GPU.
 #include <cstdio>
 #include <cstdlib>
@@ -22,6 +25,7 @@
 #include <vector>
 
 #include "../../include/distger_hip.h"
+#include "../../include/dsgl_hip.h"
 #include "../../include/train_corpus_hip.h"
 
 namespace {
@@ -79,7 +83,7 @@ int fail(dg_ctx* ctx, int rc, const char* what) {
 int main(int argc, char** argv) {
   if (argc < 2 || has_flag("-h", argc, argv) || has_flag("--help", argc, argv)) {
     std::printf("usage: huge_walk -g graph.bin -v V [-w walkers] [-o walk_prefix] [--min_L 20] "
-                "[--min_R 5] [--make-undirected] [-size 128 -window 10 -negative 5 ...] "
+                "[--min_R 5] [--make-undirected] [-trainer gpu|dsgl] [-size 128 -window 10 -negative 5 ...] "
                 "[-emb_output emb.txt]\n");
     return argc < 2 ? 1 : 0;
   }
@@ -98,10 +102,24 @@ int main(int argc, char** argv) {
     return 1;
   }
   const bool undirected = has_flag("--make-undirected", argc, argv);
+  // -trainer gpu (train_corpus_cuda's flags, src/word2vec.cu:1548-1566) or
+  // dsgl (the CPU trainer's flags and defaults, include/dsgl.hpp:1496-1535)
+  const int ti = arg_pos("-trainer", argc, argv);
+  if (ti == -2 || (ti > 0 && std::strcmp(argv[ti + 1], "gpu") && std::strcmp(argv[ti + 1], "dsgl"))) {
+    std::fprintf(stderr, "huge_walk: -trainer takes gpu or dsgl\n");
+    return 1;
+  }
+  const bool use_dsgl = ti > 0 && !std::strcmp(argv[ti + 1], "dsgl");
   dg_train_flags tf;
+  dg_dsgl_flags df;
   std::string err;
-  if (dg_parse_train_flags(argc, argv, &tf, &err) != DG_OK) {
+  if (dg_parse_train_flags(argc, argv, &tf, &err) != DG_OK ||
+      (use_dsgl && dg_parse_dsgl_flags(argc, argv, &df, &err) != DG_OK)) {
     std::fprintf(stderr, "huge_walk: %s\n", err.c_str());
+    return 1;
+  }
+  if (use_dsgl && tf.per_file_alpha) {
+    std::fprintf(stderr, "huge_walk: -alpha-per-file is the GPU trainer's schedule (-trainer gpu)\n");
     return 1;
   }
   if (gi < 0 || v_num == 0) {
@@ -109,6 +127,16 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (walkers == 0) walkers = v_num;
+  if (has_flag("--print-params", argc, argv) && use_dsgl) {
+    std::printf("{\"trainer\": \"dsgl\", \"graph\": \"%s\", \"v_num\": %llu, \"dim\": %u, \"window\": %u, "
+                "\"negative\": %u, \"min_count\": %u, \"iter\": %u, \"batch_size\": %u, "
+                "\"sample\": %g, \"alpha\": %g, \"threads\": %u, \"form\": %u, \"vocab_source\": %u, "
+                "\"eoutput\": \"%s\", \"binary\": %d, \"debug\": %d, \"ignored\": %zu}\n",
+                argv[gi + 1], v_num, df.p.dim, df.p.window, df.p.negative, df.p.min_count, df.p.iter,
+                df.p.batch_size, (double)df.p.sample, (double)df.p.alpha, df.p.workers, df.p.form,
+                df.p.vocab_source, df.emb_output.c_str(), df.binary, df.debug, df.ignored.size());
+    return 0;
+  }
   if (has_flag("--print-params", argc, argv)) {
     std::printf("{\"graph\": \"%s\", \"v_num\": %llu, \"walkers\": %llu, \"min_L\": %llu, \"min_R\": %llu, "
                 "\"undirected\": %d, \"seed\": %llu, \"max_length\": %llu, \"max_rounds\": %llu, "
@@ -194,7 +222,11 @@ int main(int argc, char** argv) {
     std::vector<uint32_t> degrees(vn);
     for (uint32_t v = 0; v < vn; ++v) degrees[v] = (uint32_t)(2 * (row_ptr[v + 1] - row_ptr[v]));
     phase("training on the walks");
-    if ((rc = train_corpus_hip(argc, argv, degrees, ctx, 0)) != DG_OK) return fail(ctx, rc, "train");
+    if (use_dsgl) {
+      if ((rc = dsgl_hip(argc, argv, nullptr, ctx)) != DG_OK) return fail(ctx, rc, "train");
+    } else if ((rc = train_corpus_hip(argc, argv, degrees, ctx, 0)) != DG_OK) {
+      return fail(ctx, rc, "train");
+    }
   }
   phase("closing the device");
   dg_close(ctx);

@@ -70,6 +70,37 @@ def test_debug_prints_each_phase_before_it_runs(tmp_path):
     assert "huge_walk: dg_open" in r.stderr or "huge_walk: load_graph" in r.stderr
 
 
+def test_dsgl_trainer_flags_and_defaults():
+    """-trainer dsgl: DSGL's flags with its defaults (dsgl.hpp:67-75; the size 128 instead of
+    100, the device form's row widths being 64, 128 and 256), batch size forced to
+    2 * window + 1 (dsgl.hpp:832), the DSGL form on the visit-count vocabulary; the
+    multi-process and vocabulary-file flags are accepted and counted."""
+    p = json.loads(run("-g", "g.bin", "-v", "10", "-trainer", "dsgl", "--print-params").stdout)
+    assert p["trainer"] == "dsgl" and p["form"] == ffi.DG_SGNS_DSGL_FP32
+    assert p["vocab_source"] == ffi.DG_VOCAB_VISITS
+    assert (p["dim"], p["window"], p["negative"], p["min_count"], p["iter"], p["threads"]) == \
+        (128, 5, 5, 5, 5, 12)
+    assert abs(p["alpha"] - 0.1) < 1e-9 and abs(p["sample"] - 1e-3) < 1e-12
+    assert p["batch_size"] == 11 and p["debug"] == 2 and p["binary"] == 0 and p["ignored"] == 0
+    p = json.loads(run("-g", "g.bin", "-v", "10", "-trainer", "dsgl", "-size", "64", "-window", "7",
+                       "-threads", "8", "-batch-size", "3", "-eoutput", "e.txt", "-binary", "1",
+                       "-sync-period", "0.5", "-message-size", "64", "--print-params").stdout)
+    assert (p["dim"], p["window"], p["threads"], p["batch_size"]) == (64, 7, 8, 15)
+    assert p["eoutput"] == "e.txt" and p["binary"] == 1 and p["ignored"] == 2
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-trainer", "cpu"], "-trainer takes gpu or dsgl"),
+    (["-trainer"], "-trainer takes gpu or dsgl"),
+    (["-trainer", "dsgl", "-threads", "0"], "-threads must be at least 1"),
+    (["-trainer", "dsgl", "-save-vocab"], "Argument missing for -save-vocab"),
+    (["-trainer", "dsgl", "-alpha-per-file", "1"], "the GPU trainer's schedule"),
+])
+def test_dsgl_trainer_errors(args, msg):
+    r = run("-g", "g.bin", "-v", "10", "--print-params", *args)  # a flag last has no value
+    assert r.returncode != 0 and msg in r.stderr
+
+
 def test_graph_and_vertex_count_required():
     assert run("-v", "10", "--print-params").returncode != 0
     assert run("-g", "g.bin", "--print-params").returncode != 0
