@@ -12,22 +12,25 @@
 // negative equal to the centre is skipped (src/word2vec.cu:366), here too.
 //
 // Pipeline of a backward call (n tuples, N = n (2 + K) gradient entries):
-//   1. tuple_fwd_kernel<bwd>: one wavefront per tuple gathers x and the 1+K
-//      y rows (D = dim/64 consecutive elements per lane), reduces the 1+K
-//      dots together (a butterfly that halves the value set at each step),
-//      and writes dL/dz, the context row's gradient sum_j dz_j y_j (a staged
-//      row per tuple) and N (key, entry) pairs: key = table bit | row.
-//   2. stable LSD radix sort of the pairs by key, 8 bits per pass: entries
-//      of one row become a segment, in tuple order.
+//   1. tuple_fwd_kernel<bwd>: a group of lanes per tuple gathers x and the
+//      1+K y rows, reduces the 1+K dots together (a butterfly that halves
+//      the value set at each step), and writes dL/dz, the context row's
+//      gradient sum_j dz_j y_j (a staged row per tuple) and N (key, entry)
+//      pairs: key = table bit | row.  It also clears the sort's counters.
+//   2. stable LSD radix sort of the pairs by key, 8 bits per pass: one
+//      histogram launch, then one launch per pass whose tiles find their
+//      offsets by look-back.  Entries of one row become a segment, in tuple
+//      order.
 //   3. tuple_seg_kernel: the sorted entries are cut into blocks of 64; one
 //      wavefront sums each block's runs of equal keys in order.  A run that
-//      starts and ends inside its block is a finished row gradient; a run
-//      that crosses a block edge leaves its partial sum, and
-//      tuple_fix_kernel adds a crossing row's partials in block order.
-//      Hot rows (Zipf heads) are thus split over many wavefronts with a
-//      fixed summation order.
-//   4. tuple_apply_kernel (optional): W -= lr * grad per row, after every
-//      gradient is read from the old rows; the BF16 shadow rows follow.
+//      starts and ends inside its block is a finished row gradient, stored
+//      at the run's first sorted position; a run that crosses a block edge
+//      leaves its partial sum, and tuple_fix_kernel adds a crossing row's
+//      partials in block order.  Hot rows (Zipf heads) are thus split over
+//      many wavefronts with a fixed summation order.
+//   4. apply (optional, in tuple_fix_kernel): W -= lr * grad per row, after
+//      every gradient input was read from the old rows; the BF16 shadow rows
+//      follow.
 // BF16 form: x and y are read from BF16 copies of the tables (half the
 // gather bytes), products and sums in FP32, updates to the FP32 rows.
 #include <algorithm>
@@ -164,6 +167,8 @@ struct TupleArgs {
   uint32_t* keys;          // backward: N
   uint32_t* vals;          // backward: N
   uint32_t row_bits;
+  uint32_t* clear;         // backward: words the sort's launches expect zero
+  uint64_t n_clear;
 };
 
 // 64 / G tuples per wavefront, one per group of G lanes (D = dim / G
@@ -190,6 +195,11 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
   const uint32_t K1 = a.K + 1, W = a.K + 2;
   const uint32_t idx = hl / kStride;  // value index this lane ends with
   const uint64_t groups = (a.n + T - 1) / T;
+  if constexpr (BWD) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n_clear;
+         i += (uint64_t)gridDim.x * blockDim.x)
+      a.clear[i] = 0u;
+  }
   uint64_t pr = wave;
   uint32_t id = 0;
   if (pr < groups) {
@@ -317,134 +327,262 @@ __global__ __launch_bounds__(256) void loss_final_kernel(const double* parts, do
   if (threadIdx.x == 0) *out = t;
 }
 
-// ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass -----------
-constexpr int kSortBlock = 256;
-constexpr int kSortItems = 4;
-constexpr int kSortTile = kSortBlock * kSortItems;
+// ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass ------------
+// One histogram launch counts the digits of every pass; each pass is then one
+// launch.  Tiles take consecutive ids from a ticket counter in the order they
+// start and rank their keys by digit.  A tile then publishes, per digit, its
+// count in its tile word (flag | 30-bit count, one 32-bit word) and adds it to
+// the word of its group of kGroup tiles (tiles added << 32 | count sum, one
+// 64-bit atomic), so relaxed device-scope accesses order both.  Its prefix
+// over earlier tiles is the sum of the earlier groups' words (each read once
+// all its tiles have added) and of the earlier tiles of its own group: a few
+// independent loads instead of a chain through every tile, and a tile only
+// waits for tiles that started before it, so every wait ends.  Each wait is
+// bounded by a poll limit and each store by the output size: a limit hit
+// sets the error word the host reads after the call.
+constexpr int kSortThreads = 1024;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortIpt = 4;                          // keys per lane per tile
+constexpr int kSortTile = kSortThreads * kSortIpt;   // 4,096 keys
+constexpr int kGroup = 16;                           // tiles per group word
+constexpr int kHistChunk = 2048;                     // keys per histogram workgroup
+constexpr int kMaxPasses = 4;
+constexpr uint32_t kCountMask = (1u << 30) - 1u;
+constexpr uint32_t kFlagReady = 1u << 30;
+constexpr uint32_t kSpinLimit = 1u << 22;            // polls of one word (under a second)
 
-// Per-tile digit counts, stored digit-major so that one exclusive scan gives
-// every (digit, tile) its output offset.
-__global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t* keys, uint64_t n,
-                                                                int shift, uint32_t n_tiles,
-                                                                uint32_t* hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  for (int i = 0; i < kSortItems; ++i) {
-    const uint64_t k = base + (uint64_t)i * kSortBlock + threadIdx.x;
-    if (k < n) atomicAdd(&h[(keys[k] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+__device__ __forceinline__ uint32_t lb_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_add(uint64_t* p, uint64_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Per digit: exclusive scan of its per-tile counts (contiguous in the
-// digit-major histogram) into offsets within the digit, and the digit total.
-// One workgroup per digit.
-__global__ __launch_bounds__(256) void radix_digit_scan_kernel(const uint32_t* hist, uint32_t n_tiles,
-                                                               uint32_t* tile_off, uint32_t* digit_total) {
-  __shared__ uint32_t sc[256];
-  const uint32_t d = blockIdx.x;
-  const uint32_t* h = hist + (uint64_t)d * n_tiles;
-  uint32_t* o = tile_off + (uint64_t)d * n_tiles;
-  uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < n_tiles; c0 += 256) {
-    const uint32_t k = c0 + threadIdx.x;
-    const uint32_t v = k < n_tiles ? h[k] : 0u;
-    sc[threadIdx.x] = v;
-    __syncthreads();
-    for (int st = 1; st < 256; st <<= 1) {
-      const uint32_t add = threadIdx.x >= (unsigned)st ? sc[threadIdx.x - st] : 0u;
-      __syncthreads();
-      sc[threadIdx.x] += add;
-      __syncthreads();
+// Look-back words of one column (a digit, or the single count of the segment
+// numbering): tile words tw[t * stride] and group words gw[g * stride].
+struct LbColumn {
+  uint32_t* tw;
+  uint64_t* gw;
+  uint32_t stride;
+};
+
+// Publish a tile's count in its tile word and its group's word.
+__device__ __forceinline__ void lb_publish(const LbColumn& c, uint32_t tile, uint32_t count) {
+  lb_store(c.tw + (uint64_t)tile * c.stride, kFlagReady | count);
+  lb_add(c.gw + (uint64_t)(tile / kGroup) * c.stride, (1ull << 32) | count);
+}
+
+// Part of a tile's exclusive prefix: items j = first, first + step, ... of
+// the list [earlier groups..., earlier tiles of the own group...], each read
+// once ready (all loads of a round in flight together).
+template <int R>
+__device__ uint32_t lb_prefix_part(const LbColumn& c, uint32_t tile, uint32_t first, uint32_t step,
+                                   uint32_t* err) {
+  const uint32_t g = tile / kGroup, t0 = g * kGroup;
+  const uint32_t m = g + (tile - t0);
+  uint32_t part = 0;
+  for (uint32_t j0 = first; j0 < m; j0 += step * R) {
+    uint64_t w[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const uint32_t j = j0 + step * u;
+      w[u] = j >= m ? 0ull : j < g ? lb_load(c.gw + (uint64_t)j * c.stride)
+                                   : (uint64_t)lb_load(c.tw + (uint64_t)(t0 + j - g) * c.stride);
     }
-    if (k < n_tiles) o[k] = carry + sc[threadIdx.x] - v;
-    carry += sc[255];
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const uint32_t j = j0 + step * u;
+      if (j >= m) continue;
+      uint32_t polls = 0;
+      while (j < g ? (w[u] >> 32) != (uint64_t)kGroup : (w[u] & kFlagReady) == 0ull) {
+        if (++polls > kSpinLimit) {
+          *err = 1u;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w[u] = j < g ? lb_load(c.gw + (uint64_t)j * c.stride)
+                     : (uint64_t)lb_load(c.tw + (uint64_t)(t0 + j - g) * c.stride);
+      }
+      part += j < g ? (uint32_t)w[u] : (uint32_t)w[u] & kCountMask;
+    }
   }
-  if (threadIdx.x == 0) digit_total[d] = carry;
+  return part;
 }
 
-// Scatter a tile in input order: within a wave, lanes with the same digit are
-// ranked by a ballot match on the digit bits; waves and rounds are ordered
-// through LDS counters, so equal keys keep their input order.  The digit
-// bases are the exclusive scan of the 256 digit totals, done per workgroup.
-__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
-    const uint32_t* keys, const uint32_t* vals, uint64_t n, int shift, uint32_t n_tiles,
-    const uint32_t* tile_off, const uint32_t* digit_total, uint32_t* okeys, uint32_t* ovals) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t wcnt[kSortBlock / 64][256];
+// Exclusive scan of one value per thread over the workgroup (at most 16
+// wavefronts).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t tot = digit_total[threadIdx.x];
-  base[threadIdx.x] = tot;
-  run[threadIdx.x] = 0;
-  for (int w = 0; w < kSortBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
-  __syncthreads();
-  for (int st = 1; st < 256; st <<= 1) {
-    const uint32_t add = threadIdx.x >= (unsigned)st ? base[threadIdx.x - st] : 0u;
-    __syncthreads();
-    base[threadIdx.x] += add;
-    __syncthreads();
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += u;
   }
-  // inclusive -> exclusive, plus this tile's offset within the digit
-  const uint32_t my_base = base[threadIdx.x] - tot + tile_off[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+  if (lane == 63) wsum[wave] = inc;
   __syncthreads();
-  base[threadIdx.x] = my_base;
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += wsum[w];
   __syncthreads();
-  const uint64_t tile0 = (uint64_t)blockIdx.x * kSortTile;
+  return before + inc - v;
+}
+
+// Digit counts of every pass over the whole input, in kHistCopies copies
+// ghist[copy][pass][digit] (workgroup w adds to copy w % kHistCopies, which
+// spreads the adds to the hot digits of skewed keys), cleared by the launch
+// that wrote the keys: counted per workgroup in LDS, then added.
+constexpr int kHistCopies = 16;
+
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t* keys, uint64_t n, int passes,
+                                                         uint32_t* ghist) {
+  __shared__ uint32_t h[kMaxPasses][256];
+  for (int p = 0; p < kMaxPasses; ++p) h[p][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * kHistChunk;
+#pragma unroll
+  for (int i = 0; i < kHistChunk / 256; ++i) {
+    const uint64_t k = lo + (uint64_t)i * 256 + threadIdx.x;
+    if (k < n) {
+      const uint32_t key = keys[k];
+      for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(key >> (8 * p)) & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t* g = ghist + (uint64_t)(blockIdx.x % kHistCopies) * kMaxPasses * 256;
+  for (int p = 0; p < passes; ++p)
+    if (h[p][threadIdx.x]) atomicAdd(&g[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+}
+
+struct SortPass {
+  const uint32_t* keys;
+  const uint32_t* vals;
+  uint64_t n;
+  int pass;
+  const uint32_t* ghist;  // [copy][pass][digit] over the whole input
+  uint32_t* tile_words;   // this pass's [tile][digit]
+  uint64_t* group_words;  // this pass's [group][digit]
+  uint32_t* ticket;       // null: tile = workgroup id (every tile is resident at once)
+  uint32_t* err;
+  uint32_t* okeys;
+  uint32_t* ovals;
+};
+
+// One pass: wavefront w of a tile ranks keys [w * 256, (w + 1) * 256) of the
+// tile, 64 at a time in order; lanes with the same digit are ranked by a
+// ballot match and the wavefront's running digit counts live in LDS (one
+// wavefront reads and writes its own counters, in order).  Per digit: the
+// offsets of the wavefronts within the tile, the tile's count, its prefix
+// over earlier tiles (four threads per digit) and the digit's global base.
+// The tile is then laid out in LDS in output order, so that each digit's
+// keys leave as one contiguous run.
+__global__ __launch_bounds__(kSortThreads) void radix_pass_kernel(SortPass a) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t wcnt[kSortWaves][256];
+  __shared__ uint32_t dbase[256], lbase[256];
+  __shared__ uint32_t wsum[kSortWaves];
+  __shared__ uint32_t skey[kSortTile], sval[kSortTile];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_tile = a.ticket ? atomicAdd(a.ticket, 1u) : blockIdx.x;
+  // keys with digit d in the whole input, summed below (threads d < 256)
+  uint32_t gv[kHistCopies];
+  if (threadIdx.x < 256u) {
+#pragma unroll
+    for (int r = 0; r < kHistCopies; ++r) gv[r] = a.ghist[((uint64_t)r * kMaxPasses + a.pass) * 256 + threadIdx.x];
+  }
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(kSortWaves * 256); i += kSortThreads) (&wcnt[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int shift = 8 * a.pass;
+  const uint64_t tbase = (uint64_t)tile * kSortTile;
+  const uint32_t tlen = (uint32_t)(a.n - tbase < (uint64_t)kSortTile ? a.n - tbase : kSortTile);
+  const uint32_t wofs = wave * 64 * kSortIpt;  // this wavefront's first key in the tile
+  uint32_t kk[kSortIpt], vv[kSortIpt], pos[kSortIpt];
+#pragma unroll
+  for (int i = 0; i < kSortIpt; ++i) {
+    const uint32_t k = wofs + i * 64 + lane;
+    kk[i] = k < tlen ? a.keys[tbase + k] : 0u;
+    vv[i] = k < tlen ? a.vals[tbase + k] : 0u;
+  }
   const uint64_t lanes_lt = (1ull << lane) - 1ull;
-  for (int i = 0; i < kSortItems; ++i) {
-    const uint64_t k = tile0 + (uint64_t)i * kSortBlock + threadIdx.x;
-    const bool valid = k < n;
-    const uint32_t key = valid ? keys[k] : 0u;
-    const uint32_t dig = (key >> shift) & 255u;
+#pragma unroll
+  for (int i = 0; i < kSortIpt; ++i) {
+    const bool valid = wofs + i * 64 + lane < tlen;
+    const uint32_t dig = (kk[i] >> shift) & 255u;
     uint64_t same = __ballot(valid);
+#pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
+      const uint64_t bal = __ballot(((dig >> b) & 1u) != 0u);
       same &= ((dig >> b) & 1u) ? bal : ~bal;
     }
+    const uint32_t before = wcnt[wave][dig];
     const uint32_t rank = (uint32_t)__popcll(same & lanes_lt);
-    const uint32_t cnt = (uint32_t)__popcll(same);
-    if (valid && rank == 0) wcnt[wave][dig] = cnt;
-    __syncthreads();
-    if (valid) {
-      uint32_t before = run[dig];
-      for (uint32_t w = 0; w < wave; ++w) before += wcnt[w][dig];
-      const uint64_t dst = (uint64_t)base[dig] + before + rank;
-      okeys[dst] = key;
-      ovals[dst] = vals[k];
-    }
-    __syncthreads();
-    uint32_t add = 0;
-    for (int w = 0; w < kSortBlock / 64; ++w) add += wcnt[w][threadIdx.x];
-    run[threadIdx.x] += add;
-    for (int w = 0; w < kSortBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
+    if (valid && rank == 0u) wcnt[wave][dig] = before + (uint32_t)__popcll(same);
+    pos[i] = (dig << 16) | (before + rank);
   }
-}
-
-// Row segments (runs of equal sorted keys) starting in each block of
-// kSegBlock sorted entries; their exclusive scan numbers the segments.
-__global__ __launch_bounds__(256) void seg_count_kernel(const uint32_t* keys, uint64_t N, uint32_t* bcnt) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
-  for (uint64_t b = wave; b < n_blocks; b += nwaves) {
-    const uint64_t i = b * kSegBlock + lane;
-    const bool head = i < N && (i == 0 || keys[i] != keys[i - 1]);
-    const uint64_t heads = __ballot(head);
-    if (lane == 0) bcnt[b] = (uint32_t)__popcll(heads);
+  __syncthreads();
+  // threads d < 256: the wavefront offsets and the count of digit d, published
+  uint32_t gcount = 0, tcount = 0;
+  if (threadIdx.x < 256u) {
+    const uint32_t d = threadIdx.x;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+      const uint32_t c = wcnt[w][d];
+      wcnt[w][d] = tcount;
+      tcount += c;
+    }
+    lb_publish(LbColumn{a.tile_words + d, a.group_words + d, 256}, tile, tcount);
+#pragma unroll
+    for (int r = 0; r < kHistCopies; ++r) gcount += gv[r];
+  }
+  // global base and offset within the tile of each digit: scans over the digits
+  const uint32_t gbase = block_excl_scan(gcount, wsum);
+  const uint32_t tb = block_excl_scan(tcount, wsum);
+  if (threadIdx.x < 256u) {
+    dbase[threadIdx.x] = gbase;
+    lbase[threadIdx.x] = tb;
+  }
+  __syncthreads();
+  // the tile in output order
+#pragma unroll
+  for (int i = 0; i < kSortIpt; ++i) {
+    if (wofs + i * 64 + lane < tlen) {
+      const uint32_t dig = pos[i] >> 16;
+      const uint32_t slot = lbase[dig] + wcnt[wave][dig] + (pos[i] & 0xffffu);
+      skey[slot] = kk[i];
+      sval[slot] = vv[i];
+    }
+  }
+  // prefix of digit d over earlier tiles: threads 4d .. 4d + 3
+  {
+    const uint32_t d = threadIdx.x >> 2, q = threadIdx.x & 3u;
+    uint32_t part = lb_prefix_part<4>(LbColumn{a.tile_words + d, a.group_words + d, 256}, tile, q, 4, a.err);
+    part += __shfl_xor(part, 1, 64);
+    part += __shfl_xor(part, 2, 64);
+    if (q == 0) dbase[d] += part;
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < tlen; j += kSortThreads) {
+    const uint32_t key = skey[j];
+    const uint32_t dig = (key >> shift) & 255u;
+    const uint64_t dst = (uint64_t)dbase[dig] + (j - lbase[dig]);
+    if (dst < a.n) {
+      a.okeys[dst] = key;
+      a.ovals[dst] = sval[j];
+    } else {
+      *a.err = 2u;
+    }
   }
 }
 
 struct SegArgs {
   const uint32_t* keys;   // sorted, N
   const uint32_t* vals;   // entry ids in sorted order
-  const uint64_t* boff;   // per block: segments starting before it (n_blocks + 1)
   uint64_t N;
   uint64_t n;             // tuples (entries [0, n) are syn0 entries)
   uint32_t K;
@@ -453,12 +591,16 @@ struct SegArgs {
   const float* dz;
   const float* stage;
   const void* xt;         // syn0 rows for the syn1 entries (float or BF16)
-  uint32_t* seg_key;      // per segment: its key
-  uint32_t* seg_first;    // per segment: sorted position of its first entry
-  float* grad;            // per segment: the row gradient
+  float* grad;            // at the sorted position of each segment's first entry: its row gradient
   float* carry;           // per block: sum of its last run when that run continues
   float* headp;           // per block: sum of its first run when that run started earlier
   uint32_t* fin;          // per block: 1 if headp finishes a row
+  // apply (W -= lr * grad), or syn0 == null for gradients only
+  float lr;
+  float* syn0;
+  float* syn1;
+  uint16_t* bf0;          // BF16 shadow rows that follow the FP32 rows, or null
+  uint16_t* bf1;
 };
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t l) {
@@ -469,6 +611,75 @@ __device__ __forceinline__ float lane_f32(float v, uint32_t l) {
 }
 
 constexpr int kRowsInFlight = 8;
+
+// W[row of key] -= lr * g for R rows at a time: keys and gradient
+// positions (sorted entry index) come from lanes of `key` and `hp` selected
+// by `mask` (one wavefront, D = dim / 64 elements per lane).
+template <int DIM, int R>
+__device__ void apply_rows(const SegArgs& a, uint64_t mask, uint32_t key, uint32_t hp) {
+  constexpr int D = DIM / 64;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t rmask = (1u << a.row_bits) - 1u;
+  while (mask) {
+    uint32_t e[R];
+    int m = 0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      e[u] = 64u;
+      if (mask) {
+        e[u] = (uint32_t)__builtin_ctzll(mask);
+        mask &= mask - 1ull;
+        m = u + 1;
+      }
+    }
+    float g[R][D], w[R][D];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if (u < m) {
+        const uint32_t k = lane_u32(key, e[u]);
+        float* W = ((k >> a.row_bits) ? a.syn1 : a.syn0) + (uint64_t)(k & rmask) * DIM + lane * D;
+        RowIO<float, D>::load(a.grad + (uint64_t)lane_u32(hp, e[u]) * DIM + lane * D, g[u]);
+        RowIO<float, D>::load(W, w[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if (u < m) {
+        const uint32_t k = lane_u32(key, e[u]);
+        const bool t1 = (k >> a.row_bits) != 0u;
+        const uint64_t off = (uint64_t)(k & rmask) * DIM + lane * D;
+#pragma unroll
+        for (int q = 0; q < D; ++q) w[u][q] = w[u][q] - a.lr * g[u][q];
+        store_row<D>((t1 ? a.syn1 : a.syn0) + off, w[u]);
+        uint16_t* sh = t1 ? a.bf1 : a.bf0;
+        if (sh) {
+#pragma unroll
+          for (int q = 0; q < D; ++q) sh[off + q] = to_bf16(w[u][q]);
+        }
+      }
+    }
+  }
+}
+
+// W[row of key k] -= lr * g, the gradient g held by the wavefront.
+template <int DIM>
+__device__ __forceinline__ void apply_one(const SegArgs& a, uint32_t k, const float (&g)[DIM / 64]) {
+  constexpr int D = DIM / 64;
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool t1 = (k >> a.row_bits) != 0u;
+  const uint64_t off = (uint64_t)(k & ((1u << a.row_bits) - 1u)) * DIM + lane * D;
+  float* W = (t1 ? a.syn1 : a.syn0) + off;
+  float w[D];
+  RowIO<float, D>::load(W, w);
+#pragma unroll
+  for (int q = 0; q < D; ++q) w[q] = w[q] - a.lr * g[q];
+  store_row<D>(W, w);
+  uint16_t* sh = t1 ? a.bf1 : a.bf0;
+  if (sh) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) sh[off + q] = to_bf16(w[q]);
+  }
+}
 
 // Wave-uniform state of one block's walk over its sorted entries.
 struct SegWalk {
@@ -482,12 +693,13 @@ struct SegWalk {
 // Sum entries [lo, hi) of the block, all reading rows of element type T at
 // `rows + src_e`; kRowsInFlight row loads are issued before they are added,
 // in entry order.  A run end stores the run's sum: as a finished row
-// gradient, as the block's carry (the run continues in the next block) or
-// as the block's head part (the run started in an earlier block).
+// gradient (at the run's first entry), as the block's carry (the run
+// continues in the next block) or as the block's head part (the run started
+// in an earlier block).
 template <typename T, int DIM>
 __device__ __forceinline__ void seg_run(const SegArgs& a, SegWalk& w, const T* rows, uint32_t lo,
                                         uint32_t hi, uint32_t lane, uint32_t key, uint32_t src,
-                                        float scale, uint32_t ord, float (&acc)[DIM / 64]) {
+                                        float scale, float (&acc)[DIM / 64]) {
   constexpr int D = DIM / 64;
   for (uint32_t e0 = lo; e0 < hi; e0 += kRowsInFlight) {
     float v[kRowsInFlight][D];
@@ -510,7 +722,7 @@ __device__ __forceinline__ void seg_run(const SegArgs& a, SegWalk& w, const T* r
           const bool continued = w.piece_start == 0 && w.base > 0 && w.pkey0 == ke;
           const bool open = e + 1 == w.nv && w.after == ke;
           if (!continued && !open) {
-            store_row<D>(a.grad + (uint64_t)lane_u32(ord, e) * DIM + lane * D, acc);
+            store_row<D>(a.grad + (w.base + w.piece_start) * DIM + lane * D, acc);
           } else if (open) {
             store_row<D>(a.carry + w.b * DIM + lane * D, acc);
           } else {
@@ -551,20 +763,6 @@ __global__ __launch_bounds__(256) void tuple_seg_kernel(SegArgs a) {
     const uint32_t nkey = (i + 1 < a.N) ? a.keys[i + 1] : 0xffffffffu;
     w.pkey0 = w.base > 0 ? a.keys[w.base - 1] : 0xffffffffu;
     w.after = w.base + w.nv < a.N ? a.keys[w.base + w.nv] : 0xffffffffu;
-    const uint32_t up_key = __shfl_up(key, 1, 64);  // every lane takes part
-    const uint32_t prev_key = lane == 0 ? w.pkey0 : up_key;
-    const bool head = valid && (i == 0 || prev_key != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t lanes_lt = (1ull << lane) - 1ull;
-    const uint64_t bo = a.boff[b];
-    if (head) {
-      const uint64_t o = bo + (uint64_t)__popcll(heads & lanes_lt);
-      a.seg_key[o] = key;
-      a.seg_first[o] = (uint32_t)i;
-    }
-    // ordinal of this entry's segment: heads up to and including this lane,
-    // after the bo segments that started in earlier blocks
-    const uint32_t ord = valid ? (uint32_t)(bo + (uint64_t)__popcll(heads & (lanes_lt | (1ull << lane))) - 1) : 0u;
     const bool is0 = (key >> a.row_bits) == 0u;
     uint32_t src = 0;  // row index into the stage (syn0 entries) or into syn0 (syn1 entries)
     float scale = 0.0f;
@@ -585,42 +783,154 @@ __global__ __launch_bounds__(256) void tuple_seg_kernel(SegArgs a) {
     float acc[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    seg_run<float, DIM>(a, w, a.stage, 0, n0, lane, key, src, scale, ord, acc);
-    seg_run<E, DIM>(a, w, xt, n0, w.nv, lane, key, src, scale, ord, acc);
+    seg_run<float, DIM>(a, w, a.stage, 0, n0, lane, key, src, scale, acc);
+    seg_run<E, DIM>(a, w, xt, n0, w.nv, lane, key, src, scale, acc);
     if (lane == 0) a.fin[b] = w.fin_here ? 1u : 0u;
   }
 }
 
-// A row whose entries cross block edges: the carries of blocks b0 .. b-1
-// (b0 = the block of the row's first entry) and the head part of block b.
-// Sixteen wavefronts sum fixed contiguous chunks of the carries, in order;
-// the chunk sums and the head part are then added in chunk order.
+// Rows whose entries cross block edges, and (with apply) the updates.
+// Workgroup g covers blocks 8 g .. 8 g + 7: wavefront w < 8 takes the first
+// task below for block 8 g + w, wavefront 8 + w the second, so that the two
+// chains of dependent loads run side by side.
+//  * When block b's first run started in an earlier block and ends in b
+//    (fin): the run's first entry is found from the first keys of the
+//    blocks before b, its sum is the carries of blocks b0 .. b-1 (b0 = the
+//    block of the first entry) plus b's head part, added in block order, and
+//    stored at the first entry.  A chain of at most kShortChain carries the
+//    wavefront sums alone, all loads in flight at once; a longer one (the
+//    hot rows) is summed afterwards by the workgroup's sixteen wavefronts,
+//    each a fixed contiguous part of the carries, the parts then added in
+//    order.  With apply the row is then updated.
+//  * With apply, the rows whose entries all lie in block b are updated from
+//    their stored gradients (every read of the old rows ended with the
+//    segmented-sum launch).
 constexpr int kFixWaves = 16;
+constexpr int kShortChain = 16;
+
+// Block b's first key k continues a run from earlier blocks: the run's first
+// entry (its sorted position) and the block holding it.  The blocks whose
+// first key is k are contiguous and end at b, so the nearest block c < b
+// whose first key differs is found by probing 64 blocks at a time at strides
+// 1, 64, 4096, ... until a probe differs, then refining at stride / 64.
+__device__ void run_start(const SegArgs& a, uint64_t b, uint32_t k, uint64_t& first, uint64_t& b0) {
+  const uint32_t lane = threadIdx.x & 63u;
+  int64_t hi = (int64_t)b;  // a block whose first key is k; c < hi
+  int64_t stride = 1;
+  int64_t c;
+  for (;;) {
+    const int64_t cb = hi - stride * (int64_t)(lane + 1);
+    const bool differs = cb < 0 || a.keys[(uint64_t)cb * kSegBlock] != k;
+    const uint64_t m = __ballot(differs);
+    if (m) {
+      const int64_t j = (int64_t)__builtin_ctzll(m);
+      if (stride == 1) {
+        c = hi - (j + 1);
+        break;
+      }
+      hi -= stride * j;  // probe j - 1 (or hi itself) starts with k; c >= hi - stride
+      stride /= 64;
+    } else {
+      hi -= stride * 64;
+      stride *= 64;
+    }
+  }
+  if (c < 0) {
+    first = 0;
+    b0 = 0;
+    return;
+  }
+  // the run's entries in block c are a suffix of it (possibly empty)
+  const uint64_t m = __ballot(a.keys[(uint64_t)c * kSegBlock + lane] == k);
+  if (m) {
+    first = (uint64_t)c * kSegBlock + (uint32_t)__builtin_ctzll(m);
+    b0 = (uint64_t)c;
+  } else {
+    first = (uint64_t)(c + 1) * kSegBlock;
+    b0 = (uint64_t)(c + 1);
+  }
+}
 
 template <int DIM>
 __global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
   constexpr int D = DIM / 64;
   __shared__ float part[kFixWaves][DIM];
+  __shared__ uint32_t s_long;
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint64_t n_blocks = (a.N + kSegBlock - 1) / kSegBlock;
-  for (uint64_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
-    if (!a.fin[b]) continue;  // uniform over the workgroup
-    // block b's first entry continues a segment that started earlier: the
-    // last one counted before block b
-    const uint64_t ord = a.boff[b] - 1;
-    const uint64_t b0 = a.seg_first[ord] / kSegBlock;
-    const uint64_t cnt = b - b0;
+  if (threadIdx.x == 0) s_long = 0u;
+  __syncthreads();
+  constexpr uint32_t kHalf = kFixWaves / 2;
+  const uint64_t b = (uint64_t)blockIdx.x * kHalf + (wv % kHalf);
+  const bool completes = wv < kHalf;
+  if (completes && b < n_blocks && a.fin[b]) {
+    const uint32_t k = a.keys[b * kSegBlock];
+    uint64_t first, b0;
+    run_start(a, b, k, first, b0);
+    const uint32_t cnt = (uint32_t)(b - b0);
+    if (cnt > (uint32_t)kShortChain) {
+      if (lane == 0) atomicOr(&s_long, 1u << wv);
+    } else {
+      float v[kShortChain][D];
+#pragma unroll
+      for (int u = 0; u < kShortChain; ++u)
+        if ((uint32_t)u < cnt) RowIO<float, D>::load(a.carry + (b0 + u) * DIM + lane * D, v[u]);
+      float h[D];
+      RowIO<float, D>::load(a.headp + b * DIM + lane * D, h);
+      float s[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) s[q] = 0.0f;
+#pragma unroll
+      for (int u = 0; u < kShortChain; ++u)
+        if ((uint32_t)u < cnt) {
+#pragma unroll
+          for (int q = 0; q < D; ++q) s[q] += v[u][q];
+        }
+#pragma unroll
+      for (int q = 0; q < D; ++q) s[q] += h[q];
+      store_row<D>(a.grad + first * DIM + lane * D, s);
+      if (a.syn0) apply_one<DIM>(a, k, s);
+    }
+  }
+  // rows whose entries all lie in block b
+  if (!completes && a.syn0 && b < n_blocks) {
+    const uint64_t base = b * kSegBlock;
+    const uint32_t nv = (uint32_t)(a.N - base < (uint64_t)kSegBlock ? a.N - base : kSegBlock);
+    const uint64_t i = base + lane;
+    const bool valid = lane < nv;
+    const uint32_t key = valid ? a.keys[i] : 0xffffffffu;
+    const uint32_t nkey = i + 1 < a.N ? a.keys[i + 1] : 0xffffffffu;
+    const uint32_t pkey = i > 0 ? a.keys[i - 1] : 0xffffffffu;
+    const bool head = valid && (i == 0 || pkey != key);
+    const uint64_t heads = __ballot(head);
+    // a run that ends here (not at the block's last entry when it continues)
+    const bool end_here = valid && key != nkey;
+    // started in this block: a head at or before the end, after the previous end
+    const uint64_t all_ends = __ballot(valid && (key != nkey || lane + 1 == nv));
+    const uint64_t prev_ends = all_ends & ((1ull << lane) - 1ull);
+    const uint32_t ps = prev_ends ? 64u - (uint32_t)__builtin_clzll(prev_ends) : 0u;  // run's first entry in block
+    const bool started = ps > 0 || ((heads & 1ull) != 0ull);
+    const uint64_t mine = __ballot(end_here && started);
+    if (mine) apply_rows<DIM, 16>(a, mine, key, (uint32_t)(base + ps));
+  }
+  __syncthreads();
+  for (uint32_t lm = s_long; lm != 0u; lm &= lm - 1u) {  // uniform over the workgroup
+    const uint64_t bl = (uint64_t)blockIdx.x * kHalf + (uint32_t)__builtin_ctz(lm);
+    const uint32_t k = a.keys[bl * kSegBlock];
+    uint64_t first, b0;
+    run_start(a, bl, k, first, b0);
+    const uint64_t cnt = bl - b0;
     const uint64_t lo = b0 + cnt * wv / kFixWaves, hi = b0 + cnt * (wv + 1) / kFixWaves;
     float acc[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] = 0.0f;
-    for (uint64_t c0 = lo; c0 < hi; c0 += kRowsInFlight) {
-      float v[kRowsInFlight][D];
+    for (uint64_t c0 = lo; c0 < hi; c0 += kShortChain) {
+      float v[kShortChain][D];
 #pragma unroll
-      for (int u = 0; u < kRowsInFlight; ++u)
+      for (int u = 0; u < kShortChain; ++u)
         if (c0 + u < hi) RowIO<float, D>::load(a.carry + (c0 + u) * DIM + lane * D, v[u]);
 #pragma unroll
-      for (int u = 0; u < kRowsInFlight; ++u)
+      for (int u = 0; u < kShortChain; ++u)
         if (c0 + u < hi) {
 #pragma unroll
           for (int q = 0; q < D; ++q) acc[q] += v[u][q];
@@ -630,73 +940,43 @@ __global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
     for (int q = 0; q < D; ++q) part[wv][lane * D + q] = acc[q];
     __syncthreads();
     if (wv == 0) {
-      float tot[D];
-      RowIO<float, D>::load(a.headp + b * DIM + lane * D, tot);
       float s[D];
 #pragma unroll
       for (int q = 0; q < D; ++q) s[q] = 0.0f;
-      for (int k = 0; k < kFixWaves; ++k) {
+      for (int kk = 0; kk < kFixWaves; ++kk) {
 #pragma unroll
-        for (int q = 0; q < D; ++q) s[q] += part[k][lane * D + q];
+        for (int q = 0; q < D; ++q) s[q] += part[kk][lane * D + q];
       }
+      float h[D];
+      RowIO<float, D>::load(a.headp + bl * DIM + lane * D, h);
 #pragma unroll
-      for (int q = 0; q < D; ++q) s[q] += tot[q];
-      store_row<D>(a.grad + ord * DIM + lane * D, s);
+      for (int q = 0; q < D; ++q) s[q] += h[q];
+      store_row<D>(a.grad + first * DIM + lane * D, s);
+      if (a.syn0) apply_one<DIM>(a, k, s);
     }
     __syncthreads();
   }
 }
 
-// W -= lr * grad for every segment; the BF16 shadow rows follow the new
-// FP32 values.  One wavefront per segment.
-template <int DIM>
-__global__ __launch_bounds__(256) void tuple_apply_kernel(const uint32_t* seg_key, const float* grad,
-                                                          const uint64_t* n_seg_p, uint32_t row_bits,
-                                                          float lr, float* syn0, float* syn1,
-                                                          uint16_t* bf0, uint16_t* bf1) {
-  constexpr int D = DIM / 64;
+// Gradient rows by (table, row): binary search over the sorted keys for the
+// row's first entry, where its gradient is stored.
+__global__ void grad_rows_kernel(const uint32_t* keys, const float* grad, uint64_t N, uint32_t row_bits,
+                                 uint32_t table, const uint32_t* rows, uint64_t n_rows, uint32_t dim,
+                                 float* out) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t n_seg = *n_seg_p;
-  for (uint64_t sg = wave; sg < n_seg; sg += nwaves) {
-    const uint32_t key = seg_key[sg];
-    const bool t1 = (key >> row_bits) != 0u;
-    const uint64_t row = key & ((1u << row_bits) - 1u);
-    float* w = (t1 ? syn1 : syn0) + row * DIM + lane * D;
-    float g[D], cur[D];
-    RowIO<float, D>::load(grad + sg * DIM + lane * D, g);
-    RowIO<float, D>::load(w, cur);
-#pragma unroll
-    for (int q = 0; q < D; ++q) cur[q] = cur[q] - lr * g[q];
-    store_row<D>(w, cur);
-    uint16_t* sh = t1 ? bf1 : bf0;
-    if (sh) {
-#pragma unroll
-      for (int q = 0; q < D; ++q) sh[row * DIM + lane * D + q] = to_bf16(cur[q]);
-    }
-  }
-}
-
-// Gradient rows by (table, row): binary search over the sorted segment keys.
-__global__ void grad_rows_kernel(const uint32_t* seg_key, const float* grad, const uint64_t* n_seg_p,
-                                 uint32_t row_bits, uint32_t table, const uint32_t* rows,
-                                 uint64_t n_rows, uint32_t dim, float* out) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t n_seg = *n_seg_p;
   for (uint64_t r = wave; r < n_rows; r += nwaves) {
     const uint32_t want = (table << row_bits) | rows[r];
-    uint64_t lo = 0, hi = n_seg;
+    uint64_t lo = 0, hi = N;
     while (lo < hi) {
       const uint64_t mid = lo + (hi - lo) / 2;
-      if (seg_key[mid] < want)
+      if (keys[mid] < want)
         lo = mid + 1;
       else
         hi = mid;
     }
-    const bool hit = lo < n_seg && seg_key[lo] == want;
+    const bool hit = lo < N && keys[lo] == want;
     for (uint32_t d = lane; d < dim; d += 64)
       out[r * dim + d] = hit ? grad[lo * dim + d] : 0.0f;
   }
@@ -741,20 +1021,10 @@ struct dg_tuple_state {
   uint32_t* d_v[2] = {nullptr, nullptr};
   size_t k_cap[2] = {0, 0};
   size_t v_cap[2] = {0, 0};
-  uint32_t* d_hist = nullptr;
-  size_t hist_cap = 0;
-  uint32_t* d_hoff = nullptr;   // per (digit, tile): offset within the digit
-  size_t hoff_cap = 0;
-  uint32_t* d_dtot = nullptr;   // per digit: total count
-  size_t dtot_cap = 0;
-  uint32_t* d_head = nullptr;
-  size_t head_cap = 0;
-  uint64_t* d_off = nullptr;
-  size_t off_cap = 0;
-  uint32_t* d_seg_key = nullptr;
-  size_t seg_key_cap = 0;
-  uint32_t* d_seg_first = nullptr;
-  size_t seg_first_cap = 0;
+  uint32_t* d_lb = nullptr;     // tickets, then the look-back words of the sort and segment launches
+  size_t lb_cap = 0;
+  uint32_t* h_err = nullptr;    // host-visible: set by a kernel whose look-back wait or store bound failed
+  uint32_t* d_err = nullptr;    // the device's address of h_err
   double* d_loss_parts = nullptr;
   float* d_grad = nullptr;
   size_t grad_cap = 0;
@@ -775,19 +1045,19 @@ struct dg_tuple_state {
   uint32_t grad_dim = 0;
   uint32_t grad_rows = 0;
   uint64_t grad_N = 0;
-  uint64_t grad_blocks = 0;  // segment count is d_off[grad_blocks]
   int sorted = 0;  // which of d_k / d_v holds the sorted pairs
 };
 
 void dg_tuples_free(dg_ctx* ctx) {
   dg_tuple_state* t = ctx->tuples;
   if (!t) return;
-  void* bufs[] = {t->d_tuples, t->d_logits, t->d_tloss, t->d_loss,  t->d_dl,   t->d_dz,
-                  t->d_stage,  t->d_k[0],   t->d_k[1],  t->d_v[0],  t->d_v[1], t->d_hist,
-                  t->d_hoff,   t->d_dtot,   t->d_head,   t->d_off,   t->d_seg_key, t->d_seg_first, t->d_loss_parts, t->d_grad, t->d_carry,
-                  t->d_headp,  t->d_fin,    t->d_bf0,   t->d_bf1};
+  void* bufs[] = {t->d_tuples, t->d_logits, t->d_tloss,    t->d_loss,       t->d_dl,    t->d_dz,
+                  t->d_stage,  t->d_k[0],   t->d_k[1],     t->d_v[0],       t->d_v[1],
+                  t->d_lb,     t->d_loss_parts,
+                  t->d_grad,   t->d_carry,  t->d_headp,    t->d_fin,        t->d_bf0,   t->d_bf1};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (t->h_err) hipHostFree(t->h_err);
   delete t;
   ctx->tuples = nullptr;
 }
@@ -1043,6 +1313,11 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   const uint32_t K1 = K + 1, dim = tv.dim;
   const uint64_t N = n * (K + 2);
   const int rb = row_bits_for(tv.rows);
+  if (N > (uint64_t)kCountMask)  // the sort's look-back words count in 30 bits
+    return dg_fail(ctx, DG_ERR_ARG, "%llu gradient entries: at most 2^30 - 1 per backward",
+                   (unsigned long long)N);
+  if ((rb + 1 + 7) / 8 > kMaxPasses)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "%u rows: keys wider than 32 bits", tv.rows);
   DG_CHECK(dg_reserve(ctx, &st->d_dz, &st->dz_cap, (size_t)(n * K1)));
   DG_CHECK(dg_reserve(ctx, &st->d_stage, &st->stage_cap, (size_t)(n * dim)));
   for (int b = 0; b < 2; ++b) {
@@ -1054,6 +1329,29 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
     DG_HIP(ctx, hipMemcpyAsync(st->d_dl, dlogits, sizeof(float) * n * K1, hipMemcpyHostToDevice,
                                ctx->stream));
   }
+  // the sort's sizes and the words it expects zero (cleared by the backward kernel)
+  if (!st->h_err) {
+    DG_HIP(ctx, hipHostMalloc((void**)&st->h_err, sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+    DG_HIP(ctx, hipHostGetDevicePointer((void**)&st->d_err, st->h_err, 0));
+  }
+  *st->h_err = 0u;  // the previous call's kernels have finished
+  const int passes = (rb + 1 + 7) / 8;
+  const uint64_t n_tiles = (N + kSortTile - 1) / kSortTile;
+  const uint64_t n_groups = (n_tiles + kGroup - 1) / kGroup;
+  const uint32_t n_hist = (uint32_t)((N + kHistChunk - 1) / kHistChunk);  // histogram workgroups
+  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
+  // words cleared by the backward kernel, in 32-bit units (64-bit group
+  // words at even offsets): the digit counts [copy][pass][digit], kMaxPasses
+  // tickets, then per pass the [tile][digit] and [group][digit] look-back
+  // words.  Tiles are numbered by workgroup id when every tile is resident
+  // at once (one 1,024-thread workgroup per CU), by ticket otherwise.
+  auto even = [](uint64_t x) { return (x + 1) & ~1ull; };
+  const uint64_t o_tick = (uint64_t)kHistCopies * kMaxPasses * 256;
+  const uint64_t o_sort = even(o_tick + kMaxPasses);
+  const uint64_t pass_words = even(n_tiles * 256) + 2 * n_groups * 256;
+  const uint64_t n_lb = o_sort + (uint64_t)passes * pass_words;
+  const bool tickets = n_tiles > (uint64_t)ctx->num_cus;
+  DG_CHECK(dg_reserve(ctx, &st->d_lb, &st->lb_cap, (size_t)n_lb));
   TupleArgs a{};
   a.tuples = st->d_tuples;
   a.n = n;
@@ -1066,38 +1364,37 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   a.keys = st->d_k[0];
   a.vals = st->d_v[0];
   a.row_bits = (uint32_t)rb;
+  a.clear = st->d_lb;
+  a.n_clear = n_lb;
   size_t tk;
   DG_CHECK(dg_time_begin(ctx, "tuple_prep", &tk));
   launch_fwd_form<true>(ctx, a, bf16, dim);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   // sort (key, entry) pairs on rb + 1 key bits
-  const uint32_t n_tiles = (uint32_t)((N + kSortTile - 1) / kSortTile);
-  DG_CHECK(dg_reserve(ctx, &st->d_hist, &st->hist_cap, (size_t)n_tiles * 256));
-  DG_CHECK(dg_reserve(ctx, &st->d_hoff, &st->hoff_cap, (size_t)n_tiles * 256));
-  DG_CHECK(dg_reserve(ctx, &st->d_dtot, &st->dtot_cap, (size_t)256));
-  DG_CHECK(dg_reserve(ctx, &st->d_head, &st->head_cap, (size_t)(N / kSegBlock + 1)));
-  DG_CHECK(dg_reserve(ctx, &st->d_off, &st->off_cap, (size_t)(N / kSegBlock + 2)));
   DG_CHECK(dg_time_begin(ctx, "tuple_sort", &tk));
+  radix_hist_kernel<<<n_hist, 256, 0, ctx->stream>>>(st->d_k[0], N, passes, st->d_lb);
   int cur = 0;
-  for (int shift = 0; shift < rb + 1; shift += 8) {
-    radix_hist_kernel<<<n_tiles, kSortBlock, 0, ctx->stream>>>(st->d_k[cur], N, shift, n_tiles,
-                                                               st->d_hist);
-    radix_digit_scan_kernel<<<256, 256, 0, ctx->stream>>>(st->d_hist, n_tiles, st->d_hoff, st->d_dtot);
-    radix_scatter_kernel<<<n_tiles, kSortBlock, 0, ctx->stream>>>(
-        st->d_k[cur], st->d_v[cur], N, shift, n_tiles, st->d_hoff, st->d_dtot, st->d_k[1 - cur],
-        st->d_v[1 - cur]);
-    DG_HIP(ctx, hipGetLastError());
+  for (int p = 0; p < passes; ++p) {
+    uint32_t* pw = st->d_lb + o_sort + (uint64_t)p * pass_words;
+    SortPass sp{};
+    sp.keys = st->d_k[cur];
+    sp.vals = st->d_v[cur];
+    sp.n = N;
+    sp.pass = p;
+    sp.ghist = st->d_lb;
+    sp.tile_words = pw;
+    sp.group_words = reinterpret_cast<uint64_t*>(pw + even(n_tiles * 256));
+    sp.ticket = tickets ? st->d_lb + o_tick + p : nullptr;
+    sp.err = st->d_err;
+    sp.okeys = st->d_k[1 - cur];
+    sp.ovals = st->d_v[1 - cur];
+    radix_pass_kernel<<<(unsigned)n_tiles, kSortThreads, 0, ctx->stream>>>(sp);
     cur = 1 - cur;
   }
-  const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
-  seg_count_kernel<<<waves_grid(ctx, n_blocks), 256, 0, ctx->stream>>>(st->d_k[cur], N, st->d_head);
-  DG_CHECK(dg_scan_lengths(ctx, st->d_head, n_blocks, st->d_off));
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
-  // segmented sums
-  DG_CHECK(dg_reserve(ctx, &st->d_seg_key, &st->seg_key_cap, (size_t)N));
-  DG_CHECK(dg_reserve(ctx, &st->d_seg_first, &st->seg_first_cap, (size_t)N));
+  // segmented sums, then the rows that cross blocks (and, with apply, the updates)
   DG_CHECK(dg_reserve(ctx, &st->d_grad, &st->grad_cap, (size_t)(N * dim)));
   DG_CHECK(dg_reserve(ctx, &st->d_carry, &st->carry_cap, (size_t)(n_blocks * dim)));
   DG_CHECK(dg_reserve(ctx, &st->d_headp, &st->headp_cap, (size_t)(n_blocks * dim)));
@@ -1105,7 +1402,6 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   SegArgs sa{};
   sa.keys = st->d_k[cur];
   sa.vals = st->d_v[cur];
-  sa.boff = st->d_off;
   sa.N = N;
   sa.n = n;
   sa.K = K;
@@ -1114,14 +1410,20 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   sa.dz = st->d_dz;
   sa.stage = st->d_stage;
   sa.xt = bf16 ? (const void*)st->d_bf0 : (const void*)tv.syn0;
-  sa.seg_key = st->d_seg_key;
-  sa.seg_first = st->d_seg_first;
   sa.grad = st->d_grad;
   sa.carry = st->d_carry;
   sa.headp = st->d_headp;
   sa.fin = st->d_fin;
+  const bool shadow = st->d_bf0 && st->bf_version == tv.version;
+  if (apply) {
+    sa.lr = lr;
+    sa.syn0 = tv.syn0;
+    sa.syn1 = tv.syn1;
+    sa.bf0 = shadow ? st->d_bf0 : nullptr;
+    sa.bf1 = shadow ? st->d_bf1 : nullptr;
+  }
   const int g = waves_grid(ctx, n_blocks);
-  const int gf = (int)std::min<uint64_t>(n_blocks, (uint64_t)ctx->num_cus * 4);
+  const int gf = (int)((n_blocks + kFixWaves / 2 - 1) / (kFixWaves / 2));
   DG_CHECK(dg_time_begin(ctx, "tuple_seg", &tk));
   if (bf16)
     launch_seg<uint16_t>(ctx, sa, dim, g, gf);
@@ -1134,29 +1436,14 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   st->grad_dim = dim;
   st->grad_rows = tv.rows;
   st->grad_N = N;
-  st->grad_blocks = n_blocks;
   if (apply) {
-    DG_CHECK(dg_time_begin(ctx, "tuple_apply", &tk));
-    const bool shadow = st->d_bf0 && st->bf_version == tv.version;
-    uint16_t* b0 = shadow ? st->d_bf0 : nullptr;
-    uint16_t* b1 = shadow ? st->d_bf1 : nullptr;
-    const int ga = waves_grid(ctx, N);
-    if (dim == 64)
-      tuple_apply_kernel<64><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
-                                                          lr, tv.syn0, tv.syn1, b0, b1);
-    else if (dim == 128)
-      tuple_apply_kernel<128><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
-                                                           lr, tv.syn0, tv.syn1, b0, b1);
-    else
-      tuple_apply_kernel<256><<<ga, 256, 0, ctx->stream>>>(st->d_seg_key, st->d_grad, st->d_off + n_blocks, rb,
-                                                           lr, tv.syn0, tv.syn1, b0, b1);
-    DG_HIP(ctx, hipGetLastError());
-    DG_CHECK(dg_time_end(ctx, tk));
     const uint64_t v = dg_sgns_written(ctx);
     if (shadow) st->bf_version = v;  // the shadow rows were updated with the FP32 rows
   }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_CHECK(dg_time_resolve(ctx));
+  if (*st->h_err)
+    return dg_fail(ctx, DG_ERR_HIP, "tuple sort: a look-back wait or an output bound failed (%u)", *st->h_err);
   st->have_grads = true;
   return DG_OK;
 }
@@ -1179,7 +1466,7 @@ extern "C" int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, u
   hipError_t e = hipMemcpy(d_rows, rows, sizeof(uint32_t) * n_rows, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     grad_rows_kernel<<<waves_grid(ctx, n_rows), 256, 0, ctx->stream>>>(
-        st->d_seg_key, st->d_grad, st->d_off + st->grad_blocks, st->grad_rb, (uint32_t)table, d_rows, n_rows,
+        st->d_k[st->sorted], st->d_grad, st->grad_N, st->grad_rb, (uint32_t)table, d_rows, n_rows,
         st->grad_dim, d_out);
     e = hipGetLastError();
   }
