@@ -160,7 +160,7 @@ struct TupleArgs {
   const void* xt;          // syn0 (float or BF16 shadow)
   const void* yt;          // syn1
   float* logits;           // forward: n x (1 + K)
-  float* tloss;            // forward: n
+  double* wloss;           // forward: one loss partial per wavefront of the grid
   const float* dlogits;    // backward: upstream dL/dz, or null for the loss's own
   float* dz;               // backward: n x (1 + K)
   float* stage;            // backward: n x dim, context-row gradient per tuple
@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
   const uint32_t K1 = a.K + 1, W = a.K + 2;
   const uint32_t idx = hl / kStride;  // value index this lane ends with
   const uint64_t groups = (a.n + T - 1) / T;
+  double wsum = 0.0;  // forward: this wavefront's loss, its tuples in processing order
   if constexpr (BWD) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n_clear;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -250,7 +251,11 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
       float acc = 0.0f;
 #pragma unroll
       for (int j = 0; j < NV; ++j) acc += __shfl(l, (int)hb + j * kStride, 64);
-      if (tv && hl == 0) a.tloss[t] = acc;
+#pragma unroll
+      for (int q = 0; q < T; ++q) {  // the step's tuples in group order, on every lane
+        const float lq = __shfl(acc, q * G, 64);
+        if ((uint64_t)T * pr + q < a.n) wsum += (double)lq;
+      }
     } else {
       float dz = 0.0f;
       if (my_live) {
@@ -295,36 +300,38 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
     }
     id = id_next;
   }
+  if constexpr (!BWD) {
+    if (lane == 0) a.wloss[wave] = wsum;
+  }
 }
 
-// Deterministic loss in two levels: block k sums the k-th of kLossParts
-// contiguous chunks (threads strided inside it, then a fixed tree), and one
-// block sums the chunk totals with the same tree.
-constexpr int kLossParts = 256;
+// Deterministic loss: the forward's wavefronts each sum their tuples in a
+// fixed order; one block of kLossThreads adds the wavefront partials, thread
+// i those at i, i + kLossThreads, ... in order (eight loads in flight), then
+// a fixed tree.  For a given tuple count the grid, and so the order, is
+// fixed on a device.
+constexpr int kLossThreads = 1024;
 
-__device__ double block_tree_sum(double v, double* part) {
-  part[threadIdx.x] = v;
+__global__ __launch_bounds__(kLossThreads) void loss_sum_kernel(const double* wloss, uint32_t count, double* out) {
+  __shared__ double part[kLossThreads];
+  double s = 0.0;
+  for (uint32_t i0 = threadIdx.x; i0 < count; i0 += 8 * kLossThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kLossThreads;
+      v[u] = i < count ? wloss[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  part[threadIdx.x] = s;
   __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
+  for (int w = kLossThreads / 2; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  return part[0];
-}
-
-__global__ __launch_bounds__(256) void loss_part_kernel(const float* tloss, uint64_t n, double* parts) {
-  __shared__ double part[256];
-  const uint64_t lo = n * blockIdx.x / kLossParts, hi = n * (blockIdx.x + 1) / kLossParts;
-  double s = 0.0;
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) s += (double)tloss[i];
-  const double t = block_tree_sum(s, part);
-  if (threadIdx.x == 0) parts[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(256) void loss_final_kernel(const double* parts, double* out) {
-  __shared__ double part[256];
-  const double t = block_tree_sum(parts[threadIdx.x], part);
-  if (threadIdx.x == 0) *out = t;
+  if (threadIdx.x == 0) *out = part[0];
 }
 
 // ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass ------------
@@ -1008,8 +1015,8 @@ struct dg_tuple_state {
   size_t tuples_cap = 0;
   float* d_logits = nullptr;
   size_t logits_cap = 0;
-  float* d_tloss = nullptr;
-  size_t tloss_cap = 0;
+  double* d_wloss = nullptr;  // loss partial per forward wavefront
+  size_t wloss_cap = 0;
   double* d_loss = nullptr;
   float* d_dl = nullptr;  // upstream dL/dz
   size_t dl_cap = 0;
@@ -1025,7 +1032,6 @@ struct dg_tuple_state {
   size_t lb_cap = 0;
   uint32_t* h_err = nullptr;    // host-visible: set by a kernel whose look-back wait or store bound failed
   uint32_t* d_err = nullptr;    // the device's address of h_err
-  double* d_loss_parts = nullptr;
   float* d_grad = nullptr;
   size_t grad_cap = 0;
   float* d_carry = nullptr;
@@ -1051,9 +1057,9 @@ struct dg_tuple_state {
 void dg_tuples_free(dg_ctx* ctx) {
   dg_tuple_state* t = ctx->tuples;
   if (!t) return;
-  void* bufs[] = {t->d_tuples, t->d_logits, t->d_tloss,    t->d_loss,       t->d_dl,    t->d_dz,
+  void* bufs[] = {t->d_tuples, t->d_logits, t->d_wloss,    t->d_loss,       t->d_dl,    t->d_dz,
                   t->d_stage,  t->d_k[0],   t->d_k[1],     t->d_v[0],       t->d_v[1],
-                  t->d_lb,     t->d_loss_parts,
+                  t->d_lb,
                   t->d_grad,   t->d_carry,  t->d_headp,    t->d_fin,        t->d_bf0,   t->d_bf1};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -1118,13 +1124,14 @@ int ensure_bf16(dg_ctx* ctx, dg_tuple_state* st, const dg_table_view& tv) {
 
 // Lanes per tuple: 8 for BF16 rows with K + 2 <= 8 ids per tuple (a BF16
 // row is half an FP32 row, so twice the tuples fit the same registers), 16
-// for K + 2 <= 16, otherwise 32.
+// for K + 2 <= 16, otherwise 32.  Returns the wavefronts launched (at most
+// 128 per CU: the forward writes one loss partial per wavefront).
 template <typename E, int DIM, bool BWD>
-void launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
+uint32_t launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
   if (sizeof(E) == 2 && DIM >= 128 && a.K + 2 <= 8) {
     const int g = waves_grid(ctx, (a.n + 7) / 8);  // a tuple per 8 lanes
     tuple_fwd_kernel<E, DIM, 8, BWD, 8><<<g, 256, 0, ctx->stream>>>(a);
-    return;
+    return 4u * (uint32_t)g;
   }
   if (a.K + 2 <= 16) {
     const int g = waves_grid(ctx, (a.n + 3) / 4);  // a tuple per 16 lanes
@@ -1132,26 +1139,26 @@ void launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
       tuple_fwd_kernel<E, DIM, 8, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
     else
       tuple_fwd_kernel<E, DIM, 16, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
-    return;
+    return 4u * (uint32_t)g;
   }
   const int g = waves_grid(ctx, (a.n + 1) / 2);  // a tuple per half-wavefront
   if (a.K + 1 <= 8)
     tuple_fwd_kernel<E, DIM, 8, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
   else
     tuple_fwd_kernel<E, DIM, 16, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
+  return 4u * (uint32_t)g;
 }
 
 template <bool BWD>
-void launch_fwd_form(dg_ctx* ctx, const TupleArgs& a, int bf16, uint32_t dim) {
+uint32_t launch_fwd_form(dg_ctx* ctx, const TupleArgs& a, int bf16, uint32_t dim) {
   if (bf16) {
-    if (dim == 64) launch_fwd<uint16_t, 64, BWD>(ctx, a);
-    else if (dim == 128) launch_fwd<uint16_t, 128, BWD>(ctx, a);
-    else launch_fwd<uint16_t, 256, BWD>(ctx, a);
-  } else {
-    if (dim == 64) launch_fwd<float, 64, BWD>(ctx, a);
-    else if (dim == 128) launch_fwd<float, 128, BWD>(ctx, a);
-    else launch_fwd<float, 256, BWD>(ctx, a);
+    if (dim == 64) return launch_fwd<uint16_t, 64, BWD>(ctx, a);
+    if (dim == 128) return launch_fwd<uint16_t, 128, BWD>(ctx, a);
+    return launch_fwd<uint16_t, 256, BWD>(ctx, a);
   }
+  if (dim == 64) return launch_fwd<float, 64, BWD>(ctx, a);
+  if (dim == 128) return launch_fwd<float, 128, BWD>(ctx, a);
+  return launch_fwd<float, 256, BWD>(ctx, a);
 }
 
 template <typename E>
@@ -1181,9 +1188,8 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
   if (bf16) DG_CHECK(ensure_bf16(ctx, st, tv));
   const uint32_t K1 = K + 1;
   DG_CHECK(dg_reserve(ctx, &st->d_logits, &st->logits_cap, (size_t)(n * K1)));
-  DG_CHECK(dg_reserve(ctx, &st->d_tloss, &st->tloss_cap, (size_t)n));
+  DG_CHECK(dg_reserve(ctx, &st->d_wloss, &st->wloss_cap, (size_t)ctx->num_cus * 128));
   if (!st->d_loss) DG_HIP(ctx, hipMalloc((void**)&st->d_loss, sizeof(double)));
-  if (!st->d_loss_parts) DG_HIP(ctx, hipMalloc((void**)&st->d_loss_parts, sizeof(double) * kLossParts));
   TupleArgs a{};
   a.tuples = st->d_tuples;
   a.n = n;
@@ -1191,12 +1197,11 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
   a.xt = bf16 ? (const void*)st->d_bf0 : (const void*)tv.syn0;
   a.yt = bf16 ? (const void*)st->d_bf1 : (const void*)tv.syn1;
   a.logits = st->d_logits;
-  a.tloss = st->d_tloss;
+  a.wloss = st->d_wloss;
   size_t tk;
   DG_CHECK(dg_time_begin(ctx, "tuple_fwd", &tk));
-  launch_fwd_form<false>(ctx, a, bf16, tv.dim);
-  loss_part_kernel<<<kLossParts, 256, 0, ctx->stream>>>(st->d_tloss, n, st->d_loss_parts);
-  loss_final_kernel<<<1, 256, 0, ctx->stream>>>(st->d_loss_parts, st->d_loss);
+  const uint32_t waves = launch_fwd_form<false>(ctx, a, bf16, tv.dim);
+  loss_sum_kernel<<<1, kLossThreads, 0, ctx->stream>>>(st->d_wloss, waves, st->d_loss);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   if (logits)
@@ -1213,15 +1218,15 @@ extern "C" int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint6
 namespace {
 
 // ---- the allocation-free forward ---------------------------------------------
-// Scratch of dg_sgns_tuples_forward_noalloc: the loss's partial sums
-// (kLossParts doubles), a range-check flag, then one loss term per tuple.
-// Each device holds its own copy; host code reaches it only through the
-// address hipGetSymbolAddress gives on that device (dg_tuples_scratch_setup).
+// Scratch of dg_sgns_tuples_forward_noalloc: the loss partial of each
+// forward wavefront (at most kNoAllocMaxWaves doubles), then a range-check
+// flag.  Each device holds its own copy; host code reaches it only through
+// the address hipGetSymbolAddress gives on that device
+// (dg_tuples_scratch_setup).
 constexpr uint64_t kNoAllocMaxTuples = 1ull << 20;
-constexpr int kNoAllocParts = 2 * kLossParts;    // floats holding the partial sums
-constexpr int kNoAllocFlag = kNoAllocParts;      // float index of the flag
-constexpr int kNoAllocTerms = kNoAllocParts + 64;  // first loss term (256-byte aligned)
-static __device__ __align__(256) float scratch[kNoAllocTerms + kNoAllocMaxTuples];
+constexpr uint32_t kNoAllocMaxWaves = 65536;              // 128 per CU on up to 512 CUs
+constexpr int kNoAllocFlag = 2 * kNoAllocMaxWaves;        // float index of the flag
+static __device__ __align__(256) float scratch[kNoAllocFlag + 64];
 
 constexpr int kMaxScratchDevices = 64;
 std::mutex g_scratch_mu;
@@ -1270,11 +1275,13 @@ extern "C" int dg_sgns_tuples_forward_noalloc(dg_ctx* ctx, const uint32_t* d_tup
   if (n == 0 || n > kNoAllocMaxTuples)
     return dg_fail(ctx, DG_ERR_ARG, "%llu tuples: the allocation-free forward takes 1 to %llu",
                    (unsigned long long)n, (unsigned long long)kNoAllocMaxTuples);
+  if ((uint64_t)ctx->num_cus * 128 > kNoAllocMaxWaves)
+    return dg_fail(ctx, DG_ERR_UNSUPPORTED, "%d CUs: the allocation-free forward covers at most %u",
+                   ctx->num_cus, kNoAllocMaxWaves / 128);
   DG_HIP(ctx, hipSetDevice(ctx->device));
   float* base = ctx->noalloc_scratch;
-  double* parts = reinterpret_cast<double*>(base);
+  double* wloss = reinterpret_cast<double*>(base);
   uint32_t* flag = reinterpret_cast<uint32_t*>(base + kNoAllocFlag);
-  float* terms = base + kNoAllocTerms;
   // every row id is dereferenced by the forward: check them on the device
   const uint64_t count = n * (K + 2);
   noalloc_clear_kernel<<<1, 64, 0, ctx->stream>>>(flag);
@@ -1292,10 +1299,9 @@ extern "C" int dg_sgns_tuples_forward_noalloc(dg_ctx* ctx, const uint32_t* d_tup
   a.xt = tv.syn0;
   a.yt = tv.syn1;
   a.logits = d_logits;
-  a.tloss = terms;
-  launch_fwd_form<false>(ctx, a, 0, tv.dim);
-  loss_part_kernel<<<kLossParts, 256, 0, ctx->stream>>>(terms, n, parts);
-  loss_final_kernel<<<1, 256, 0, ctx->stream>>>(parts, d_loss);
+  a.wloss = wloss;
+  const uint32_t waves = launch_fwd_form<false>(ctx, a, 0, tv.dim);
+  loss_sum_kernel<<<1, kLossThreads, 0, ctx->stream>>>(wloss, waves, d_loss);
   DG_HIP(ctx, hipGetLastError());
   return DG_OK;
 }

@@ -352,7 +352,8 @@ int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uin
  * passes device pointers: d_tuples[n * (K + 2)] (rows of the FP32 tables),
  * d_logits[n * (1 + K)] and d_loss (one double); the results are those of
  * dg_sgns_tuples_forward with bf16 = 0, bit for bit.  Working memory is a
- * fixed array in each device's memory (4 MiB, held until the process ends):
+ * fixed array in each device's memory (512 KiB: one loss partial per forward
+ * wavefront, held until the process ends; devices of up to 512 CUs):
  * n is at most 1,048,576 (DG_ERR_ARG beyond).  The row ids are checked on the
  * device first (DG_ERR_ARG if one is out of range; this waits for the
  * context's stream once); the forward itself is queued on the context's
