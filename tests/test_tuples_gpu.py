@@ -214,3 +214,36 @@ def test_differentiation_workload_fd(tctx, bf16):
     print(f"[tuples-gpu]   gradient vs FP64 central differences on 256 coordinates: {err:.2e}",
           flush=True)
     assert err < (TOL_BF16_TOTAL if bf16 else 1e-4)
+
+
+@pytest.mark.parametrize("rows,dim,n,label", [
+    (3_000_000, 64, 200_000, "1.4M entries: more sort tiles than CUs, tiles ordered by ticket"),
+    (8_500_000, 64, 40_000, "rows beyond 2^23: 25-bit keys, four radix passes"),
+])
+def test_sort_paths_at_scale(tctx, rows, dim, n, label):
+    """The backward's sort takes tiles in ticket order once there are more
+    tiles (4,096 entries each) than CUs, and a fourth 8-bit pass for keys
+    wider than 24 bits: gradient rows against the oracle, and bit-equal over
+    two backward calls.  A third of the tuples get uniform negatives, so the
+    high digits vary."""
+    K = 5
+    print(f"[tuples-gpu] {label}: rows={rows} dim={dim} n={n}", flush=True)
+    t = zipf_tuples(rows, n, K, 91)
+    rng = np.random.default_rng(5)
+    t[::3, 2:] = rng.integers(0, rows, size=t[::3, 2:].shape).astype(np.uint32)
+    s0, s1 = normal_tables(rows, dim, 93)
+    tctx.sgns_tables_create(s0, s1)
+    o = so.Tuples(t, K, s0, s1)
+    r0, r1 = touched(t)
+    # the hottest rows and a sample of the rest
+    probe0 = np.unique(np.concatenate([r0[:64], rng.choice(r0, min(len(r0), 1000), replace=False)]))
+    probe1 = np.unique(np.concatenate([r1[:64], r1[-64:], rng.choice(r1, min(len(r1), 1000), replace=False)]))
+    got = []
+    for _ in range(2):
+        tctx.tuples_backward(t, K)
+        got.append((tctx.grad_rows(0, probe0, dim), tctx.grad_rows(1, probe1, dim)))
+    assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
+    e0 = rel(got[0][0], o.grad_rows(0, probe0))
+    e1 = rel(got[0][1], o.grad_rows(1, probe1))
+    print(f"[tuples-gpu]   gradient rows vs oracle: {e0:.2e} {e1:.2e}", flush=True)
+    assert e0 < TOL32 and e1 < TOL32
