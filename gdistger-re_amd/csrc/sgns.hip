@@ -102,6 +102,34 @@ __device__ __forceinline__ float sigmoid_of(float f, const float* expt, int exac
 // The draw of corpus token i in epoch e is word 0 of Philox(counter (0, i lo,
 // i hi, SAMPLE), key dg_epoch_key(seed, e)).
 // A token whose vertex id is not below rank_size has no rank (skipped).
+// The walk's tokens eight at a time: their ids, then their ranks, then the
+// keep limits, each as eight independent loads in flight (a thread per walk).
+constexpr int kSubChunk = 8;
+
+struct SubChunk {
+  uint32_t r[kSubChunk];   // rank, kNoRank for a token without one or past the walk
+  bool kept[kSubChunk];
+};
+
+__device__ __forceinline__ void subsample_chunk(const uint32_t* tokens, uint64_t i0, uint64_t end,
+                                                const uint32_t* rank, uint32_t rank_size,
+                                                const uint32_t* keep, uint64_t key, SubChunk& c) {
+  uint32_t v[kSubChunk], kp[kSubChunk];
+#pragma unroll
+  for (int u = 0; u < kSubChunk; ++u) v[u] = i0 + u < end ? tokens[i0 + u] : 0xffffffffu;
+#pragma unroll
+  for (int u = 0; u < kSubChunk; ++u) c.r[u] = v[u] < rank_size ? rank[v[u]] : kNoRank;
+#pragma unroll
+  for (int u = 0; u < kSubChunk; ++u) kp[u] = c.r[u] != kNoRank ? keep[c.r[u]] : 0u;
+#pragma unroll
+  for (int u = 0; u < kSubChunk; ++u) {
+    const uint64_t i = i0 + u;
+    const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
+                                        (uint32_t)key, (uint32_t)(key >> 32));
+    c.kept[u] = c.r[u] != kNoRank && (o.x0 & 0xFFFFu) <= kp[u];
+  }
+}
+
 __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* offsets,
                                        uint64_t n_walks, const uint32_t* rank, uint32_t rank_size,
                                        const uint32_t* keep, uint64_t seed, uint32_t epoch,
@@ -110,13 +138,12 @@ __global__ void subsample_count_kernel(const uint32_t* tokens, const uint64_t* o
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = 0;
-    for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
-      const uint32_t v = tokens[i];
-      const uint32_t r = v < rank_size ? rank[v] : kNoRank;
-      if (r == kNoRank) continue;
-      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
-                                          (uint32_t)key, (uint32_t)(key >> 32));
-      c += ((o.x0 & 0xFFFFu) <= keep[r]);
+    const uint64_t end = offsets[w + 1];
+    for (uint64_t i0 = offsets[w]; i0 < end; i0 += kSubChunk) {
+      SubChunk ch;
+      subsample_chunk(tokens, i0, end, rank, rank_size, keep, key, ch);
+#pragma unroll
+      for (int u = 0; u < kSubChunk; ++u) c += ch.kept[u] ? 1u : 0u;
     }
     kept[w] = c;
   }
@@ -130,13 +157,13 @@ __global__ void subsample_write_kernel(const uint32_t* tokens, const uint64_t* o
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_walks;
        w += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t o_pos = sen_off[w];
-    for (uint64_t i = offsets[w]; i < offsets[w + 1]; ++i) {
-      const uint32_t v = tokens[i];
-      const uint32_t r = v < rank_size ? rank[v] : kNoRank;
-      if (r == kNoRank) continue;
-      const dg_u32x4 o = dg_philox4x32_10(0, (uint32_t)i, (uint32_t)(i >> 32), DG_TAG_SAMPLE,
-                                          (uint32_t)key, (uint32_t)(key >> 32));
-      if ((o.x0 & 0xFFFFu) <= keep[r]) sen[o_pos++] = (int32_t)r;
+    const uint64_t end = offsets[w + 1];
+    for (uint64_t i0 = offsets[w]; i0 < end; i0 += kSubChunk) {
+      SubChunk ch;
+      subsample_chunk(tokens, i0, end, rank, rank_size, keep, key, ch);
+#pragma unroll
+      for (int u = 0; u < kSubChunk; ++u)
+        if (ch.kept[u]) sen[o_pos++] = (int32_t)ch.r[u];
     }
   }
 }
