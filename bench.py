@@ -152,6 +152,17 @@ def tuple_workload(ctx, iters=20, warmup=3):
             "algorithmic_bytes_per_step": alg, "achieved_gbs": gbs, "frac_hbm": gbs / HBM_GBS}
         log(f"tuples {'bf16' if bf16 else 'fp32'}: {step_us:.1f} us per forward+backward+apply "
             f"of {n} tuples ({n / (step_us * 1e-6):.3e} tuples/s)")
+        # the same step as one call (dg_sgns_tuples_step: one gather pass for
+        # the logits, loss and gradients)
+        for _ in range(warmup):
+            ctx.tuples_step(t, K, bf16=bf16, lr=0.0, apply=True)
+        ctx.timing_reset()
+        for _ in range(iters):
+            ctx.tuples_step(t, K, bf16=bf16, lr=0.0, apply=True)
+        fused_us = sum(1e3 * ctx.timing(ph)[1] / iters for ph in phases)
+        res["bf16" if bf16 else "fp32"]["fused_step_us"] = fused_us
+        log(f"tuples {'bf16' if bf16 else 'fp32'}: {fused_us:.1f} us per fused step "
+            f"(dg_sgns_tuples_step)")
     return res
 
 

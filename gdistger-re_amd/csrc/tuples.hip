@@ -159,8 +159,9 @@ struct TupleArgs {
   uint32_t K;
   const void* xt;          // syn0 (float or BF16 shadow)
   const void* yt;          // syn1
-  float* logits;           // forward: n x (1 + K)
-  double* wloss;           // forward: one loss partial per wavefront of the grid
+  float* logits;           // forward (and the fused step's backward): n x (1 + K)
+  double* wloss;           // forward: one loss partial per wavefront of the grid; null in a
+                           // plain backward
   const float* dlogits;    // backward: upstream dL/dz, or null for the loss's own
   float* dz;               // backward: n x (1 + K)
   float* stage;            // backward: n x dim, context-row gradient per tuple
@@ -176,10 +177,11 @@ struct TupleArgs {
 // per lane at dim 128, G = 16 does the same for a BF16 row (four tuples per
 // wavefront, twice the rows in flight).  The next group of tuples' ids are
 // loaded before the current one is reduced.
-// BWD = false: logits and per-tuple loss.  BWD = true: dL/dz, the staged
+// OUT: the logits and the wavefront's loss partial (the forward, and the
+// backward of the fused step).  BWD: dL/dz, the staged
 // context-row gradient and the sort pairs (entries [0, n) for syn0 rows,
 // [n, N) for syn1 rows, in tuple order).
-template <typename E, int DIM, int NV, bool BWD, int G>
+template <typename E, int DIM, int NV, bool BWD, int G, bool OUT>
 __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
   constexpr int T = 64 / G;                 // tuples per wavefront
   constexpr int D = DIM / G;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
     }
     const float z = group_multi_sum<NV, G>(p, lane);
     const bool writer = tv && (hl % kStride) == 0 && idx < K1;
-    if constexpr (!BWD) {
+    if constexpr (OUT) {  // the forward's outputs (also from the backward of the fused step)
       if (writer) a.logits[t * K1 + idx] = z;
       // loss: one lane per live value index, summed in index order
       const float l = (writer && my_live) ? softplus_f(idx == 0 ? -z : z) : 0.0f;
@@ -256,7 +258,8 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
         const float lq = __shfl(acc, q * G, 64);
         if ((uint64_t)T * pr + q < a.n) wsum += (double)lq;
       }
-    } else {
+    }
+    if constexpr (BWD) {
       float dz = 0.0f;
       if (my_live) {
         if (a.dlogits)
@@ -300,7 +303,7 @@ __global__ __launch_bounds__(256) void tuple_fwd_kernel(TupleArgs a) {
     }
     id = id_next;
   }
-  if constexpr (!BWD) {
+  if constexpr (OUT) {
     if (lane == 0) a.wloss[wave] = wsum;
   }
 }
@@ -1126,39 +1129,39 @@ int ensure_bf16(dg_ctx* ctx, dg_tuple_state* st, const dg_table_view& tv) {
 // row is half an FP32 row, so twice the tuples fit the same registers), 16
 // for K + 2 <= 16, otherwise 32.  Returns the wavefronts launched (at most
 // 128 per CU: the forward writes one loss partial per wavefront).
-template <typename E, int DIM, bool BWD>
+template <typename E, int DIM, bool BWD, bool OUT>
 uint32_t launch_fwd(dg_ctx* ctx, const TupleArgs& a) {
   if (sizeof(E) == 2 && DIM >= 128 && a.K + 2 <= 8) {
     const int g = waves_grid(ctx, (a.n + 7) / 8);  // a tuple per 8 lanes
-    tuple_fwd_kernel<E, DIM, 8, BWD, 8><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fwd_kernel<E, DIM, 8, BWD, 8, OUT><<<g, 256, 0, ctx->stream>>>(a);
     return 4u * (uint32_t)g;
   }
   if (a.K + 2 <= 16) {
     const int g = waves_grid(ctx, (a.n + 3) / 4);  // a tuple per 16 lanes
     if (a.K + 1 <= 8)
-      tuple_fwd_kernel<E, DIM, 8, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
+      tuple_fwd_kernel<E, DIM, 8, BWD, 16, OUT><<<g, 256, 0, ctx->stream>>>(a);
     else
-      tuple_fwd_kernel<E, DIM, 16, BWD, 16><<<g, 256, 0, ctx->stream>>>(a);
+      tuple_fwd_kernel<E, DIM, 16, BWD, 16, OUT><<<g, 256, 0, ctx->stream>>>(a);
     return 4u * (uint32_t)g;
   }
   const int g = waves_grid(ctx, (a.n + 1) / 2);  // a tuple per half-wavefront
   if (a.K + 1 <= 8)
-    tuple_fwd_kernel<E, DIM, 8, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fwd_kernel<E, DIM, 8, BWD, 32, OUT><<<g, 256, 0, ctx->stream>>>(a);
   else
-    tuple_fwd_kernel<E, DIM, 16, BWD, 32><<<g, 256, 0, ctx->stream>>>(a);
+    tuple_fwd_kernel<E, DIM, 16, BWD, 32, OUT><<<g, 256, 0, ctx->stream>>>(a);
   return 4u * (uint32_t)g;
 }
 
-template <bool BWD>
+template <bool BWD, bool OUT = !BWD>
 uint32_t launch_fwd_form(dg_ctx* ctx, const TupleArgs& a, int bf16, uint32_t dim) {
   if (bf16) {
-    if (dim == 64) return launch_fwd<uint16_t, 64, BWD>(ctx, a);
-    if (dim == 128) return launch_fwd<uint16_t, 128, BWD>(ctx, a);
-    return launch_fwd<uint16_t, 256, BWD>(ctx, a);
+    if (dim == 64) return launch_fwd<uint16_t, 64, BWD, OUT>(ctx, a);
+    if (dim == 128) return launch_fwd<uint16_t, 128, BWD, OUT>(ctx, a);
+    return launch_fwd<uint16_t, 256, BWD, OUT>(ctx, a);
   }
-  if (dim == 64) return launch_fwd<float, 64, BWD>(ctx, a);
-  if (dim == 128) return launch_fwd<float, 128, BWD>(ctx, a);
-  return launch_fwd<float, 256, BWD>(ctx, a);
+  if (dim == 64) return launch_fwd<float, 64, BWD, OUT>(ctx, a);
+  if (dim == 128) return launch_fwd<float, 128, BWD, OUT>(ctx, a);
+  return launch_fwd<float, 256, BWD, OUT>(ctx, a);
 }
 
 template <typename E>
@@ -1306,8 +1309,13 @@ extern "C" int dg_sgns_tuples_forward_noalloc(dg_ctx* ctx, const uint32_t* d_tup
   return DG_OK;
 }
 
-extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K,
-                                       int bf16, const float* dlogits, float lr, int apply) {
+namespace {
+
+// The backward (and, fused, the forward's logits and loss from the same pass
+// over the rows, with the loss's own gradient).
+int tuples_backward_run(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
+                        const float* dlogits, float lr, int apply, bool fused, float* logits,
+                        double* loss) {
   if (!ctx || !tuples) return DG_ERR_ARG;
   dg_table_view tv;
   DG_CHECK(dg_sgns_view(ctx, &tv));
@@ -1334,6 +1342,11 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
     DG_CHECK(dg_reserve(ctx, &st->d_dl, &st->dl_cap, (size_t)(n * K1)));
     DG_HIP(ctx, hipMemcpyAsync(st->d_dl, dlogits, sizeof(float) * n * K1, hipMemcpyHostToDevice,
                                ctx->stream));
+  }
+  if (fused) {
+    DG_CHECK(dg_reserve(ctx, &st->d_logits, &st->logits_cap, (size_t)(n * K1)));
+    DG_CHECK(dg_reserve(ctx, &st->d_wloss, &st->wloss_cap, (size_t)ctx->num_cus * 128));
+    if (!st->d_loss) DG_HIP(ctx, hipMalloc((void**)&st->d_loss, sizeof(double)));
   }
   // the sort's sizes and the words it expects zero (cleared by the backward kernel)
   if (!st->h_err) {
@@ -1372,9 +1385,15 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
   a.row_bits = (uint32_t)rb;
   a.clear = st->d_lb;
   a.n_clear = n_lb;
+  if (fused) {
+    a.logits = st->d_logits;
+    a.wloss = st->d_wloss;
+  }
   size_t tk;
   DG_CHECK(dg_time_begin(ctx, "tuple_prep", &tk));
-  launch_fwd_form<true>(ctx, a, bf16, dim);
+  const uint32_t waves = fused ? launch_fwd_form<true, true>(ctx, a, bf16, dim)
+                               : launch_fwd_form<true, false>(ctx, a, bf16, dim);
+  if (fused) loss_sum_kernel<<<1, kLossThreads, 0, ctx->stream>>>(st->d_wloss, waves, st->d_loss);
   DG_HIP(ctx, hipGetLastError());
   DG_CHECK(dg_time_end(ctx, tk));
   // sort (key, entry) pairs on rb + 1 key bits
@@ -1446,12 +1465,32 @@ extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint
     const uint64_t v = dg_sgns_written(ctx);
     if (shadow) st->bf_version = v;  // the shadow rows were updated with the FP32 rows
   }
+  double h_loss = 0.0;
+  if (fused) {
+    if (logits)
+      DG_HIP(ctx, hipMemcpyAsync(logits, st->d_logits, sizeof(float) * n * K1, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    DG_HIP(ctx, hipMemcpyAsync(&h_loss, st->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_CHECK(dg_time_resolve(ctx));
   if (*st->h_err)
     return dg_fail(ctx, DG_ERR_HIP, "tuple sort: a look-back wait or an output bound failed (%u)", *st->h_err);
   st->have_grads = true;
+  if (fused && loss) *loss = h_loss;
   return DG_OK;
+}
+
+}  // namespace
+
+extern "C" int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K,
+                                       int bf16, const float* dlogits, float lr, int apply) {
+  return tuples_backward_run(ctx, tuples, n, K, bf16, dlogits, lr, apply, false, nullptr, nullptr);
+}
+
+extern "C" int dg_sgns_tuples_step(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
+                                   float lr, int apply, float* logits, double* loss) {
+  return tuples_backward_run(ctx, tuples, n, K, bf16, nullptr, lr, apply, true, logits, loss);
 }
 
 extern "C" int dg_sgns_grad_rows(dg_ctx* ctx, int table, const uint32_t* rows, uint64_t n_rows,

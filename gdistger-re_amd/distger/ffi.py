@@ -201,6 +201,8 @@ def load_library():
                                              c.POINTER(c.c_double)]),
         "dg_sgns_tuples_backward": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, vp,
                                               c.c_float, c.c_int]),
+        "dg_sgns_tuples_step": (c.c_int, [vp, vp, c.c_uint64, c.c_uint32, c.c_int, c.c_float, c.c_int,
+                                          vp, c.POINTER(c.c_double)]),
         "dg_sgns_tables_create": (c.c_int, [vp, c.c_uint32, c.c_uint32, vp, vp]),
         "dg_gen_zipf": (c.c_int, [c.c_uint32, c.c_double, c.c_uint64, c.c_uint64, vp]),
         "dg_gen_normal": (c.c_int, [c.c_uint64, c.c_double, c.c_uint64, vp]),
@@ -453,6 +455,17 @@ class Context:
             raise ValueError("dlogits needs n * (K + 1) values")
         self._check(self.L.dg_sgns_tuples_backward(self.h, _vp(t), t.shape[0], K, int(bf16), _vp(dl),
                                                    float(lr), int(apply)))
+
+    def tuples_step(self, tuples, K, bf16=False, lr=0.0, apply=False):
+        """Forward and own-gradient backward in one call: (logits, loss), the
+        gradients readable through grad_rows, the update applied if asked."""
+        t = np.ascontiguousarray(tuples, dtype=np.uint32)
+        n = t.shape[0]
+        logits = np.zeros(n * (1 + K), np.float32)
+        loss = ctypes.c_double()
+        self._check(self.L.dg_sgns_tuples_step(self.h, _vp(t), n, K, int(bf16), float(lr), int(apply),
+                                               _vp(logits), ctypes.byref(loss)))
+        return logits.reshape(n, 1 + K), loss.value
 
     def sgns_tables_create(self, syn0, syn1):
         s0 = np.ascontiguousarray(syn0, dtype=np.float32)

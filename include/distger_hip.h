@@ -346,6 +346,16 @@ int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint
  * gradients stay readable through dg_sgns_grad_rows until the next backward. */
 int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
                             const float* dlogits, float lr, int apply);
+/* The forward and the backward with the loss's own gradient in one call:
+ * the logits and loss of dg_sgns_tuples_forward and the row gradients of
+ * dg_sgns_tuples_backward (dlogits = NULL), bit for bit, from one pass over
+ * the rows instead of two; apply != 0 then updates W -= lr * dL/dW.  For
+ * training loops whose loss is the tuple loss itself.  The reference has no
+ * separate forward and backward: its skip_gram_kernel takes this step in
+ * place, pair by pair (src/word2vec.cu:359-396).  logits[n * (1 + K)] and
+ * loss may be NULL; the gradients stay readable through dg_sgns_grad_rows. */
+int dg_sgns_tuples_step(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16, float lr,
+                        int apply, float* logits, double* loss);
 /* The forward without any allocation, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, hosts
  * that forbid hipMalloc).  No counterpart in the reference.  The caller

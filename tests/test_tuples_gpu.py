@@ -247,3 +247,30 @@ def test_sort_paths_at_scale(tctx, rows, dim, n, label):
     e1 = rel(got[0][1], o.grad_rows(1, probe1))
     print(f"[tuples-gpu]   gradient rows vs oracle: {e0:.2e} {e1:.2e}", flush=True)
     assert e0 < TOL32 and e1 < TOL32
+
+
+@pytest.mark.parametrize("dim,K,bf16", [(128, 5, False), (128, 5, True), (256, 15, False), (64, 10, True)])
+def test_fused_step_equals_forward_and_backward(tctx, dim, K, bf16):
+    """dg_sgns_tuples_step: the logits and loss of the forward and the
+    gradients of the backward (the loss's own gradient), bit for bit, and
+    the same update as the backward's apply."""
+    rows, n = 5000, 4096
+    print(f"[tuples-gpu] fused step: dim={dim} K={K} bf16={bf16}", flush=True)
+    t = zipf_tuples(rows, n, K, 61 + K)
+    s0, s1 = normal_tables(rows, dim, 71)
+    tctx.sgns_tables_create(s0, s1)
+    r0, r1 = touched(t)
+    logits, loss = tctx.tuples_forward(t, K, bf16=bf16)
+    tctx.tuples_backward(t, K, bf16=bf16)
+    g0, g1 = tctx.grad_rows(0, r0, dim), tctx.grad_rows(1, r1, dim)
+    fl, floss = tctx.tuples_step(t, K, bf16=bf16)
+    assert np.array_equal(fl, logits) and floss == loss
+    assert np.array_equal(tctx.grad_rows(0, r0, dim), g0)
+    assert np.array_equal(tctx.grad_rows(1, r1, dim), g1)
+    tctx.tuples_backward(t, K, bf16=bf16, lr=0.01, apply=True)
+    a0, a1 = tctx.sgns_tables(dim)
+    tctx.sgns_tables_create(s0, s1)
+    tctx.tuples_step(t, K, bf16=bf16, lr=0.01, apply=True)
+    b0, b1 = tctx.sgns_tables(dim)
+    assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+    assert not np.array_equal(a1, s1)
