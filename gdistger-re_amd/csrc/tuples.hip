@@ -374,8 +374,21 @@ __device__ __forceinline__ void lb_add(uint64_t* p, uint64_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Look-back words of one column (a digit, or the single count of the segment
-// numbering): tile words tw[t * stride] and group words gw[g * stride].
+// A failed look-back wait or output bound: the code goes to the host word
+// (read after the call) and to a device word (the update kernel then leaves
+// the tables unchanged).
+struct SortErr {
+  uint32_t* host;
+  uint32_t* dev;
+};
+
+__device__ __forceinline__ void sort_fail(const SortErr& e, uint32_t code) {
+  *e.host = code;
+  *e.dev = code;
+}
+
+// Look-back words of one column (a digit): tile words tw[t * stride] and
+// group words gw[g * stride].
 struct LbColumn {
   uint32_t* tw;
   uint64_t* gw;
@@ -393,7 +406,7 @@ __device__ __forceinline__ void lb_publish(const LbColumn& c, uint32_t tile, uin
 // once ready (all loads of a round in flight together).
 template <int R>
 __device__ uint32_t lb_prefix_part(const LbColumn& c, uint32_t tile, uint32_t first, uint32_t step,
-                                   uint32_t* err) {
+                                   const SortErr& err) {
   const uint32_t g = tile / kGroup, t0 = g * kGroup;
   const uint32_t m = g + (tile - t0);
   uint32_t part = 0;
@@ -412,7 +425,7 @@ __device__ uint32_t lb_prefix_part(const LbColumn& c, uint32_t tile, uint32_t fi
       uint32_t polls = 0;
       while (j < g ? (w[u] >> 32) != (uint64_t)kGroup : (w[u] & kFlagReady) == 0ull) {
         if (++polls > kSpinLimit) {
-          *err = 1u;
+          sort_fail(err, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -478,7 +491,7 @@ struct SortPass {
   uint32_t* tile_words;   // this pass's [tile][digit]
   uint64_t* group_words;  // this pass's [group][digit]
   uint32_t* ticket;       // null: tile = workgroup id (every tile is resident at once)
-  uint32_t* err;
+  SortErr err;
   uint32_t* okeys;
   uint32_t* ovals;
 };
@@ -585,7 +598,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_pass_kernel(SortPass a) {
       a.okeys[dst] = key;
       a.ovals[dst] = sval[j];
     } else {
-      *a.err = 2u;
+      sort_fail(a.err, 2u);
     }
   }
 }
@@ -605,6 +618,7 @@ struct SegArgs {
   float* carry;           // per block: sum of its last run when that run continues
   float* headp;           // per block: sum of its first run when that run started earlier
   uint32_t* fin;          // per block: 1 if headp finishes a row
+  const uint32_t* sort_err;  // non-zero: the sort failed, the tables are left unchanged
   // apply (W -= lr * grad), or syn0 == null for gradients only
   float lr;
   float* syn0;
@@ -873,6 +887,7 @@ __global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
   constexpr uint32_t kHalf = kFixWaves / 2;
   const uint64_t b = (uint64_t)blockIdx.x * kHalf + (wv % kHalf);
   const bool completes = wv < kHalf;
+  const bool upd = a.syn0 != nullptr && *a.sort_err == 0u;  // the update, unless the sort failed
   if (completes && b < n_blocks && a.fin[b]) {
     const uint32_t k = a.keys[b * kSegBlock];
     uint64_t first, b0;
@@ -899,11 +914,11 @@ __global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
 #pragma unroll
       for (int q = 0; q < D; ++q) s[q] += h[q];
       store_row<D>(a.grad + first * DIM + lane * D, s);
-      if (a.syn0) apply_one<DIM>(a, k, s);
+      if (upd) apply_one<DIM>(a, k, s);
     }
   }
   // rows whose entries all lie in block b
-  if (!completes && a.syn0 && b < n_blocks) {
+  if (!completes && upd && b < n_blocks) {
     const uint64_t base = b * kSegBlock;
     const uint32_t nv = (uint32_t)(a.N - base < (uint64_t)kSegBlock ? a.N - base : kSegBlock);
     const uint64_t i = base + lane;
@@ -962,7 +977,7 @@ __global__ __launch_bounds__(64 * kFixWaves) void tuple_fix_kernel(SegArgs a) {
 #pragma unroll
       for (int q = 0; q < D; ++q) s[q] += h[q];
       store_row<D>(a.grad + first * DIM + lane * D, s);
-      if (a.syn0) apply_one<DIM>(a, k, s);
+      if (upd) apply_one<DIM>(a, k, s);
     }
     __syncthreads();
   }
@@ -1361,12 +1376,13 @@ int tuples_backward_run(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_
   const uint64_t n_blocks = (N + kSegBlock - 1) / kSegBlock;
   // words cleared by the backward kernel, in 32-bit units (64-bit group
   // words at even offsets): the digit counts [copy][pass][digit], kMaxPasses
-  // tickets, then per pass the [tile][digit] and [group][digit] look-back
-  // words.  Tiles are numbered by workgroup id when every tile is resident
+  // tickets, the device error word, then per pass the [tile][digit] and
+  // [group][digit] look-back words.  Tiles are numbered by workgroup id when every tile is resident
   // at once (one 1,024-thread workgroup per CU), by ticket otherwise.
   auto even = [](uint64_t x) { return (x + 1) & ~1ull; };
   const uint64_t o_tick = (uint64_t)kHistCopies * kMaxPasses * 256;
-  const uint64_t o_sort = even(o_tick + kMaxPasses);
+  const uint64_t o_err = o_tick + kMaxPasses;  // the device error word
+  const uint64_t o_sort = even(o_err + 1);
   const uint64_t pass_words = even(n_tiles * 256) + 2 * n_groups * 256;
   const uint64_t n_lb = o_sort + (uint64_t)passes * pass_words;
   const bool tickets = n_tiles > (uint64_t)ctx->num_cus;
@@ -1411,7 +1427,7 @@ int tuples_backward_run(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_
     sp.tile_words = pw;
     sp.group_words = reinterpret_cast<uint64_t*>(pw + even(n_tiles * 256));
     sp.ticket = tickets ? st->d_lb + o_tick + p : nullptr;
-    sp.err = st->d_err;
+    sp.err = SortErr{st->d_err, st->d_lb + o_err};
     sp.okeys = st->d_k[1 - cur];
     sp.ovals = st->d_v[1 - cur];
     radix_pass_kernel<<<(unsigned)n_tiles, kSortThreads, 0, ctx->stream>>>(sp);
@@ -1439,6 +1455,7 @@ int tuples_backward_run(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_
   sa.carry = st->d_carry;
   sa.headp = st->d_headp;
   sa.fin = st->d_fin;
+  sa.sort_err = st->d_lb + o_err;
   const bool shadow = st->d_bf0 && st->bf_version == tv.version;
   if (apply) {
     sa.lr = lr;
@@ -1475,7 +1492,9 @@ int tuples_backward_run(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DG_CHECK(dg_time_resolve(ctx));
   if (*st->h_err)
-    return dg_fail(ctx, DG_ERR_HIP, "tuple sort: a look-back wait or an output bound failed (%u)", *st->h_err);
+    return dg_fail(ctx, DG_ERR_HIP,
+                   "tuple sort: a look-back wait or an output bound failed (%u); tables left unchanged",
+                   *st->h_err);
   st->have_grads = true;
   if (fused && loss) *loss = h_loss;
   return DG_OK;
