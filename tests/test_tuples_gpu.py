@@ -274,3 +274,27 @@ def test_fused_step_equals_forward_and_backward(tctx, dim, K, bf16):
     b0, b1 = tctx.sgns_tables(dim)
     assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
     assert not np.array_equal(a1, s1)
+
+
+@pytest.mark.parametrize("n,K", [(1, 0), (1, 5), (2, 1), (3, 5), (17, 2), (64, 0), (700, 15)])
+def test_tiny_batches(tctx, n, K):
+    """Batches far below one sort tile (4,096 entries) and one 64-entry block:
+    gradient rows, logits and loss against the oracle, and the fused step
+    against the separate calls."""
+    rows, dim = 50, 64
+    print(f"[tuples-gpu] tiny batch: n={n} K={K}", flush=True)
+    rng = np.random.default_rng(n * 31 + K)
+    t = rng.integers(0, rows, size=(n, K + 2)).astype(np.uint32)
+    s0, s1 = normal_tables(rows, dim, 3)
+    tctx.sgns_tables_create(s0, s1)
+    o = so.Tuples(t, K, s0, s1)
+    logits, loss = tctx.tuples_forward(t, K)
+    ol, oloss = o.forward()
+    assert rel(logits, ol) < TOL32 and abs(loss - oloss) < TOL32 * abs(oloss)
+    tctx.tuples_backward(t, K)
+    r = np.arange(rows, dtype=np.uint32)
+    g0, g1 = tctx.grad_rows(0, r, dim), tctx.grad_rows(1, r, dim)
+    assert rel(g0, o.grad_rows(0, r)) < TOL32 and rel(g1, o.grad_rows(1, r)) < TOL32
+    fl, floss = tctx.tuples_step(t, K)
+    assert np.array_equal(fl, logits) and floss == loss
+    assert np.array_equal(tctx.grad_rows(0, r, dim), g0) and np.array_equal(tctx.grad_rows(1, r, dim), g1)
