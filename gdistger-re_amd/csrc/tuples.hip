@@ -50,6 +50,19 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
   return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// D FP32 values as BF16 at p (p aligned to 2 D bytes): pairs as 32-bit stores.
+template <int D>
+__device__ __forceinline__ void store_bf16(uint16_t* p, const float (&v)[D]) {
+  if constexpr (D % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < D; q += 2)
+      *reinterpret_cast<uint32_t*>(p + q) = (uint32_t)to_bf16(v[q]) | ((uint32_t)to_bf16(v[q + 1]) << 16);
+  } else {
+#pragma unroll
+    for (int q = 0; q < D; ++q) p[q] = to_bf16(v[q]);
+  }
+}
+
 // D consecutive elements of a row starting at element `first`, as floats.
 template <typename E, int D>
 struct RowIO;
@@ -679,10 +692,7 @@ __device__ void apply_rows(const SegArgs& a, uint64_t mask, uint32_t key, uint32
         for (int q = 0; q < D; ++q) w[u][q] = w[u][q] - a.lr * g[u][q];
         store_row<D>((t1 ? a.syn1 : a.syn0) + off, w[u]);
         uint16_t* sh = t1 ? a.bf1 : a.bf0;
-        if (sh) {
-#pragma unroll
-          for (int q = 0; q < D; ++q) sh[off + q] = to_bf16(w[u][q]);
-        }
+        if (sh) store_bf16<D>(sh + off, w[u]);
       }
     }
   }
@@ -702,10 +712,7 @@ __device__ __forceinline__ void apply_one(const SegArgs& a, uint32_t k, const fl
   for (int q = 0; q < D; ++q) w[q] = w[q] - a.lr * g[q];
   store_row<D>(W, w);
   uint16_t* sh = t1 ? a.bf1 : a.bf0;
-  if (sh) {
-#pragma unroll
-    for (int q = 0; q < D; ++q) sh[off + q] = to_bf16(w[q]);
-  }
+  if (sh) store_bf16<D>(sh + off, w);
 }
 
 // Wave-uniform state of one block's walk over its sorted entries.
