@@ -1,10 +1,10 @@
 """The allocation-free tuple forward (dg_sgns_tuples_forward_noalloc) against
 the allocating one (dg_sgns_tuples_forward, bf16 = 0): the same logits and
 the same loss, bit for bit, over two calls in a row (the second smaller, so
-the scratch array still holds the first call's terms beyond its end); sizes
-beyond the fixed array and out-of-range rows are refused.  The test owns
-its device buffers (hipMalloc through the HIP runtime), as such a caller
-would."""
+the scratch array still holds loss partials of the first call's extra
+wavefronts past those the second one writes); tuple counts beyond the
+documented maximum and out-of-range rows are refused.  The test owns its
+device buffers (hipMalloc through the HIP runtime), as such a caller would."""
 
 import ctypes
 
