@@ -355,8 +355,11 @@ __global__ __launch_bounds__(kLossThreads) void loss_sum_kernel(const double* wl
 
 // ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass ------------
 // One histogram launch counts the digits of every pass; each pass is then one
-// launch.  Tiles take consecutive ids from a ticket counter in the order they
-// start and rank their keys by digit.  A tile then publishes, per digit, its
+// launch.  Tiles are numbered by workgroup id when every tile of the pass is
+// resident at once (no more tiles than CUs), otherwise they take consecutive
+// ids from a ticket counter in the order they start; either way a tile's
+// predecessors have started before it.  A tile ranks its keys by digit, then
+// publishes, per digit, its
 // count in its tile word (flag | 30-bit count, one 32-bit word) and adds it to
 // the word of its group of kGroup tiles (tiles added << 32 | count sum, one
 // 64-bit atomic), so relaxed device-scope accesses order both.  Its prefix
@@ -375,7 +378,7 @@ constexpr int kHistChunk = 2048;                     // keys per histogram workg
 constexpr int kMaxPasses = 4;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kFlagReady = 1u << 30;
-constexpr uint32_t kSpinLimit = 1u << 22;            // polls of one word (under a second)
+constexpr uint32_t kSpinLimit = 1u << 22;            // polls of one word (seconds, never reached)
 
 __device__ __forceinline__ uint32_t lb_load(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
