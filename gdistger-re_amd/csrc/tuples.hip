@@ -359,10 +359,10 @@ __global__ __launch_bounds__(kLossThreads) void loss_sum_kernel(const double* wl
 // resident at once (no more tiles than CUs), otherwise they take consecutive
 // ids from a ticket counter in the order they start; either way a tile's
 // predecessors have started before it.  A tile ranks its keys by digit, then
-// publishes, per digit, its
-// count in its tile word (flag | 30-bit count, one 32-bit word) and adds it to
-// the word of its group of kGroup tiles (tiles added << 32 | count sum, one
-// 64-bit atomic), so relaxed device-scope accesses order both.  Its prefix
+// publishes, per digit, its count in its tile word (flag | 30-bit count, one
+// 32-bit word) and adds it to the word of its group of kGroup tiles (tiles
+// added << 32 | count sum, one 64-bit atomic), so relaxed device-scope
+// accesses order both.  Its prefix
 // over earlier tiles is the sum of the earlier groups' words (each read once
 // all its tiles have added) and of the earlier tiles of its own group: a few
 // independent loads instead of a chain through every tile, and a tile only
