@@ -343,7 +343,10 @@ int dg_sgns_tuples_forward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint
  * dlogits[n * (1 + K)] when not NULL; row gradients dL/dW are summed per row
  * by a sorted-key segmented reduction in a fixed order (the result does not
  * depend on scheduling).  apply != 0 then updates W -= lr * dL/dW.  The
- * gradients stay readable through dg_sgns_grad_rows until the next backward. */
+ * gradients stay readable through dg_sgns_grad_rows until the next backward.
+ * n (K + 2) is at most 2^30 - 1 (DG_ERR_ARG beyond).  Should the sort's
+ * bounded waits fail (a fault of the device, not of the input), the call
+ * returns DG_ERR_HIP and leaves the tables unchanged. */
 int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16,
                             const float* dlogits, float lr, int apply);
 /* The forward and the backward with the loss's own gradient in one call:
@@ -353,7 +356,8 @@ int dg_sgns_tuples_backward(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uin
  * training loops whose loss is the tuple loss itself.  The reference has no
  * separate forward and backward: its skip_gram_kernel takes this step in
  * place, pair by pair (src/word2vec.cu:359-396).  logits[n * (1 + K)] and
- * loss may be NULL; the gradients stay readable through dg_sgns_grad_rows. */
+ * loss may be NULL; the gradients stay readable through dg_sgns_grad_rows.
+ * Limits and errors are those of the backward. */
 int dg_sgns_tuples_step(dg_ctx* ctx, const uint32_t* tuples, uint64_t n, uint32_t K, int bf16, float lr,
                         int apply, float* logits, double* loss);
 /* The forward without any allocation, for callers that cannot allocate
