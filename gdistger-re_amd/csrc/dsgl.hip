@@ -74,6 +74,19 @@ __device__ __forceinline__ float load_l2(const float* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Waits until every memory operation this wavefront has issued is complete:
+// its stores and atomics have reached the level-2 cache of its XCD, where
+// the level-2 loads above read them.  The rows a wavefront reads back this
+// way are its own scratch rows (and, in the deterministic order, every row),
+// so an agent-scope release, which also writes the XCD's level-2 cache back
+// to memory, is not needed; it made each step 38% slower.
+__device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Rows whose loads are issued together before any of them is used: the
+// kernel runs at one or two wavefronts per SIMD (its LDS sets that), so a
+// row at a time would expose one memory round trip per row.
+constexpr int kRowBatch = 8;
+
 __device__ __forceinline__ uint64_t uni64(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -335,13 +348,24 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
           }
           nwords[s] = nwd;
           // live rows in LDS, snapshots in scratch (dsgl.hpp:942-945)
-          for (int u = 0; u < nwd; ++u) {
-            const float* src = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+          for (int u0 = 0; u0 < nwd; u0 += kRowBatch) {
+            float v[kRowBatch][PER];
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-              const float v = src[lane + 64 * j];
-              L.V[(s * lcap + u) * LDV + lane + 64 * j] = v;
-              snap[((size_t)s * lcap + u) * DIM + lane + 64 * j] = v;
+            for (int i = 0; i < kRowBatch; ++i) {
+              if (u0 + i >= nwd) break;
+              const float* src = a.syn0 + (uint64_t)L.word[s * lcap + u0 + i] * DIM;
+#pragma unroll
+              for (int j = 0; j < PER; ++j) v[i][j] = src[lane + 64 * j];
+            }
+#pragma unroll
+            for (int i = 0; i < kRowBatch; ++i) {
+              const int u = u0 + i;
+              if (u >= nwd) break;
+#pragma unroll
+              for (int j = 0; j < PER; ++j) {
+                L.V[(s * lcap + u) * LDV + lane + 64 * j] = v[i][j];
+                snap[((size_t)s * lcap + u) * DIM + lane + 64 * j] = v[i][j];
+              }
             }
           }
           wave_sync();
@@ -393,16 +417,27 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
           wave_sync();
         }
         nneg = nd_first;
-        for (int k = 0; k < nneg; ++k) {
-          const float* src = a.syn1 + (uint64_t)L.nid[k] * DIM;
+        for (int k0 = 0; k0 < nneg; k0 += kRowBatch) {
+          float v[kRowBatch][PER];
 #pragma unroll
-          for (int j = 0; j < PER; ++j) {
-            const float v = src[lane + 64 * j];
-            nsnap[(size_t)k * DIM + lane + 64 * j] = v;
-            nlive[(size_t)k * DIM + lane + 64 * j] = v;
+          for (int i = 0; i < kRowBatch; ++i) {
+            if (k0 + i >= nneg) break;
+            const float* src = a.syn1 + (uint64_t)L.nid[k0 + i] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) v[i][j] = src[lane + 64 * j];
+          }
+#pragma unroll
+          for (int i = 0; i < kRowBatch; ++i) {
+            const size_t k = (size_t)(k0 + i);
+            if (k0 + i >= nneg) break;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              nsnap[k * DIM + lane + 64 * j] = v[i][j];
+              nlive[k * DIM + lane + 64 * j] = v[i][j];
+            }
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        drain();
         ++batch;
         change_neg = 0;
       }
@@ -469,20 +504,27 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
         L.out_meta[O - 1] = 1;
       }
       wave_sync();
-      // output rows: the targets from syn1, the negatives from the batch buffer
-      for (int r = 0; r < 16; ++r) {
-        float* dst = L.Y + r * LDV;
-        if (r < O) {
-          const bool tab = (r == 0 || r == O - 1);
-          const float* src = tab ? a.syn1 + (uint64_t)L.out_id[r] * DIM
-                                 : nlive + (size_t)L.out_slot[r] * DIM;
-          // from the level-2 cache: the rows this wavefront's atomics moved
+      // output rows: the targets from syn1, the negatives from the batch
+      // buffer, from the level-2 cache (the rows this wavefront's atomics
+      // moved); every row's loads are issued before the first is stored
+      {
+        float yv[16][PER];
 #pragma unroll
-          for (int j = 0; j < PER; ++j) dst[lane + 64 * j] = load_l2(src + lane + 64 * j);
-        } else {
+        for (int r = 0; r < 16; ++r) {
 #pragma unroll
-          for (int j = 0; j < PER; ++j) dst[lane + 64 * j] = 0.0f;
+          for (int j = 0; j < PER; ++j) yv[r][j] = 0.0f;
+          if (r < O) {
+            const bool tab = (r == 0 || r == O - 1);
+            const float* src = tab ? a.syn1 + (uint64_t)L.out_id[r] * DIM
+                                   : nlive + (size_t)L.out_slot[r] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) yv[r][j] = load_l2(src + lane + 64 * j);
+          }
         }
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+#pragma unroll
+          for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
       }
       wave_sync();
       // G = Y X^T and g (dsgl.hpp:1241-1273)
@@ -560,29 +602,54 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
             if (e < E) L.V[L.ent_row[e] * LDV + 16 * tt + l16] += acc[tt][x];
           }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      drain();
       wave_sync();
       // ---- finished sentences and batch back to the tables (dsgl.hpp:1358-1404)
       for (int s = 0; s < 2; ++s) {
         if (++pos[s] < len[s]) continue;
-        for (int u = 0; u < nwords[s]; ++u) {
-          float* dst = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+        for (int u0 = 0; u0 < nwords[s]; u0 += kRowBatch) {
+          float sv[kRowBatch][PER];
 #pragma unroll
-          for (int j = 0; j < PER; ++j) {
-            const int d = lane + 64 * j;
-            atomicAdd(dst + d, L.V[(s * lcap + u) * LDV + d] - load_l2(snap + ((size_t)s * lcap + u) * DIM + d));
+          for (int i = 0; i < kRowBatch; ++i) {
+            if (u0 + i >= nwords[s]) break;
+#pragma unroll
+            for (int j = 0; j < PER; ++j)
+              sv[i][j] = load_l2(snap + ((size_t)s * lcap + u0 + i) * DIM + lane + 64 * j);
+          }
+#pragma unroll
+          for (int i = 0; i < kRowBatch; ++i) {
+            const int u = u0 + i;
+            if (u >= nwords[s]) break;
+            float* dst = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              const int d = lane + 64 * j;
+              atomicAdd(dst + d, L.V[(s * lcap + u) * LDV + d] - sv[i][j]);
+            }
           }
         }
         len[s] = 0;
         ++tsn;
       }
       if (tsn >= 2) {
-        for (int k = 0; k < nneg; ++k) {
-          float* dst = a.syn1 + (uint64_t)L.nid[k] * DIM;
+        for (int k0 = 0; k0 < nneg; k0 += kRowBatch) {
+          float lv[kRowBatch][PER], sv[kRowBatch][PER];
 #pragma unroll
-          for (int j = 0; j < PER; ++j) {
-            const int d = lane + 64 * j;
-            atomicAdd(dst + d, load_l2(nlive + (size_t)k * DIM + d) - load_l2(nsnap + (size_t)k * DIM + d));
+          for (int i = 0; i < kRowBatch; ++i) {
+            if (k0 + i >= nneg) break;
+            const size_t k = (size_t)(k0 + i);
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              lv[i][j] = load_l2(nlive + k * DIM + lane + 64 * j);
+              sv[i][j] = load_l2(nsnap + k * DIM + lane + 64 * j);
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < kRowBatch; ++i) {
+            if (k0 + i >= nneg) break;
+            float* dst = a.syn1 + (uint64_t)L.nid[k0 + i] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) atomicAdd(dst + lane + 64 * j, lv[i][j] - sv[i][j]);
           }
         }
         change_neg = 1;
