@@ -527,80 +527,114 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
           for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
       }
       wave_sync();
-      // G = Y X^T and g (dsgl.hpp:1241-1273)
+      // LDS offsets (from V) of the context rows: xg[et] the row lane l16
+      // supplies to G's tile et, xo[ec][x] the row 16 ec + 4 g4 + x (the k
+      // index of dY's tile ec, the row of dX's output); the zero row past E.
+      // The tiles' products are unrolled to kET (E <= kEntCap) so that their
+      // MFMA chains and LDS reads interleave.
+      constexpr int kET = kEntCap / 16;
       const int ET = (E + 15) >> 4;
-      for (int et = 0; et < ET; ++et) {
+      const int zoff = (int)(L.Z - L.V);
+      int xg[kET], xo[kET][4];
+#pragma unroll
+      for (int et = 0; et < kET; ++et) {
         const int e = 16 * et + l16;
-        const float* xr = e < E ? L.V + L.ent_row[e] * LDV : L.Z;
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        xg[et] = e < E ? L.ent_row[e] * LDV : zoff;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int e4 = 16 * et + 4 * g4 + x;
+          xo[et][x] = e4 < E ? L.ent_row[e4] * LDV : zoff;
+        }
+      }
+      // G = Y X^T and g (dsgl.hpp:1241-1273)
+      {
+        f32x4_t acc[kET];
+#pragma unroll
+        for (int et = 0; et < kET; ++et) acc[et] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kc = 0; kc < NT; ++kc) {
           const float4 ya = *reinterpret_cast<const float4*>(L.Y + l16 * LDV + 16 * kc + 4 * g4);
-          const float4 xb = *reinterpret_cast<const float4*>(xr + 16 * kc + 4 * g4);
-          acc = mma4(ya, xb, acc);
+#pragma unroll
+          for (int et = 0; et < kET; ++et) {
+            if (et >= ET) break;
+            const float4 xb = *reinterpret_cast<const float4*>(L.V + xg[et] + 16 * kc + 4 * g4);
+            acc[et] = mma4(ya, xb, acc[et]);
+          }
         }
 #pragma unroll
-        for (int x = 0; x < 4; ++x) {
-          const int i = 4 * g4 + x;
-          float g = 0.0f;
-          if (i < O && e < E) {
-            const bool s1 = e < E1;
-            const float label = ((i == 0 && s1) || (i == O - 1 && !s1)) ? 1.0f : 0.0f;
-            g = (label - dsgl_sig(acc[x], a.expt, a.exact)) * alpha * (float)L.out_meta[i] *
-                (float)L.ent_cnt[e];
+        for (int et = 0; et < kET; ++et) {
+          if (et >= ET) break;
+          const int e = 16 * et + l16;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const int i = 4 * g4 + x;
+            float g = 0.0f;
+            if (i < O && e < E) {
+              const bool s1 = e < E1;
+              const float label = ((i == 0 && s1) || (i == O - 1 && !s1)) ? 1.0f : 0.0f;
+              g = (label - dsgl_sig(acc[et][x], a.expt, a.exact)) * alpha * (float)L.out_meta[i] *
+                  (float)L.ent_cnt[e];
+            }
+            L.G[i * kGL + e] = g;
           }
-          L.G[i * kGL + e] = g;
         }
       }
       wave_sync();
-      // dY = G X: lane holds dY[4 g4 + x][16 tt + l16]
-      for (int tt = 0; tt < NT; ++tt) {
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-        for (int ec = 0; ec < ET; ++ec) {
-          const float4 ga = *reinterpret_cast<const float4*>(L.G + l16 * kGL + 16 * ec + 4 * g4);
-          float xv[4];
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            const int e = 16 * ec + 4 * g4 + x;
-            xv[x] = (e < E ? L.V + L.ent_row[e] * LDV : L.Z)[16 * tt + l16];
-          }
-          acc = mma4(ga, make_float4(xv[0], xv[1], xv[2], xv[3]), acc);
-        }
-        const int d = 16 * tt + l16;
+      // dY = G X: lane holds dY[4 g4 + x][16 tt + l16], added to the row of
+      // output 4 g4 + x (the targets in syn1, the negatives in the batch buffer)
+      {
+        float* dst[4];
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
           const int i = 4 * g4 + x;
-          if (i >= O) continue;
-          if (i == 0)
-            atomicAdd(a.syn1 + (uint64_t)target * DIM + d, acc[x]);
-          else if (i == O - 1)
-            atomicAdd(a.syn1 + (uint64_t)target2 * DIM + d, acc[x]);
-          else
-            atomicAdd(nlive + (size_t)L.out_slot[i] * DIM + d, acc[x]);
+          dst[x] = i >= O ? nullptr
+                          : (i == 0 ? a.syn1 + (uint64_t)target * DIM
+                                    : (i == O - 1 ? a.syn1 + (uint64_t)target2 * DIM
+                                                  : nlive + (size_t)L.out_slot[i] * DIM));
         }
-      }
-      // dX = G^T Y into the sentence buffers: lane holds dX[16 et + 4 g4 + x][16 tt + l16]
-      for (int et = 0; et < ET; ++et) {
-        float ga[4];
-#pragma unroll
-        for (int x = 0; x < 4; ++x) ga[x] = L.G[(4 * g4 + x) * kGL + 16 * et + l16];
-        const float4 a4 = make_float4(ga[0], ga[1], ga[2], ga[3]);
         f32x4_t acc[NT];
 #pragma unroll
-        for (int tt = 0; tt < NT; ++tt) {
-          float yv[4];
+        for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int x = 0; x < 4; ++x) yv[x] = L.Y[(4 * g4 + x) * LDV + 16 * tt + l16];
-          acc[tt] = mma4(a4, make_float4(yv[0], yv[1], yv[2], yv[3]), f32x4_t{0.f, 0.f, 0.f, 0.f});
+        for (int ec = 0; ec < kET; ++ec) {
+          if (ec >= ET) break;
+          const float4 ga = *reinterpret_cast<const float4*>(L.G + l16 * kGL + 16 * ec + 4 * g4);
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) {
+            float xv[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) xv[x] = L.V[xo[ec][x] + 16 * tt + l16];
+            acc[tt] = mma4(ga, make_float4(xv[0], xv[1], xv[2], xv[3]), acc[tt]);
+          }
         }
-        wave_sync();
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            const int e = 16 * et + 4 * g4 + x;
-            if (e < E) L.V[L.ent_row[e] * LDV + 16 * tt + l16] += acc[tt][x];
-          }
+          for (int x = 0; x < 4; ++x)
+            if (dst[x]) atomicAdd(dst[x] + 16 * tt + l16, acc[tt][x]);
+      }
+      // dX = G^T Y into the sentence buffers: lane holds dX[16 et + 4 g4 + x][16 tt + l16]
+      {
+        float4 yb[NT];
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt)
+          yb[tt] = make_float4(L.Y[(4 * g4 + 0) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 1) * LDV + 16 * tt + l16],
+                               L.Y[(4 * g4 + 2) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 3) * LDV + 16 * tt + l16]);
+#pragma unroll
+        for (int et = 0; et < kET; ++et) {
+          if (et >= ET) break;
+          const float4 a4 = make_float4(L.G[(4 * g4 + 0) * kGL + 16 * et + l16], L.G[(4 * g4 + 1) * kGL + 16 * et + l16],
+                                        L.G[(4 * g4 + 2) * kGL + 16 * et + l16], L.G[(4 * g4 + 3) * kGL + 16 * et + l16]);
+          f32x4_t acc[NT];
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt) acc[tt] = mma4(a4, yb[tt], f32x4_t{0.f, 0.f, 0.f, 0.f});
+          wave_sync();
+#pragma unroll
+          for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+              if (16 * et + 4 * g4 + x < E) L.V[xo[et][x] + 16 * tt + l16] += acc[tt][x];
+        }
       }
       drain();
       wave_sync();
