@@ -124,8 +124,10 @@ struct DsglArgs {
   uint32_t window;
   uint32_t K;
   uint32_t workers;
-  uint32_t lcap;           // sentence tokens (and distinct words) one buffer holds
+  uint32_t lcap;           // sentence tokens one buffer holds
+  uint32_t vcap;           // distinct words one buffer holds (<= lcap)
   uint32_t ncap;           // negative draws of one batch at most (lcap * K)
+  uint32_t yrows;          // output rows held: 8 when K + 2 <= 8, else 16
   float alpha0;
   float total;             // (float)(iter * train_words + 1)
   uint64_t key;            // Philox key of the epoch
@@ -136,21 +138,23 @@ struct DsglArgs {
   unsigned long long* actual;  // per worker: counted words, carried across epochs
   float* alpha;            // per worker: learning rate, carried across epochs
   unsigned long long* kept;    // tokens kept by subsampling
+  uint32_t* err;           // set when a sentence holds more than vcap distinct words
 };
 
-// LDS of one wavefront (dynamic; sizes follow lcap, ncap and DIM).
+// LDS of one wavefront (dynamic; sizes follow lcap, vcap, ncap, yrows and
+// DIM).  Output rows past yrows, and their G rows, are read as the zero row Z.
 struct DsglLds {
-  float* V;       // [2][lcap][LDV] live sentence rows
-  float* Y;       // [16][LDV] the step's output rows
-  float* G;       // [16][kGL]
+  float* V;       // [2][vcap][LDV] live sentence rows
+  float* Y;       // [yrows][LDV] the step's output rows
+  float* G;       // [yrows][kGL]
   float* Z;       // [LDV] zeros
   int* sen;       // [2][lcap]
   int* slot;      // [2][lcap] word index of each position
-  int* word;      // [2][lcap] distinct words
+  int* word;      // [2][vcap] distinct words
   int* nword;     // [ncap] negative draws of the batch
   int* nslot;     // [ncap] batch-buffer index of each draw
   int* nid;       // [ncap] distinct negatives
-  int* ent_row;   // [kEntCap] row of each distinct context (s * lcap + word index)
+  int* ent_row;   // [kEntCap] row of each distinct context (s * vcap + word index)
   int* ent_cnt;   // [kEntCap]
   int* out_id;    // [16]
   int* out_meta;  // [16]
@@ -162,24 +166,25 @@ template <int DIM>
 __host__ __device__ constexpr int dsgl_ldv() { return DIM + 4; }
 
 template <int DIM>
-__host__ __device__ inline size_t dsgl_lds_bytes(uint32_t lcap, uint32_t ncap) {
+__host__ __device__ inline size_t dsgl_lds_bytes(uint32_t lcap, uint32_t vcap, uint32_t ncap,
+                                                 uint32_t yrows) {
   constexpr int LDV = dsgl_ldv<DIM>();
-  size_t f = (size_t)2 * lcap * LDV + 16 * LDV + 16 * kGL + LDV;
-  size_t i = (size_t)3 * 2 * lcap + 3 * (size_t)ncap + 2 * kEntCap + 3 * 16 + 64;
+  size_t f = (size_t)2 * vcap * LDV + (size_t)yrows * (LDV + kGL) + LDV;
+  size_t i = (size_t)2 * 2 * lcap + 2 * (size_t)vcap + 3 * (size_t)ncap + 2 * kEntCap + 3 * 16 + 64;
   return ((f * 4 + i * 4) + 15) & ~(size_t)15;
 }
 
 template <int DIM>
-__device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t ncap) {
+__device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows) {
   constexpr int LDV = dsgl_ldv<DIM>();
   DsglLds L;
   float* f = reinterpret_cast<float*>(base);
   L.V = f;
-  f += (size_t)2 * lcap * LDV;
+  f += (size_t)2 * vcap * LDV;
   L.Y = f;
-  f += 16 * LDV;
+  f += (size_t)yrows * LDV;
   L.G = f;
-  f += 16 * kGL;
+  f += (size_t)yrows * kGL;
   L.Z = f;
   f += LDV;
   int* p = reinterpret_cast<int*>(f);
@@ -188,7 +193,7 @@ __device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t ncap) {
   L.slot = p;
   p += 2 * lcap;
   L.word = p;
-  p += 2 * lcap;
+  p += 2 * vcap;
   L.nword = p;
   p += ncap;
   L.nslot = p;
@@ -216,15 +221,18 @@ __device__ __forceinline__ float dsgl_sig(float f, const float* expt, int exact)
   return expt[(int)((f + (float)kDsglMaxExp) * (float)(kDsglExpTable / kDsglMaxExp / 2))];
 }
 
+// Two wavefronts per SIMD up to dim 128 (at most 256 registers): the LDS of
+// a wavefront (about 28 KB at dim 128 on 22-token walks) fits five per CU.
 template <int DIM>
-__global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DIM <= 128 ? 2 : 1)))
+void dsgl_kernel(DsglArgs a) {
   extern __shared__ __attribute__((aligned(16))) char dsgl_smem[];
   constexpr int LDV = dsgl_ldv<DIM>();
   constexpr int NT = DIM / 16;  // dim tiles / k chunks
   constexpr int PER = DIM / 64; // dims per lane in row copies
   const int lane = threadIdx.x & 63;
   const int l16 = lane & 15, g4 = lane >> 4;
-  DsglLds L = dsgl_carve<DIM>(dsgl_smem, a.lcap, a.ncap);
+  DsglLds L = dsgl_carve<DIM>(dsgl_smem, a.lcap, a.vcap, a.ncap, a.yrows);
   for (int i = lane; i < LDV; i += 64) L.Z[i] = 0.0f;
   const uint64_t gw = blockIdx.x;
   uint64_t nw = gridDim.x;
@@ -236,7 +244,8 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
   float* nsnap = snap + (size_t)2 * a.lcap * DIM;          // [ncap][DIM]
   float* nlive = nsnap + (size_t)a.ncap * DIM;             // [ncap][DIM]
   const int W = (int)a.window, K = (int)a.K;
-  const uint32_t lcap = a.lcap;
+  const uint32_t lcap = a.lcap, vcap = a.vcap;
+  const int yr = (int)a.yrows;
   unsigned long long kept_total = 0;
 
   for (uint64_t t = gw; t < a.workers; t += nw) {
@@ -335,12 +344,18 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
             const int w = L.sen[s * lcap + i];
             int found = -1;
             for (int c0 = 0; c0 < nwd && found < 0; c0 += 64) {
-              const uint64_t m = __ballot(c0 + lane < nwd && L.word[s * lcap + c0 + lane] == w);
+              const uint64_t m = __ballot(c0 + lane < nwd && L.word[s * vcap + c0 + lane] == w);
               if (m) found = c0 + __builtin_ctzll(m);
+            }
+            if (found < 0 && nwd == (int)vcap) {
+              // more distinct words than the launch planned for: flagged
+              // (the host fails the call), never written past the buffer
+              if (lane == 0) *a.err = 1u;
+              found = 0;
             }
             if (found < 0) {
               found = nwd;
-              if (lane == 0) L.word[s * lcap + nwd] = w;
+              if (lane == 0) L.word[s * vcap + nwd] = w;
               ++nwd;
             }
             if (lane == 0) L.slot[s * lcap + i] = found;
@@ -353,7 +368,7 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
 #pragma unroll
             for (int i = 0; i < kRowBatch; ++i) {
               if (u0 + i >= nwd) break;
-              const float* src = a.syn0 + (uint64_t)L.word[s * lcap + u0 + i] * DIM;
+              const float* src = a.syn0 + (uint64_t)L.word[s * vcap + u0 + i] * DIM;
 #pragma unroll
               for (int j = 0; j < PER; ++j) v[i][j] = src[lane + 64 * j];
             }
@@ -363,7 +378,7 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
               if (u >= nwd) break;
 #pragma unroll
               for (int j = 0; j < PER; ++j) {
-                L.V[(s * lcap + u) * LDV + lane + 64 * j] = v[i][j];
+                L.V[(s * vcap + u) * LDV + lane + 64 * j] = v[i][j];
                 snap[((size_t)s * lcap + u) * DIM + lane + 64 * j] = v[i][j];
               }
             }
@@ -469,7 +484,7 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
       const int E1 = __popcll(em & 0xffffffffull);
       if (first) {
         const int e = (int)lanes_below(em);
-        L.ent_row[e] = hs * (int)lcap + my;
+        L.ent_row[e] = hs * (int)vcap + my;
         L.ent_cnt[e] = cnt;
       }
       // outputs: [target, distinct negatives with multiplicity, target2]
@@ -523,8 +538,9 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < 16; ++r)
+          if (r < yr)  // rows O .. yr - 1 zero
 #pragma unroll
-          for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
+            for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
       }
       wave_sync();
       // LDS offsets (from V) of the context rows: xg[et] the row lane l16
@@ -551,9 +567,10 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
         f32x4_t acc[kET];
 #pragma unroll
         for (int et = 0; et < kET; ++et) acc[et] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        const float* yrow = l16 < yr ? L.Y + l16 * LDV : L.Z;
 #pragma unroll
         for (int kc = 0; kc < NT; ++kc) {
-          const float4 ya = *reinterpret_cast<const float4*>(L.Y + l16 * LDV + 16 * kc + 4 * g4);
+          const float4 ya = *reinterpret_cast<const float4*>(yrow + 16 * kc + 4 * g4);
 #pragma unroll
           for (int et = 0; et < kET; ++et) {
             if (et >= ET) break;
@@ -575,7 +592,7 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
               g = (label - dsgl_sig(acc[et][x], a.expt, a.exact)) * alpha * (float)L.out_meta[i] *
                   (float)L.ent_cnt[e];
             }
-            L.G[i * kGL + e] = g;
+            if (i < yr) L.G[i * kGL + e] = g;
           }
         }
       }
@@ -595,10 +612,11 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
         f32x4_t acc[NT];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        const float* grow = l16 < yr ? L.G + l16 * kGL : L.Z;
 #pragma unroll
         for (int ec = 0; ec < kET; ++ec) {
           if (ec >= ET) break;
-          const float4 ga = *reinterpret_cast<const float4*>(L.G + l16 * kGL + 16 * ec + 4 * g4);
+          const float4 ga = *reinterpret_cast<const float4*>(grow + 16 * ec + 4 * g4);
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) {
             float xv[4];
@@ -615,16 +633,19 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
       }
       // dX = G^T Y into the sentence buffers: lane holds dX[16 et + 4 g4 + x][16 tt + l16]
       {
+        const bool yin = 4 * g4 < yr;  // this lane's k rows 4 g4 .. 4 g4 + 3 are held
         float4 yb[NT];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt)
-          yb[tt] = make_float4(L.Y[(4 * g4 + 0) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 1) * LDV + 16 * tt + l16],
-                               L.Y[(4 * g4 + 2) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 3) * LDV + 16 * tt + l16]);
+          yb[tt] = yin ? make_float4(L.Y[(4 * g4 + 0) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 1) * LDV + 16 * tt + l16],
+                                     L.Y[(4 * g4 + 2) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 3) * LDV + 16 * tt + l16])
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int et = 0; et < kET; ++et) {
           if (et >= ET) break;
-          const float4 a4 = make_float4(L.G[(4 * g4 + 0) * kGL + 16 * et + l16], L.G[(4 * g4 + 1) * kGL + 16 * et + l16],
-                                        L.G[(4 * g4 + 2) * kGL + 16 * et + l16], L.G[(4 * g4 + 3) * kGL + 16 * et + l16]);
+          const float4 a4 = yin ? make_float4(L.G[(4 * g4 + 0) * kGL + 16 * et + l16], L.G[(4 * g4 + 1) * kGL + 16 * et + l16],
+                                              L.G[(4 * g4 + 2) * kGL + 16 * et + l16], L.G[(4 * g4 + 3) * kGL + 16 * et + l16])
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
           f32x4_t acc[NT];
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) acc[tt] = mma4(a4, yb[tt], f32x4_t{0.f, 0.f, 0.f, 0.f});
@@ -654,11 +675,11 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
           for (int i = 0; i < kRowBatch; ++i) {
             const int u = u0 + i;
             if (u >= nwords[s]) break;
-            float* dst = a.syn0 + (uint64_t)L.word[s * lcap + u] * DIM;
+            float* dst = a.syn0 + (uint64_t)L.word[s * vcap + u] * DIM;
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
               const int d = lane + 64 * j;
-              atomicAdd(dst + d, L.V[(s * lcap + u) * LDV + d] - sv[i][j]);
+              atomicAdd(dst + d, L.V[(s * vcap + u) * LDV + d] - sv[i][j]);
             }
           }
         }
@@ -699,6 +720,43 @@ __global__ __launch_bounds__(64) void dsgl_kernel(DsglArgs a) {
   if (lane == 0 && kept_total) atomicAdd(a.kept, kept_total);
 }
 
+// The most distinct vertices any walk holds (out[0], by atomicMax): a
+// sentence is a run of one walk's kept tokens, so its distinct words, which
+// the sentence buffers hold, are at most this.  One wavefront per walk, 64
+// positions at a time; position p counts when no earlier position of its
+// walk holds the same vertex.
+__global__ __launch_bounds__(256) void walk_distinct_kernel(const uint32_t* tok, const uint64_t* off,
+                                                            uint64_t n_walks, uint32_t* out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  uint32_t best = 0;
+  for (uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n_walks; w += nw) {
+    const uint64_t b = uni64(off[w]);
+    const uint64_t L = uni64(off[w + 1]) - b;
+    uint32_t cnt = 0;
+    for (uint64_t a0 = 0; a0 < L; a0 += 64) {
+      const bool va = a0 + (uint64_t)lane < L;
+      const uint32_t x = va ? tok[b + a0 + lane] : 0u;
+      bool first = va;
+      for (uint64_t b0 = 0; b0 <= a0; b0 += 64) {
+        const uint32_t y = b0 == a0 ? x : tok[b + b0 + lane];  // b0 < a0: a whole earlier chunk
+        const int jn = (int)(L - b0 < 64 ? L - b0 : 64);
+        for (int j = 0; j < jn; ++j) {
+          const uint32_t yj = (uint32_t)__shfl((int)y, j, 64);
+          if (b0 + (uint64_t)j < a0 + (uint64_t)lane && yj == x) first = false;
+        }
+      }
+      cnt += (uint32_t)__popcll(__ballot(first));
+    }
+    best = cnt > best ? cnt : best;
+  }
+  if (lane == 0 && best) atomicMax(out, best);
+}
+
+__global__ void dsgl_flags_init_kernel(uint32_t* flags) {
+  if (threadIdx.x < 2) flags[threadIdx.x] = 0u;
+}
+
 __global__ void dsgl_state_init_kernel(unsigned long long* actual, float* alpha, uint32_t n,
                                        float alpha0, unsigned long long* kept) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -720,8 +778,9 @@ int launch_dims(dg_ctx* ctx, const DsglArgs& a, size_t lds, unsigned grid) {
 }
 
 template <int DIM>
-int plan_dims(dg_ctx* ctx, uint32_t lcap, uint32_t ncap, size_t* lds, unsigned* per_cu) {
-  *lds = dsgl_lds_bytes<DIM>(lcap, ncap);
+int plan_dims(dg_ctx* ctx, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows, size_t* lds,
+              unsigned* per_cu) {
+  *lds = dsgl_lds_bytes<DIM>(lcap, vcap, ncap, yrows);
   if (*lds > 160 * 1024)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED,
                    "DSGL form: sentences of %u tokens at dim %d need %zu bytes of LDS per wavefront "
@@ -745,6 +804,7 @@ struct dg_dsgl_state {
   float* alpha = nullptr;
   size_t alpha_cap = 0;
   unsigned long long* kept = nullptr;
+  uint32_t* flags = nullptr;  // [0] the most distinct vertices of a walk, [1] the kernel's overflow flag
 };
 
 void dg_dsgl_free(dg_ctx* ctx) {
@@ -754,6 +814,7 @@ void dg_dsgl_free(dg_ctx* ctx) {
   if (s->actual) hipFree(s->actual);
   if (s->alpha) hipFree(s->alpha);
   if (s->kept) hipFree(s->kept);
+  if (s->flags) hipFree(s->flags);
   delete s;
   ctx->dsgl = nullptr;
 }
@@ -766,12 +827,32 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   const uint32_t lcap = c.max_walk < (uint64_t)kDsglMaxSentence ? (uint32_t)(c.max_walk ? c.max_walk : 1)
                                                                  : (uint32_t)kDsglMaxSentence;
   const uint32_t ncap = lcap * (c.K ? c.K : 1);
+  // the sentence buffers hold a walk's distinct vertices, fewer than its
+  // positions when walks revisit vertices (12.3 of 22 on average for the
+  // 2.2M-vertex Chung-Lu walks, 18 at most): the smaller LDS footprint fits
+  // more wavefronts per CU
+  if (!st->flags) DG_HIP(ctx, hipMalloc((void**)&st->flags, 2 * sizeof(uint32_t)));
+  size_t tk;
+  DG_CHECK(dg_time_begin(ctx, "sgns_dsgl", &tk));
+  dsgl_flags_init_kernel<<<1, 64, 0, ctx->stream>>>(st->flags);
+  {
+    const uint64_t waves = std::min<uint64_t>(c.n_walks, (uint64_t)ctx->num_cus * 32);
+    walk_distinct_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, ctx->stream>>>(c.tok, c.off, c.n_walks,
+                                                                              st->flags);
+  }
+  DG_HIP(ctx, hipGetLastError());
+  DG_CHECK(dg_time_end(ctx, tk));
+  uint32_t most = 0;
+  DG_HIP(ctx, hipMemcpyAsync(&most, st->flags, sizeof(most), hipMemcpyDeviceToHost, ctx->stream));
+  DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const uint32_t vcap = std::max<uint32_t>(1u, std::min<uint32_t>(lcap, most));
+  const uint32_t yrows = c.K + 2 <= 8 ? 8u : 16u;
   size_t lds = 0;
   unsigned per_cu = 1;
   switch (c.dim) {
-    case 64: DG_CHECK(plan_dims<64>(ctx, lcap, ncap, &lds, &per_cu)); break;
-    case 128: DG_CHECK(plan_dims<128>(ctx, lcap, ncap, &lds, &per_cu)); break;
-    default: DG_CHECK(plan_dims<256>(ctx, lcap, ncap, &lds, &per_cu)); break;
+    case 64: DG_CHECK(plan_dims<64>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
+    case 128: DG_CHECK(plan_dims<128>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
+    default: DG_CHECK(plan_dims<256>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
   }
   const uint64_t resident = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
   uint32_t workers = c.workers ? c.workers : (uint32_t)std::min<uint64_t>(resident, 1u << 20);
@@ -803,7 +884,10 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   a.K = c.K;
   a.workers = workers;
   a.lcap = lcap;
+  a.vcap = vcap;
   a.ncap = ncap;
+  a.yrows = yrows;
+  a.err = st->flags + 1;
   a.alpha0 = c.alpha0;
   a.total = (float)((uint64_t)c.iter * c.train_words + 1);
   a.exact = c.exact;
@@ -816,7 +900,6 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   double total_ms = 0.0;
   for (uint32_t ep = 0; ep < c.iter; ++ep) {
     a.key = dg_epoch_key(c.seed, c.epoch0 + ep);
-    size_t tk;
     DG_CHECK(dg_time_begin(ctx, "sgns_dsgl", &tk));
     switch (c.dim) {
       case 64: DG_CHECK(launch_dims<64>(ctx, a, lds, grid)); break;
@@ -833,6 +916,12 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
       total_ms += m;
     }
   }
+  uint32_t over = 0;
+  DG_HIP(ctx, hipMemcpy(&over, st->flags + 1, sizeof(over), hipMemcpyDeviceToHost));
+  if (over)
+    return dg_fail(ctx, DG_ERR_STATE,
+                   "DSGL form: a sentence held more than %u distinct words (the corpus changed during the call?)",
+                   vcap);
   unsigned long long k = 0;
   DG_HIP(ctx, hipMemcpy(&k, st->kept, sizeof(k), hipMemcpyDeviceToHost));
   *kept = k;
