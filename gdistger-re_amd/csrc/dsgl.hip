@@ -367,7 +367,7 @@ void dsgl_kernel(DsglArgs a) {
             float v[kRowBatch][PER];
 #pragma unroll
             for (int i = 0; i < kRowBatch; ++i) {
-              if (u0 + i >= nwd) break;
+              if (u0 + i >= nwd) continue;
               const float* src = a.syn0 + (uint64_t)L.word[s * vcap + u0 + i] * DIM;
 #pragma unroll
               for (int j = 0; j < PER; ++j) v[i][j] = src[lane + 64 * j];
@@ -375,7 +375,7 @@ void dsgl_kernel(DsglArgs a) {
 #pragma unroll
             for (int i = 0; i < kRowBatch; ++i) {
               const int u = u0 + i;
-              if (u >= nwd) break;
+              if (u >= nwd) continue;
 #pragma unroll
               for (int j = 0; j < PER; ++j) {
                 L.V[(s * vcap + u) * LDV + lane + 64 * j] = v[i][j];
@@ -436,7 +436,7 @@ void dsgl_kernel(DsglArgs a) {
           float v[kRowBatch][PER];
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
-            if (k0 + i >= nneg) break;
+            if (k0 + i >= nneg) continue;
             const float* src = a.syn1 + (uint64_t)L.nid[k0 + i] * DIM;
 #pragma unroll
             for (int j = 0; j < PER; ++j) v[i][j] = src[lane + 64 * j];
@@ -444,7 +444,7 @@ void dsgl_kernel(DsglArgs a) {
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
             const size_t k = (size_t)(k0 + i);
-            if (k0 + i >= nneg) break;
+            if (k0 + i >= nneg) continue;
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
               nsnap[k * DIM + lane + 64 * j] = v[i][j];
@@ -459,34 +459,6 @@ void dsgl_kernel(DsglArgs a) {
       // ---- one step: both positions (dsgl.hpp:1085-1356)
       const int target = uni(L.sen[pos[0]]);
       const int target2 = uni(L.sen[lcap + pos[1]]);
-      const int b = (int)(draw_word(a.key, (uint32_t)t, step, DG_TAG_DSGL_WIN, 0) % (uint32_t)W);
-      ++step;
-      // contexts: lanes 0..31 the first window, 32..63 the second
-      const int hs = lane >> 5, li = lane & 31;
-      const int span = 2 * (W - b);
-      int my = -1;
-      {
-        const int i = b + li + (b + li >= W ? 1 : 0);
-        const int c = pos[hs] - W + i;
-        if (li < span && c >= 0 && c < len[hs]) my = L.slot[hs * lcap + c];
-      }
-      L.tmp[lane] = my;
-      wave_sync();
-      bool first = my >= 0;
-      int cnt = 0;
-      for (int j = 0; j < 32; ++j) {
-        const int o = L.tmp[hs * 32 + j];
-        cnt += (my >= 0 && o == my) ? 1 : 0;
-        if (j < li && o == my) first = false;
-      }
-      const uint64_t em = __ballot(first);
-      const int E = __popcll(em);
-      const int E1 = __popcll(em & 0xffffffffull);
-      if (first) {
-        const int e = (int)lanes_below(em);
-        L.ent_row[e] = hs * (int)vcap + my;
-        L.ent_cnt[e] = cnt;
-      }
       // outputs: [target, distinct negatives with multiplicity, target2]
       ++neg_count;
       const int nst = (int)(neg_count % (uint32_t)nbn);
@@ -521,26 +493,72 @@ void dsgl_kernel(DsglArgs a) {
       wave_sync();
       // output rows: the targets from syn1, the negatives from the batch
       // buffer, from the level-2 cache (the rows this wavefront's atomics
-      // moved); every row's loads are issued before the first is stored
+      // moved); the first eight rows' loads are in flight while the contexts
+      // are found
+      float yv[8][PER];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) yv[r][j] = 0.0f;
+        if (r < O) {
+          const bool tab = (r == 0 || r == O - 1);
+          const float* src = tab ? a.syn1 + (uint64_t)L.out_id[r] * DIM
+                                 : nlive + (size_t)L.out_slot[r] * DIM;
+#pragma unroll
+          for (int j = 0; j < PER; ++j) yv[r][j] = load_l2(src + lane + 64 * j);
+        }
+      }
+      const int b = (int)(draw_word(a.key, (uint32_t)t, step, DG_TAG_DSGL_WIN, 0) % (uint32_t)W);
+      ++step;
+      // contexts: lanes 0..31 the first window, 32..63 the second
+      const int hs = lane >> 5, li = lane & 31;
+      const int span = 2 * (W - b);
+      int my = -1;
       {
-        float yv[16][PER];
+        const int i = b + li + (b + li >= W ? 1 : 0);
+        const int c = pos[hs] - W + i;
+        if (li < span && c >= 0 && c < len[hs]) my = L.slot[hs * lcap + c];
+      }
+      L.tmp[lane] = my;
+      wave_sync();
+      bool first = my >= 0;
+      int cnt = 0;
+      for (int j = 0; j < 32; ++j) {
+        const int o = L.tmp[hs * 32 + j];
+        cnt += (my >= 0 && o == my) ? 1 : 0;
+        if (j < li && o == my) first = false;
+      }
+      const uint64_t em = __ballot(first);
+      const int E = __popcll(em);
+      const int E1 = __popcll(em & 0xffffffffull);
+      if (first) {
+        const int e = (int)lanes_below(em);
+        L.ent_row[e] = hs * (int)vcap + my;
+        L.ent_cnt[e] = cnt;
+      }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < 8; ++r)
 #pragma unroll
-          for (int j = 0; j < PER; ++j) yv[r][j] = 0.0f;
-          if (r < O) {
-            const bool tab = (r == 0 || r == O - 1);
-            const float* src = tab ? a.syn1 + (uint64_t)L.out_id[r] * DIM
-                                   : nlive + (size_t)L.out_slot[r] * DIM;
+        for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
+      if (yr > 8) {
+        // more than six negatives: rows 8 .. 15
+        float yw[8][PER];
 #pragma unroll
-            for (int j = 0; j < PER; ++j) yv[r][j] = load_l2(src + lane + 64 * j);
+        for (int r = 0; r < 8; ++r) {
+#pragma unroll
+          for (int j = 0; j < PER; ++j) yw[r][j] = 0.0f;
+          if (8 + r < O) {
+            const bool tab = 8 + r == O - 1;
+            const float* src = tab ? a.syn1 + (uint64_t)L.out_id[8 + r] * DIM
+                                   : nlive + (size_t)L.out_slot[8 + r] * DIM;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) yw[r][j] = load_l2(src + lane + 64 * j);
           }
         }
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-          if (r < yr)  // rows O .. yr - 1 zero
+        for (int r = 0; r < 8; ++r)
 #pragma unroll
-            for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
+          for (int j = 0; j < PER; ++j) L.Y[(8 + r) * LDV + lane + 64 * j] = yw[r][j];
       }
       wave_sync();
       // LDS offsets (from V) of the context rows: xg[et] the row lane l16
@@ -666,7 +684,7 @@ void dsgl_kernel(DsglArgs a) {
           float sv[kRowBatch][PER];
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
-            if (u0 + i >= nwords[s]) break;
+            if (u0 + i >= nwords[s]) continue;
 #pragma unroll
             for (int j = 0; j < PER; ++j)
               sv[i][j] = load_l2(snap + ((size_t)s * lcap + u0 + i) * DIM + lane + 64 * j);
@@ -674,7 +692,7 @@ void dsgl_kernel(DsglArgs a) {
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
             const int u = u0 + i;
-            if (u >= nwords[s]) break;
+            if (u >= nwords[s]) continue;
             float* dst = a.syn0 + (uint64_t)L.word[s * vcap + u] * DIM;
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -691,7 +709,7 @@ void dsgl_kernel(DsglArgs a) {
           float lv[kRowBatch][PER], sv[kRowBatch][PER];
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
-            if (k0 + i >= nneg) break;
+            if (k0 + i >= nneg) continue;
             const size_t k = (size_t)(k0 + i);
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -701,7 +719,7 @@ void dsgl_kernel(DsglArgs a) {
           }
 #pragma unroll
           for (int i = 0; i < kRowBatch; ++i) {
-            if (k0 + i >= nneg) break;
+            if (k0 + i >= nneg) continue;
             float* dst = a.syn1 + (uint64_t)L.nid[k0 + i] * DIM;
 #pragma unroll
             for (int j = 0; j < PER; ++j) atomicAdd(dst + lane + 64 * j, lv[i][j] - sv[i][j]);

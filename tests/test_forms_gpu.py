@@ -63,25 +63,27 @@ def _dsgl_oracle(counts, dim, tok, off, window, K, alpha, workers, seed, exact):
     return o, init0
 
 
-@pytest.mark.parametrize("dim,workers", [(64, 1), (128, 7), (256, 3)])
-def test_dsgl_deterministic_vs_fp64_oracle(ctx, walked, dim, workers):
+@pytest.mark.parametrize("dim,workers,K,window", [(64, 1, 5, 10), (128, 7, 5, 10), (256, 3, 5, 10),
+                                                 (128, 2, 10, 5)])
+def test_dsgl_deterministic_vs_fp64_oracle(ctx, walked, dim, workers, K, window):
     """Exact sigmoid, every row the epoch moved within 1e-4 of the FP64
     restatement, relative to the largest update of its table (FP32 rows and
-    MFMA accumulation against FP64)."""
+    MFMA accumulation against FP64).  Ten negatives take the 16 output rows
+    of a step (up to six, eight rows are held)."""
     tok, off, counts, _ = walked
     p = _params(ctx, form=ffi.DG_SGNS_DSGL_FP32, dim=dim, exact_sigmoid=1, init_float_ops=1,
-                workers=workers)
+                workers=workers, negative=K, window=window)
     ctx.sgns_init(p)
     init0, _ = ctx.sgns_tables(dim)
     st = ctx.sgns_train(p, corpus=ffi.DG_CORPUS_ROUND)
     s0, s1 = ctx.sgns_tables(dim)
-    o, oinit0 = _dsgl_oracle(counts, dim, tok, off, 10, 5, 0.025, workers, 13, True)
+    o, oinit0 = _dsgl_oracle(counts, dim, tok, off, window, K, 0.025, workers, 13, True)
     assert np.array_equal(ctx.sgns_vocab(), o.vertex_of_rank)
     assert np.array_equal(init0, oinit0)
     d0 = o.syn0.astype(np.float64) - init0
     e0 = np.abs((s0.astype(np.float64) - init0) - d0).max() / np.abs(d0).max()
     e1 = np.abs(s1.astype(np.float64) - o.syn1).max() / np.abs(o.syn1).max()
-    log(f"DSGL dim {dim}, {workers} workers: {st.kept} kept of {st.words} words, max relative "
+    log(f"DSGL dim {dim}, {workers} workers, K {K}, window {window}: {st.kept} kept of {st.words} words, max relative "
         f"error syn0 {e0:.3e} syn1 {e1:.3e}; rate at the end {st.alpha_end:.6f}")
     assert np.abs(o.syn1).max() > 0 and np.abs(d0).max() > 0
     assert e0 < 1e-4 and e1 < 1e-4
