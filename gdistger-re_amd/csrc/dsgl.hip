@@ -51,7 +51,18 @@ constexpr int kDsglMaxSentence = 1000;  // MAX_SENTENCE_LENGTH (dsgl.hpp:23)
 constexpr int kDsglMaxExp = 6;
 constexpr int kDsglExpTable = 1000;
 constexpr int kEntCap = 64;        // distinct contexts of one step (2 windows x 2W <= 60)
-constexpr int kGL = kEntCap + 4;   // G row stride
+
+// G's row stride for window W: the step's distinct contexts are at most 4 W,
+// in tiles of 16, plus 4 floats against bank conflicts.
+__host__ __device__ inline uint32_t dsgl_gl(uint32_t window) { return 16u * ((4u * window + 15u) / 16u) + 4u; }
+
+// Floats of the G buffer: yrows rows of gl, and at least the 64 ints the
+// step's context scan and the negative de-duplication stage there (G is dead
+// while they run).
+__host__ __device__ inline size_t dsgl_g_floats(uint32_t yrows, uint32_t gl) {
+  const size_t n = (size_t)yrows * gl;
+  return n < 64 ? 64 : n;
+}
 
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
@@ -127,7 +138,8 @@ struct DsglArgs {
   uint32_t lcap;           // sentence tokens one buffer holds
   uint32_t vcap;           // distinct words one buffer holds (<= lcap)
   uint32_t ncap;           // negative draws of one batch at most (lcap * K)
-  uint32_t yrows;          // output rows held: 8 when K + 2 <= 8, else 16
+  uint32_t yrows;          // output rows held: K + 2
+  uint32_t gl;             // G row stride (dsgl_gl of the window)
   float alpha0;
   float total;             // (float)(iter * train_words + 1)
   uint64_t key;            // Philox key of the epoch
@@ -141,12 +153,13 @@ struct DsglArgs {
   uint32_t* err;           // set when a sentence holds more than vcap distinct words
 };
 
-// LDS of one wavefront (dynamic; sizes follow lcap, vcap, ncap, yrows and
-// DIM).  Output rows past yrows, and their G rows, are read as the zero row Z.
+// LDS of one wavefront (dynamic; sizes follow lcap, vcap, ncap, yrows, gl
+// and DIM).  Output rows past yrows, and their G rows, are read as the zero
+// row Z.
 struct DsglLds {
   float* V;       // [2][vcap][LDV] live sentence rows
   float* Y;       // [yrows][LDV] the step's output rows
-  float* G;       // [yrows][kGL]
+  float* G;       // [yrows][gl]
   float* Z;       // [LDV] zeros
   int* sen;       // [2][lcap]
   int* slot;      // [2][lcap] word index of each position
@@ -155,11 +168,11 @@ struct DsglLds {
   int* nslot;     // [ncap] batch-buffer index of each draw
   int* nid;       // [ncap] distinct negatives
   int* ent_row;   // [kEntCap] row of each distinct context (s * vcap + word index)
-  int* ent_cnt;   // [kEntCap]
   int* out_id;    // [16]
   int* out_meta;  // [16]
   int* out_slot;  // [16]
-  int* tmp;       // [64]
+  uint8_t* ent_cnt;  // [kEntCap] occurrences of each distinct context (<= 32)
+  int* tmp;       // [64], in G's storage
 };
 
 template <int DIM>
@@ -167,15 +180,16 @@ __host__ __device__ constexpr int dsgl_ldv() { return DIM + 4; }
 
 template <int DIM>
 __host__ __device__ inline size_t dsgl_lds_bytes(uint32_t lcap, uint32_t vcap, uint32_t ncap,
-                                                 uint32_t yrows) {
+                                                 uint32_t yrows, uint32_t gl) {
   constexpr int LDV = dsgl_ldv<DIM>();
-  size_t f = (size_t)2 * vcap * LDV + (size_t)yrows * (LDV + kGL) + LDV;
-  size_t i = (size_t)2 * 2 * lcap + 2 * (size_t)vcap + 3 * (size_t)ncap + 2 * kEntCap + 3 * 16 + 64;
-  return ((f * 4 + i * 4) + 15) & ~(size_t)15;
+  size_t f = (size_t)2 * vcap * LDV + (size_t)yrows * LDV + dsgl_g_floats(yrows, gl) + LDV;
+  size_t i = (size_t)2 * 2 * lcap + 2 * (size_t)vcap + 3 * (size_t)ncap + kEntCap + 3 * 16;
+  return ((f * 4 + i * 4 + kEntCap) + 15) & ~(size_t)15;
 }
 
 template <int DIM>
-__device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows) {
+__device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows,
+                              uint32_t gl) {
   constexpr int LDV = dsgl_ldv<DIM>();
   DsglLds L;
   float* f = reinterpret_cast<float*>(base);
@@ -184,7 +198,8 @@ __device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t vcap, uint32_t
   L.Y = f;
   f += (size_t)yrows * LDV;
   L.G = f;
-  f += (size_t)yrows * kGL;
+  L.tmp = reinterpret_cast<int*>(f);
+  f += dsgl_g_floats(yrows, gl);
   L.Z = f;
   f += LDV;
   int* p = reinterpret_cast<int*>(f);
@@ -202,15 +217,13 @@ __device__ DsglLds dsgl_carve(char* base, uint32_t lcap, uint32_t vcap, uint32_t
   p += ncap;
   L.ent_row = p;
   p += kEntCap;
-  L.ent_cnt = p;
-  p += kEntCap;
   L.out_id = p;
   p += 16;
   L.out_meta = p;
   p += 16;
   L.out_slot = p;
   p += 16;
-  L.tmp = p;
+  L.ent_cnt = reinterpret_cast<uint8_t*>(p);
   return L;
 }
 
@@ -222,7 +235,7 @@ __device__ __forceinline__ float dsgl_sig(float f, const float* expt, int exact)
 }
 
 // Two wavefronts per SIMD up to dim 128 (at most 256 registers): the LDS of
-// a wavefront (about 28 KB at dim 128 on 22-token walks) fits five per CU.
+// a wavefront (27 KB at dim 128 on 22-token walks) fits six per CU.
 template <int DIM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DIM <= 128 ? 2 : 1)))
 void dsgl_kernel(DsglArgs a) {
@@ -232,7 +245,7 @@ void dsgl_kernel(DsglArgs a) {
   constexpr int PER = DIM / 64; // dims per lane in row copies
   const int lane = threadIdx.x & 63;
   const int l16 = lane & 15, g4 = lane >> 4;
-  DsglLds L = dsgl_carve<DIM>(dsgl_smem, a.lcap, a.vcap, a.ncap, a.yrows);
+  DsglLds L = dsgl_carve<DIM>(dsgl_smem, a.lcap, a.vcap, a.ncap, a.yrows, a.gl);
   for (int i = lane; i < LDV; i += 64) L.Z[i] = 0.0f;
   const uint64_t gw = blockIdx.x;
   uint64_t nw = gridDim.x;
@@ -245,7 +258,7 @@ void dsgl_kernel(DsglArgs a) {
   float* nlive = nsnap + (size_t)a.ncap * DIM;             // [ncap][DIM]
   const int W = (int)a.window, K = (int)a.K;
   const uint32_t lcap = a.lcap, vcap = a.vcap;
-  const int yr = (int)a.yrows;
+  const int yr = (int)a.yrows, gl = (int)a.gl;
   unsigned long long kept_total = 0;
 
   for (uint64_t t = gw; t < a.workers; t += nw) {
@@ -534,12 +547,13 @@ void dsgl_kernel(DsglArgs a) {
       if (first) {
         const int e = (int)lanes_below(em);
         L.ent_row[e] = hs * (int)vcap + my;
-        L.ent_cnt[e] = cnt;
+        L.ent_cnt[e] = (uint8_t)cnt;
       }
 #pragma unroll
       for (int r = 0; r < 8; ++r)
+        if (r < yr)  // rows O .. yr - 1 zero
 #pragma unroll
-        for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
+          for (int j = 0; j < PER; ++j) L.Y[r * LDV + lane + 64 * j] = yv[r][j];
       if (yr > 8) {
         // more than six negatives: rows 8 .. 15
         float yw[8][PER];
@@ -557,8 +571,9 @@ void dsgl_kernel(DsglArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < 8; ++r)
+          if (8 + r < yr)
 #pragma unroll
-          for (int j = 0; j < PER; ++j) L.Y[(8 + r) * LDV + lane + 64 * j] = yw[r][j];
+            for (int j = 0; j < PER; ++j) L.Y[(8 + r) * LDV + lane + 64 * j] = yw[r][j];
       }
       wave_sync();
       // LDS offsets (from V) of the context rows: xg[et] the row lane l16
@@ -610,7 +625,7 @@ void dsgl_kernel(DsglArgs a) {
               g = (label - dsgl_sig(acc[et][x], a.expt, a.exact)) * alpha * (float)L.out_meta[i] *
                   (float)L.ent_cnt[e];
             }
-            if (i < yr) L.G[i * kGL + e] = g;
+            if (i < yr) L.G[i * gl + e] = g;
           }
         }
       }
@@ -630,7 +645,7 @@ void dsgl_kernel(DsglArgs a) {
         f32x4_t acc[NT];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-        const float* grow = l16 < yr ? L.G + l16 * kGL : L.Z;
+        const float* grow = l16 < yr ? L.G + l16 * gl : L.Z;
 #pragma unroll
         for (int ec = 0; ec < kET; ++ec) {
           if (ec >= ET) break;
@@ -651,19 +666,24 @@ void dsgl_kernel(DsglArgs a) {
       }
       // dX = G^T Y into the sentence buffers: lane holds dX[16 et + 4 g4 + x][16 tt + l16]
       {
-        const bool yin = 4 * g4 < yr;  // this lane's k rows 4 g4 .. 4 g4 + 3 are held
+        // this lane's k rows 4 g4 .. 4 g4 + 3, rows past yr read from the zero row
+        const float* yk[4];
+        const float* gk[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int i = 4 * g4 + x;
+          yk[x] = i < yr ? L.Y + i * LDV : L.Z;
+          gk[x] = i < yr ? L.G + i * gl : L.Z;
+        }
         float4 yb[NT];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt)
-          yb[tt] = yin ? make_float4(L.Y[(4 * g4 + 0) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 1) * LDV + 16 * tt + l16],
-                                     L.Y[(4 * g4 + 2) * LDV + 16 * tt + l16], L.Y[(4 * g4 + 3) * LDV + 16 * tt + l16])
-                       : make_float4(0.f, 0.f, 0.f, 0.f);
+          yb[tt] = make_float4(yk[0][16 * tt + l16], yk[1][16 * tt + l16], yk[2][16 * tt + l16], yk[3][16 * tt + l16]);
 #pragma unroll
         for (int et = 0; et < kET; ++et) {
           if (et >= ET) break;
-          const float4 a4 = yin ? make_float4(L.G[(4 * g4 + 0) * kGL + 16 * et + l16], L.G[(4 * g4 + 1) * kGL + 16 * et + l16],
-                                              L.G[(4 * g4 + 2) * kGL + 16 * et + l16], L.G[(4 * g4 + 3) * kGL + 16 * et + l16])
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 a4 = make_float4(gk[0][16 * et + l16], gk[1][16 * et + l16], gk[2][16 * et + l16],
+                                        gk[3][16 * et + l16]);
           f32x4_t acc[NT];
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt) acc[tt] = mma4(a4, yb[tt], f32x4_t{0.f, 0.f, 0.f, 0.f});
@@ -796,9 +816,9 @@ int launch_dims(dg_ctx* ctx, const DsglArgs& a, size_t lds, unsigned grid) {
 }
 
 template <int DIM>
-int plan_dims(dg_ctx* ctx, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows, size_t* lds,
-              unsigned* per_cu) {
-  *lds = dsgl_lds_bytes<DIM>(lcap, vcap, ncap, yrows);
+int plan_dims(dg_ctx* ctx, uint32_t lcap, uint32_t vcap, uint32_t ncap, uint32_t yrows, uint32_t gl,
+              size_t* lds, unsigned* per_cu) {
+  *lds = dsgl_lds_bytes<DIM>(lcap, vcap, ncap, yrows, gl);
   if (*lds > 160 * 1024)
     return dg_fail(ctx, DG_ERR_UNSUPPORTED,
                    "DSGL form: sentences of %u tokens at dim %d need %zu bytes of LDS per wavefront "
@@ -864,16 +884,21 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   DG_HIP(ctx, hipMemcpyAsync(&most, st->flags, sizeof(most), hipMemcpyDeviceToHost, ctx->stream));
   DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint32_t vcap = std::max<uint32_t>(1u, std::min<uint32_t>(lcap, most));
-  const uint32_t yrows = c.K + 2 <= 8 ? 8u : 16u;
+  const uint32_t yrows = c.K + 2, gl = dsgl_gl(c.window);
   size_t lds = 0;
   unsigned per_cu = 1;
   switch (c.dim) {
-    case 64: DG_CHECK(plan_dims<64>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
-    case 128: DG_CHECK(plan_dims<128>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
-    default: DG_CHECK(plan_dims<256>(ctx, lcap, vcap, ncap, yrows, &lds, &per_cu)); break;
+    case 64: DG_CHECK(plan_dims<64>(ctx, lcap, vcap, ncap, yrows, gl, &lds, &per_cu)); break;
+    case 128: DG_CHECK(plan_dims<128>(ctx, lcap, vcap, ncap, yrows, gl, &lds, &per_cu)); break;
+    default: DG_CHECK(plan_dims<256>(ctx, lcap, vcap, ncap, yrows, gl, &lds, &per_cu)); break;
   }
   const uint64_t resident = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
-  uint32_t workers = c.workers ? c.workers : (uint32_t)std::min<uint64_t>(resident, 1u << 20);
+  // default: one worker per resident wavefront, at most five per CU.  Every
+  // worker has the same share of the corpus, so the launch lasts as long as
+  // its slowest wavefront; at dim 128 on the 2.2M-vertex round six per CU
+  // (two SIMDs holding two) ran 33% slower than five, four 17% slower.
+  const uint64_t dflt = std::min<uint64_t>(std::min<uint64_t>(per_cu, 5u) * (uint64_t)ctx->num_cus, 1u << 20);
+  uint32_t workers = c.workers ? c.workers : (uint32_t)dflt;
   if ((uint64_t)workers > c.n_walks + c.n_tok) workers = (uint32_t)std::max<uint64_t>(1, (c.n_walks + c.n_tok) / 64);
   const unsigned grid = c.deterministic ? 1u : (unsigned)std::min<uint64_t>(workers, resident);
   const uint64_t stride = ((uint64_t)2 * lcap + 2 * (uint64_t)ncap) * c.dim;
@@ -905,6 +930,7 @@ int dg_dsgl_train(dg_ctx* ctx, const dg_dsgl_call& c, uint64_t* kept, double* ms
   a.vcap = vcap;
   a.ncap = ncap;
   a.yrows = yrows;
+  a.gl = gl;
   a.err = st->flags + 1;
   a.alpha0 = c.alpha0;
   a.total = (float)((uint64_t)c.iter * c.train_words + 1);
