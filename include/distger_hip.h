@@ -234,7 +234,8 @@ typedef struct {
                               sentence on the 32-row pool (0 = 16); does not change results */
   uint64_t seed;           /* Philox key for subsampling, window shrink, negatives */
   uint32_t workers;        /* DSGL form: compute threads the corpus is split over (0 = one per
-                              resident wavefront; dsgl.hpp:814-825 splits over its threads) */
+                              resident wavefront, at most five per CU; dsgl.hpp:814-825 splits
+                              over its threads) */
   uint32_t reserved;
 } dg_sgns_params;
 
