@@ -234,6 +234,24 @@ __device__ __forceinline__ float dsgl_sig(float f, const float* expt, int exact)
   return expt[(int)((f + (float)kDsglMaxExp) * (float)(kDsglExpTable / kDsglMaxExp / 2))];
 }
 
+// For lane i = i0 + lane of a list v[0, n): the first j < i with v[j] ==
+// v[i], or -1.  Earlier entries are compared 64 at a time from a register
+// (v_readlane), not by a loop of dependent LDS reads per lane.
+__device__ __forceinline__ int first_earlier(const int* v, int n, int i0, int i) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const int w = i < n ? v[i] : 0;
+  int first_of = -1;
+  for (int c0 = 0; c0 <= i0; c0 += 64) {
+    const int cj = c0 + lane < n ? v[c0 + lane] : 0;
+    const int kn = n - c0 < 64 ? n - c0 : 64;
+    for (int k = 0; k < kn; ++k) {
+      const int vk = __builtin_amdgcn_readlane(cj, k);
+      if (first_of < 0 && c0 + k < i && vk == w) first_of = c0 + k;
+    }
+  }
+  return i < n ? first_of : -1;
+}
+
 // Two wavefronts per SIMD up to dim 128 (at most 256 registers): the LDS of
 // a wavefront (27 KB at dim 128 on 22-token walks) fits six per CU.
 template <int DIM>
@@ -352,28 +370,33 @@ void dsgl_kernel(DsglArgs a) {
           len[s] = n;
           pos[s] = 0;
           // distinct words in order of first appearance, and each position's word
+          // (64 positions at a time: a position's first occurrence takes the
+          // next word index, a repeat the index of its first occurrence)
           int nwd = 0;
-          for (int i = 0; i < n; ++i) {
-            const int w = L.sen[s * lcap + i];
-            int found = -1;
-            for (int c0 = 0; c0 < nwd && found < 0; c0 += 64) {
-              const uint64_t m = __ballot(c0 + lane < nwd && L.word[s * vcap + c0 + lane] == w);
-              if (m) found = c0 + __builtin_ctzll(m);
+          const int* sen_s = L.sen + s * lcap;
+          for (int a0 = 0; a0 < n; a0 += 64) {
+            const int i = a0 + lane;
+            const int first_of = first_earlier(sen_s, n, a0, i);
+            const bool first = i < n && first_of < 0;
+            const uint64_t fm = __ballot(first);
+            if (first) {
+              int widx = nwd + (int)lanes_below(fm);
+              if (widx >= (int)vcap) {
+                // more distinct words than the launch planned for: flagged
+                // (the host fails the call), never written past the buffer
+                *a.err = 1u;
+                widx = 0;
+              } else {
+                L.word[s * vcap + widx] = sen_s[i];
+              }
+              L.slot[s * lcap + i] = widx;
             }
-            if (found < 0 && nwd == (int)vcap) {
-              // more distinct words than the launch planned for: flagged
-              // (the host fails the call), never written past the buffer
-              if (lane == 0) *a.err = 1u;
-              found = 0;
-            }
-            if (found < 0) {
-              found = nwd;
-              if (lane == 0) L.word[s * vcap + nwd] = w;
-              ++nwd;
-            }
-            if (lane == 0) L.slot[s * lcap + i] = found;
+            nwd += __popcll(fm);
+            wave_sync();
+            if (i < n && !first) L.slot[s * lcap + i] = L.slot[s * lcap + first_of];
             wave_sync();
           }
+          if (nwd > (int)vcap) nwd = (int)vcap;
           nwords[s] = nwd;
           // live rows in LDS, snapshots in scratch (dsgl.hpp:942-945)
           for (int u0 = 0; u0 < nwd; u0 += kRowBatch) {
@@ -422,15 +445,7 @@ void dsgl_kernel(DsglArgs a) {
         int nd_first = 0;
         for (int i0 = 0; i0 < nd; i0 += 64) {
           const int i = i0 + lane;
-          int first_of = -1;
-          if (i < nd) {
-            const int w = L.nword[i];
-            for (int j = 0; j < i; ++j)
-              if (L.nword[j] == w) {
-                first_of = j;
-                break;
-              }
-          }
+          const int first_of = first_earlier(L.nword, nd, i0, i);
           const uint64_t fm = __ballot(i < nd && first_of < 0);
           if (i < nd && first_of < 0) {
             const int k = nd_first + (int)lanes_below(fm);
